@@ -1,0 +1,195 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes bindings of the FP64 CPU oracle (liboracle.so) and of the
+reference's own CPU code built from its sources (_ref/libamr_ref.so, build container only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module, and
+only as the checker of the HIP path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import _native as N
+
+HERE = Path(__file__).resolve().parent
+ORACLE_PATH = HERE / "liboracle.so"
+REF_PATH = HERE / "_ref" / "libamr_ref.so"
+REFERENCE_SOURCES = Path("/root/reference/src")
+
+
+def build(reference: bool = True) -> None:
+    """Compile the oracle (and the reference shim when the reference checkout exists)."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "all"], check=True)
+    if reference and REFERENCE_SOURCES.is_dir():
+        subprocess.run(["make", "-s", "-C", str(HERE), "ref"], check=True)
+
+
+_oracle = None
+_ref = None
+
+
+def oracle_lib():
+    global _oracle
+    if _oracle is None:
+        if not ORACLE_PATH.exists():
+            build(reference=False)
+        lib = C.CDLL(str(ORACLE_PATH))
+        V, P = C.POINTER(N.amr_mesh_view), C.POINTER(N.amr_params)
+        lib.oracle_rhs.restype = C.c_int
+        lib.oracle_rhs.argtypes = [V, P, N.c_double_p, C.c_uint64, N.c_double_p, C.c_uint64]
+        lib.oracle_run.restype = C.c_int
+        lib.oracle_run.argtypes = [V, P, C.c_double, C.c_int64, N.c_double_p, C.c_uint64,
+                                   C.POINTER(N.amr_state_meta), C.c_char_p, C.c_size_t]
+        lib.oracle_redistribute.restype = C.c_int
+        lib.oracle_redistribute.argtypes = [V, N.c_double_p, C.c_uint64]
+        lib.oracle_monitor.restype = C.c_int
+        lib.oracle_monitor.argtypes = [V, P, N.c_double_p, C.c_uint64, N.c_double_p, N.c_double_p, C.c_double,
+                                       C.c_double, N.c_double_p]
+        lib.oracle_flux.restype = None
+        lib.oracle_flux.argtypes = [C.c_int, N.c_double_p, N.c_double_p, C.c_double, C.c_double, N.c_double_p]
+        lib.oracle_boundary_state.restype = C.c_int
+        lib.oracle_boundary_state.argtypes = [C.POINTER(N.amr_boundary), C.c_int, C.c_double, C.c_int,
+                                              N.c_double_p, N.c_double_p]
+        lib.oracle_sa_point.restype = None
+        lib.oracle_sa_point.argtypes = [C.c_double] * 4 + [N.c_double_p]
+        lib.oracle_viscosity.restype = C.c_double
+        lib.oracle_viscosity.argtypes = [C.POINTER(N.amr_transport), C.c_double]
+        lib.oracle_time_rhs.restype = C.c_double
+        lib.oracle_time_rhs.argtypes = [V, P, N.c_double_p, C.c_uint64, C.c_int]
+        _oracle = lib
+    return _oracle
+
+
+def ref_available() -> bool:
+    return REF_PATH.exists()
+
+
+def ref_lib():
+    """The reference's CPU code (shim over its own sources); None when not built."""
+    global _ref
+    if _ref is None:
+        if not REF_PATH.exists():
+            return None
+        lib = C.CDLL(str(REF_PATH))
+        N._bind_mesh_api(lib, "ref_mesh", "ref_mesh")
+        H, P = C.c_void_p, C.POINTER(N.amr_params)
+        lib.ref_rhs.restype = C.c_int
+        lib.ref_rhs.argtypes = [H, P, N.c_double_p, C.c_uint64, N.c_double_p, C.c_uint64]
+        lib.ref_run.restype = C.c_int
+        lib.ref_run.argtypes = [H, P, C.c_double, C.c_int64, N.c_double_p, C.c_uint64,
+                                C.POINTER(N.amr_state_meta), C.c_char_p, C.c_size_t]
+        lib.ref_redistribute.restype = C.c_int
+        lib.ref_redistribute.argtypes = [H, N.c_double_p, C.c_uint64]
+        lib.ref_wall_flux.restype = C.c_int
+        lib.ref_wall_flux.argtypes = [H, P, N.c_double_p, C.c_uint64, N.c_double_p, C.c_uint64]
+        _ref = lib
+    return _ref
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def rhs(mesh, params, u: np.ndarray) -> np.ndarray:
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    r = np.zeros_like(u)
+    view = mesh.view
+    if oracle_lib().oracle_rhs(C.byref(view), C.byref(params), N.dptr(u), u.size, N.dptr(r), r.size):
+        raise OracleError("oracle rhs failed (nonphysical state)")
+    return r
+
+
+def run(mesh, params, u: np.ndarray, tend: float, maxsteps: int = 50_000_000, time: float = 0.0, steps: int = 0,
+        rejected: int = 0):
+    """Advance to tend; returns (u, meta dict).  Raises on failure with the reference message."""
+    u = np.array(u, dtype=np.float64, copy=True)
+    meta = N.amr_state_meta(time, steps, rejected)
+    err = N.err_buffer()
+    view = mesh.view
+    status = oracle_lib().oracle_run(C.byref(view), C.byref(params), tend, maxsteps, N.dptr(u), u.size,
+                                     C.byref(meta), err, N.ERR_LEN)
+    out = dict(time=meta.time, steps=meta.steps, rejected=meta.rejected, status=status, error=err.value.decode())
+    return u, out
+
+
+def redistribute(mesh, u: np.ndarray) -> np.ndarray:
+    u = np.array(u, dtype=np.float64, copy=True)
+    view = mesh.view
+    if oracle_lib().oracle_redistribute(C.byref(view), N.dptr(u), u.size):
+        raise OracleError("oracle redistribute failed")
+    return u
+
+
+def monitor(mesh, params, u: np.ndarray, region, moment, mach: float, aoa: float) -> dict:
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros(11)
+    reg = np.ascontiguousarray(region, dtype=np.float64)
+    mom = np.ascontiguousarray(moment, dtype=np.float64)
+    view = mesh.view
+    if oracle_lib().oracle_monitor(C.byref(view), C.byref(params), N.dptr(u), u.size, N.dptr(reg), N.dptr(mom),
+                                   mach, aoa, N.dptr(out)):
+        raise OracleError("oracle monitor failed")
+    return dict(residual=out[:5].copy(), cl=out[5], cd=out[6], cm=out[7], fx=out[8], fy=out[9], mz=out[10])
+
+
+def flux(kind: int, left, right, gamma: float = 1.4, mco: float = 1.0) -> np.ndarray:
+    a = np.ascontiguousarray(left, dtype=np.float64)
+    b = np.ascontiguousarray(right, dtype=np.float64)
+    f = np.zeros(4)
+    oracle_lib().oracle_flux(kind, N.dptr(a), N.dptr(b), gamma, mco, N.dptr(f))
+    return f
+
+
+def boundary_state(boundary, legacy: int, gamma: float, side: int, interior):
+    wi = np.ascontiguousarray(interior, dtype=np.float64)
+    wb = np.zeros(4)
+    ok = oracle_lib().oracle_boundary_state(C.byref(boundary), legacy, gamma, side, N.dptr(wi), N.dptr(wb))
+    return bool(ok), wb
+
+
+def sa_point(working, molecular, distance, omega) -> dict:
+    out = np.zeros(7)
+    oracle_lib().oracle_sa_point(working, molecular, distance, omega, N.dptr(out))
+    return dict(zip(["fv1", "modified_vorticity", "fw", "source", "source_rate", "diffusion", "eddy"], out))
+
+
+def time_rhs(mesh, params, u: np.ndarray, repeats: int) -> float:
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    view = mesh.view
+    return oracle_lib().oracle_time_rhs(C.byref(view), C.byref(params), N.dptr(u), u.size, repeats)
+
+
+# ---- the reference's own CPU path (test-only; build container) ----
+def ref_rhs(ref_mesh, params, u: np.ndarray) -> np.ndarray:
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    r = np.zeros_like(u)
+    if ref_lib().ref_rhs(ref_mesh.handle, C.byref(params), N.dptr(u), u.size, N.dptr(r), r.size):
+        raise OracleError("reference rhs failed")
+    return r
+
+
+def ref_run(ref_mesh, params, u: np.ndarray, tend: float, maxsteps: int = 50_000_000, time: float = 0.0):
+    u = np.array(u, dtype=np.float64, copy=True)
+    meta = N.amr_state_meta(time, 0, 0)
+    err = N.err_buffer()
+    status = ref_lib().ref_run(ref_mesh.handle, C.byref(params), tend, maxsteps, N.dptr(u), u.size, C.byref(meta),
+                               err, N.ERR_LEN)
+    return u, dict(time=meta.time, steps=meta.steps, rejected=meta.rejected, status=status, error=err.value.decode())
+
+
+def ref_redistribute(ref_mesh, u: np.ndarray) -> np.ndarray:
+    u = np.array(u, dtype=np.float64, copy=True)
+    if ref_lib().ref_redistribute(ref_mesh.handle, N.dptr(u), u.size):
+        raise OracleError("reference redistribute failed")
+    return u
+
+
+def ref_wall_flux(ref_mesh, params, u: np.ndarray, n_walls: int) -> np.ndarray:
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros(4 * n_walls)
+    if ref_lib().ref_wall_flux(ref_mesh.handle, C.byref(params), N.dptr(u), u.size, N.dptr(out), out.size):
+        raise OracleError("reference viscous diagnostics failed")
+    return out

@@ -1,0 +1,228 @@
+// ref_shim.cpp -- extern "C" handles over the reference's own CPU code, used only by tests to
+// pin this project's mesher and oracle against the real reference in the build container.
+//
+// Compiled (see oracle/Makefile) together with the reference sources where they lie; the
+// reference is never copied into this repository.  Entry points mirror amr_mesh_* of
+// include/amr_hip.h with a ref_ prefix, plus the reference's CPU time step:
+//   ref_rhs          static_amr2d_rhs        (src/static_amr2d.cpp:369-377)
+//   ref_run          static_amr2d_run        (src/static_amr2d.cpp:384-432)
+//   ref_redistribute static_amr2d_redistribute (src/static_amr2d.cpp:379-382)
+//   ref_wall_flux    static_amr2d_viscous_diagnostics (src/static_amr2d_viscous.cpp:137-142)
+#include <cstring>
+#include <string>
+
+#include "amr_hip.h"
+#include "static_amr2d.h"
+#include "static_amr2d_sa.h"
+
+using namespace cfd;
+
+struct ref_mesh {
+  StaticAmrMesh2D mesh;
+};
+
+static int ref_report(char* err, size_t len, const std::string& message) {
+  if (err && len) {
+    const size_t k = message.size() < len - 1 ? message.size() : len - 1;
+    std::memcpy(err, message.data(), k);
+    err[k] = 0;
+  }
+  return 1;
+}
+
+static Params2D ref_params(const amr_params& c) {
+  Params2D p = params2d_default();
+  p.gamma = c.gamma;
+  p.cfl = c.cfl;
+  p.flux = c.flux;
+  p.limiter = c.limiter;
+  p.ausm_mco = c.ausm_mco;
+  for (int k = 0; k < 4; ++k) p.bc[k] = c.bc[k];
+  p.dt_fixed = c.dt_fixed;
+  p.time_integrator = c.time_integrator;
+  return p;
+}
+
+extern "C" {
+
+int ref_mesh_build(const amr_build_options* options, const double* px, const double* py, const int32_t* patches,
+                   int32_t n_points, ref_mesh** out, char* err, size_t err_len) {
+  if (!out || !options) return 1;
+  *out = nullptr;
+  StaticAmrOptions2D o;
+  o.nx = options->nx;
+  o.ny = options->ny;
+  o.levels = options->levels;
+  o.buffer = options->buffer;
+  o.xmin = options->xmin;
+  o.xmax = options->xmax;
+  o.ymin = options->ymin;
+  o.ymax = options->ymax;
+  o.max_cells = options->max_cells;
+  o.agglomerate_small = options->agglomerate_small != 0;
+  for (int32_t b = 0; b < options->n_boxes; ++b)
+    o.boxes.push_back({options->boxes[b].x0, options->boxes[b].x1, options->boxes[b].y0, options->boxes[b].y1,
+                       options->boxes[b].level});
+  std::vector<double> x(px, px + n_points), y(py, py + n_points);
+  std::vector<int> pt;
+  if (patches) pt.assign(patches, patches + n_points);
+  ref_mesh* m = new ref_mesh;
+  std::string error;
+  if (static_amr2d_build(o, x, y, pt, &m->mesh, &error)) {
+    delete m;
+    return ref_report(err, err_len, error);
+  }
+  *out = m;
+  return 0;
+}
+
+int ref_mesh_set_physics(ref_mesh* m, const amr_boundary* boundaries4, const amr_transport* transport) {
+  if (!m) return 1;
+  if (boundaries4) std::memcpy(m->mesh.boundaries, boundaries4, 4 * sizeof(amr_boundary));
+  if (transport) std::memcpy(&m->mesh.transport, transport, sizeof(amr_transport));
+  return 0;
+}
+
+int ref_mesh_prepare_viscous(ref_mesh* m, const amr_params* params, char* err, size_t err_len) {
+  if (!m || !params) return 1;
+  std::string error;
+  if (static_amr2d_prepare_viscous(&m->mesh, ref_params(*params), &error)) return ref_report(err, err_len, error);
+  return 0;
+}
+
+int ref_mesh_get_view(const ref_mesh* mm, amr_mesh_view* v) {
+  if (!mm || !v) return 1;
+  const StaticAmrMesh2D& m = mm->mesh;
+  std::memset(v, 0, sizeof(*v));
+  v->n_cells = m.cells.size();
+  v->n_faces = m.faces.size();
+  v->n_walls = m.walls.size();
+  v->cells = reinterpret_cast<const amr_cell*>(m.cells.data());
+  v->faces = reinterpret_cast<const amr_face*>(m.faces.data());
+  v->walls = reinterpret_cast<const amr_wall*>(m.walls.data());
+  v->face_offsets = reinterpret_cast<const uint64_t*>(m.face_offsets.data());
+  v->cell_faces = reinterpret_cast<const uint64_t*>(m.cell_faces.data());
+  v->group_offsets = reinterpret_cast<const uint64_t*>(m.group_offsets.data());
+  v->group_members = reinterpret_cast<const uint64_t*>(m.group_members.data());
+  v->member_offsets = reinterpret_cast<const uint64_t*>(m.member_offsets.data());
+  v->member_groups = reinterpret_cast<const uint64_t*>(m.member_groups.data());
+  v->group_volume = m.group_volume.data();
+  v->small_cells = m.small_cells;
+  v->agglomerate_small = m.options.agglomerate_small ? 1 : 0;
+  std::memcpy(v->boundaries, m.boundaries, sizeof(v->boundaries));
+  std::memcpy(&v->transport, &m.transport, sizeof(v->transport));
+  if (!m.viscous.offsets.empty()) {
+    v->visc_offsets = reinterpret_cast<const uint64_t*>(m.viscous.offsets.data());
+    v->visc_terms = reinterpret_cast<const amr_grad_term*>(m.viscous.terms.data());
+  }
+  if (!m.viscous.wall_samples.empty()) v->wall_samples = reinterpret_cast<const amr_wall_sample*>(m.viscous.wall_samples.data());
+  if (!m.viscous.face_samples.empty()) v->face_samples = reinterpret_cast<const amr_wall_sample*>(m.viscous.face_samples.data());
+  if (!m.viscous.face_geometry.empty())
+    v->face_geometry = reinterpret_cast<const amr_face_geometry*>(m.viscous.face_geometry.data());
+  if (!m.viscous.wall_distance.empty()) v->wall_distance = m.viscous.wall_distance.data();
+  return 0;
+}
+
+int ref_mesh_get_info(const ref_mesh* mm, amr_mesh_info* out) {
+  if (!mm || !out) return 1;
+  const StaticAmrMesh2D& s = mm->mesh;
+  out->n_cells = s.cells.size();
+  out->n_faces = s.faces.size();
+  out->n_walls = s.walls.size();
+  out->n_cell_faces = s.cell_faces.size();
+  out->n_group_members = s.group_members.size();
+  out->n_visc_terms = s.viscous.terms.size();
+  out->background_leaves = s.background_leaves;
+  out->split_leaves = s.split_leaves;
+  out->coarse_fine_faces = s.coarse_fine_faces;
+  out->small_cells = s.small_cells;
+  out->n_partition = s.partition.size();
+  out->n_leaf_components = s.leaf_components.size();
+  out->n_agglomeration = s.agglomeration.size();
+  return 0;
+}
+
+int ref_mesh_initialize(const ref_mesh* m, const amr_params* params, const double* primitive4, double* u,
+                        uint64_t u_len) {
+  if (!m || !params) return 1;
+  StaticAmrState2D s;
+  if (static_amr2d_initialize(m->mesh, ref_params(*params), {primitive4[0], primitive4[1], primitive4[2], primitive4[3]}, &s))
+    return 1;
+  if (s.u.size() != u_len) return 1;
+  std::memcpy(u, s.u.data(), u_len * sizeof(double));
+  return 0;
+}
+
+int ref_mesh_copy_partition(const ref_mesh* m, int32_t* out, uint64_t rows) {
+  if (!m || rows != m->mesh.partition.size()) return 1;
+  for (size_t k = 0; k < rows; ++k)
+    for (int j = 0; j < 5; ++j) out[5 * k + j] = m->mesh.partition[k][j];
+  return 0;
+}
+
+int ref_mesh_copy_leaf_components(const ref_mesh* m, uint64_t* out, uint64_t rows) {
+  if (!m || rows != m->mesh.leaf_components.size()) return 1;
+  for (size_t k = 0; k < rows; ++k) {
+    out[2 * k] = m->mesh.leaf_components[k][0];
+    out[2 * k + 1] = m->mesh.leaf_components[k][1];
+  }
+  return 0;
+}
+
+int ref_mesh_copy_agglomeration(const ref_mesh* m, double* out, uint64_t rows) {
+  if (!m || rows != m->mesh.agglomeration.size()) return 1;
+  for (size_t k = 0; k < rows; ++k)
+    for (int j = 0; j < 9; ++j) out[9 * k + j] = m->mesh.agglomeration[k][j];
+  return 0;
+}
+
+void ref_mesh_destroy(ref_mesh* m) { delete m; }
+
+int ref_rhs(const ref_mesh* m, const amr_params* params, const double* u, uint64_t u_len, double* r, uint64_t r_len) {
+  if (!m || !params) return 1;
+  std::vector<double> state(u, u + u_len), out;
+  if (static_amr2d_rhs(m->mesh, ref_params(*params), state, &out)) return 1;
+  if (out.size() != r_len) return 2;
+  std::memcpy(r, out.data(), r_len * sizeof(double));
+  return 0;
+}
+
+int ref_run(const ref_mesh* m, const amr_params* params, double tend, int64_t maxsteps, double* u, uint64_t u_len,
+            amr_state_meta* meta, char* err, size_t err_len) {
+  if (!m || !params || !meta) return 1;
+  StaticAmrState2D s;
+  s.u.assign(u, u + u_len);
+  s.time = meta->time;
+  s.steps = meta->steps;
+  s.rejected = meta->rejected;
+  std::string error;
+  const int status = static_amr2d_run(m->mesh, ref_params(*params), tend, maxsteps, &s, &error);
+  std::memcpy(u, s.u.data(), u_len * sizeof(double));
+  meta->time = s.time;
+  meta->steps = s.steps;
+  meta->rejected = s.rejected;
+  if (status) return ref_report(err, err_len, error);
+  return 0;
+}
+
+int ref_redistribute(const ref_mesh* m, double* u, uint64_t u_len) {
+  if (!m) return 1;
+  std::vector<double> state(u, u + u_len);
+  if (static_amr2d_redistribute(m->mesh, &state)) return 1;
+  std::memcpy(u, state.data(), u_len * sizeof(double));
+  return 0;
+}
+
+// Viscous wall flux (4 * n_walls, already times fragment length) at a conserved state.
+int ref_wall_flux(const ref_mesh* m, const amr_params* params, const double* u, uint64_t u_len, double* wall_flux,
+                  uint64_t len) {
+  if (!m || !params) return 1;
+  std::vector<double> state(u, u + u_len);
+  AmrViscousFields2D fields;
+  if (static_amr2d_viscous_diagnostics(m->mesh, ref_params(*params), state, &fields)) return 1;
+  if (fields.wall_flux.size() != len) return 2;
+  std::memcpy(wall_flux, fields.wall_flux.data(), len * sizeof(double));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,6 @@
+"""MI355X-native (gfx950) explicit cut-cell finite-volume airfoil time step.
+
+Host mesher + HIP time step behind the reference's static_amr2d_gpu.h backend API, with the C
+ABI of include/amr_hip.h underneath; see DESIGN.md for the layout and kernels.
+"""
+__version__ = "0.1.0"
