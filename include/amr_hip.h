@@ -175,6 +175,11 @@ int amr_hip_rhs(amr_session* s, const double* u, uint64_t u_len, double* r, char
                 size_t err_len);
 /* Number of visible HIP devices (0 when no device or no driver). */
 int amr_hip_device_count(void);
+/* Host-only check of the device layout of n meshes (views[n], params[n]) at a launch tile of
+ * 64, 128 or 256: every index a kernel dereferences stays inside its array and its mesh.
+ * 0 consistent, 1 error.  No device is touched. */
+int amr_hip_check_layout(const amr_mesh_view* views, const amr_params* params, int32_t n, int32_t tile, char* err,
+                         size_t err_len);
 
 #ifdef __cplusplus
 }

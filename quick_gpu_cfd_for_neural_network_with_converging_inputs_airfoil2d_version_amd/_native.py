@@ -210,6 +210,9 @@ def load_library(path: os.PathLike | None = None):
     lib.amr_hip_rhs.argtypes = [H, c_double_p, C.c_uint64, c_double_p, C.c_char_p, C.c_size_t]
     lib.amr_hip_device_count.restype = C.c_int
     lib.amr_hip_device_count.argtypes = []
+    lib.amr_hip_check_layout.restype = C.c_int
+    lib.amr_hip_check_layout.argtypes = [C.POINTER(amr_mesh_view), C.POINTER(amr_params), C.c_int32, C.c_int32,
+                                         C.c_char_p, C.c_size_t]
     if path is None:
         _LIB = lib
     return lib

@@ -1,0 +1,70 @@
+"""Build libamr_hip.so in-tree: host C++ (g++) + HIP kernels for gfx950 (hipcc), one shared library.
+
+The mesher sources are compiled with g++ -O3 -ffp-contract=off so that the mesh arrays round
+exactly as the reference CPU build does; the .hip sources are compiled by hipcc for gfx950
+with packed FP32 instructions disabled (-Xclang -target-feature -Xclang -packed-fp32-ops).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+PACKAGE = Path(__file__).resolve().parent
+ROOT = PACKAGE.parent
+CSRC = PACKAGE / "csrc"
+OUT = PACKAGE / "libamr_hip.so"
+OBJ = ROOT / "_build" / "obj"
+
+HOST_SOURCES = ["host/cutcell.cpp", "host/mesher.cpp", "host/viscous_prep.cpp", "host/mesh_capi.cpp",
+                "host/cfd_api.cpp"]
+HIP_SOURCES = ["hip/amr_kernels.hip", "hip/amr_device.hip", "hip/amr_session.hip"]
+INCLUDES = ["-I", str(ROOT / "include"), "-I", str(CSRC / "host"), "-I", str(CSRC / "hip")]
+ARCH = os.environ.get("HIPAMR_ARCH", "gfx950")
+
+
+def _hipcc() -> str:
+    for candidate in (shutil.which("hipcc"), "/opt/rocm/bin/hipcc"):
+        if candidate and Path(candidate).exists():
+            return candidate
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    done = subprocess.run(cmd, capture_output=True, text=True)
+    if done.returncode != 0:
+        raise RuntimeError(f"command failed ({done.returncode}): {' '.join(cmd)}\n{done.stdout}\n{done.stderr}")
+    return done
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> Path:
+    """Compile every source and link PACKAGE/libamr_hip.so (always rebuilds)."""
+    OBJ.mkdir(parents=True, exist_ok=True)
+    hipcc = _hipcc()
+    commands, objects = [], []
+    for rel in HOST_SOURCES:
+        obj = OBJ / (Path(rel).stem + ".o")
+        commands.append(["g++", "-std=c++17", "-O3", "-DNDEBUG", "-fPIC", "-ffp-contract=off", "-Wall",
+                         *INCLUDES, "-c", str(CSRC / rel), "-o", str(obj)])
+        objects.append(obj)
+    for rel in HIP_SOURCES:
+        obj = OBJ / (Path(rel).stem + ".hip.o")
+        commands.append([hipcc, "-x", "hip", f"--offload-arch={ARCH}", "-std=c++17", "-O3", "-DNDEBUG", "-fPIC",
+                         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", *INCLUDES,
+                         "-c", str(CSRC / rel), "-o", str(obj)])
+        objects.append(obj)
+    with ThreadPoolExecutor(max_workers=jobs or min(8, len(commands))) as pool:
+        list(pool.map(lambda c: _run(c, verbose), commands))
+    tmp = OUT.with_suffix(".so.tmp")
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objects), "-o", str(tmp)], verbose)
+    tmp.replace(OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

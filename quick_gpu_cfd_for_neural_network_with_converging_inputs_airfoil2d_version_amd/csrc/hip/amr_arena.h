@@ -1,0 +1,138 @@
+// amr_arena.h -- device layout of one or many meshes advanced in lockstep.
+//
+// Every mesh of an arena owns block-aligned ranges of a global cell, face and wall index
+// space (each range padded to the launch tile, so a thread block never spans two meshes).
+// All index arrays hold global ids.  Per-element data are structure-of-arrays planes of one
+// width per array: cell planes have stride nc, face planes nf, wall planes nw.
+//
+// Per mesh, an on-device Control drives the SSP-RK3 trial state machine of the reference
+// (src/static_amr2d_cuda.cu:17-21,51-59,170-187,212-216): time, target, dt, step and
+// rejection counters, the admissibility flag and fatal codes 2 (nonphysical base / bad at dt
+// choice), 3 (maxsteps), 4 (dt underflow), 5 (21 rejections).  The accepted state alternates
+// between two buffers (Control::cur), so no commit copy is needed.
+#ifndef HIPAMR_AMR_ARENA_H
+#define HIPAMR_AMR_ARENA_H
+
+#include <cstdint>
+
+#include "amr_physics.h"
+
+namespace hipamr {
+
+struct Control {
+  double time, target, dt;   // dt: step size of the last trial (halved on rejection)
+  double trial_dt;           // step size of the running trial
+  unsigned long long dt_min; // bit pattern of the minimum local dt of the base pass
+  long long steps, rejected, maxsteps;
+  int bad, fatal, retrying, attempt, finished, active;
+  int cur;                   // buffer holding the accepted state: U[cur]
+  int w_valid;               // W/PHI hold primitives of U[cur]
+  int r0_valid;              // R0 holds L(U[cur]) (set after a rejection)
+  int pad0;
+};
+static_assert(sizeof(Control) == 104, "Control layout");
+
+enum : int { VISC_VOLUME = 0, VISC_SRD_RATE = 1, VISC_SRD_GROUP = 2 };
+
+template <class T>
+struct MeshDesc {
+  int cell_begin, n_cells, face_begin, n_faces, wall_begin, n_walls;
+  int first_cell_block;  // block index (in cell launches) of the mesh's first tile
+  int nv;                // 4, or 5 with SA-neg
+  int laminar, sa, flux, small, visc_mode, agglomerated;
+  double dt_fixed;
+  T gamma, cfl, mco;
+  Transport<T> transport;
+  Side<T> side[4];
+};
+
+// Device pointers of an arena (passed by value to every kernel).
+template <class T>
+struct ArenaPtrs {
+  int n_meshes, nc, nf, nw, tile;
+  int n_cell_blocks, n_face_blocks, n_wall_blocks;
+  int any_laminar, any_sa, any_small, any_visc_group;
+  const MeshDesc<T>* desc;
+  Control* ctrl;
+  const int* cell_block_mesh;  // n_cell_blocks
+  const int* face_block_mesh;  // n_face_blocks + n_wall_blocks (faces first, then walls)
+  // cells
+  const T* vol;
+  const T* inv_vol;
+  const T* dt_area;
+  const T* wall_dist;
+  const int* csr_off;    // nc + 1
+  const int* csr_face;   // face id q (cell is the face's left) or ~q (right)
+  const int* wall_first; // first wall of the cell
+  const int* wall_cnt;
+  const int* term_off;   // nc + 1
+  const int* term_src;   // >= 0 cell, -1 body wall, -2 - side: no-slip domain side
+  const T* term_wx;
+  const T* term_wy;
+  // SRD neighbourhoods: group g = cell g
+  const int* grp_off;    // nc + 1
+  const int* grp_mem;
+  const T* grp_w;        // V_k / (N_k Vhat_g)
+  const int* mem_off;    // nc + 1
+  const int* mem_grp;
+  const T* mem_inv;      // 1 / N_k
+  // faces
+  const int* face_l;
+  const int* face_r;
+  const int* face_info;  // bit 0 axis, bits 1..3 side + 1 (0 interior)
+  const T* face_len;
+  const T* face_dx;
+  const T* face_dy;
+  const T* face_ex;
+  const T* face_ey;
+  const T* face_dist;
+  // walls
+  const int* wall_owner;
+  const int* wall_donor;
+  const T* wall_nx;
+  const T* wall_ny;
+  const T* wall_len;
+  const T* wall_ddist;
+  // state and work arrays
+  T* U[2];
+  T* U2;
+  T* R0;
+  T* W;       // rho, u, v, p
+  T* PHI;     // nutilde = rho nutilde / rho
+  T* GRAD;    // ux, uy, vx, vy
+  T* MU;
+  T* EDDY;
+  T* SGRAD;   // nutilde_x, nutilde_y
+  T* SDIFF;
+  T* SSRC;
+  T* SRATE;
+  T* FLUX;    // 4 planes, times face length, viscous part subtracted
+  T* SPEED;
+  T* FCOEF;
+  T* SFLUX;
+  T* SFCOEF;
+  T* WFLUX;   // 2 planes (x, y); mass and energy wall fluxes vanish
+  T* WCOEF;
+  T* SWFLUX;
+  T* SWCOEF;
+  T* QHAT;    // SRD group averages, 4 planes
+  T* CELL_DT; // deferred local dt (SRD-grouped viscous limit only)
+  T* DRATE;
+  T* DGROUP;
+};
+
+// Launch one full trial (one SSP-RK3 step attempt) of every mesh on `stream`.
+// Returns the number of kernel launches it issued.
+template <class T>
+int launch_trial(const ArenaPtrs<T>& a, hipStream_t stream);
+// Base pass only: primitives of U[cur] and R0 = L(U[cur]) (the residual evaluation).
+template <class T>
+int launch_residual(const ArenaPtrs<T>& a, hipStream_t stream);
+// Set the target time and step cap of every mesh and start a trial.  targets/maxsteps are
+// device arrays of n_meshes entries, or null to use the scalar values for all meshes.
+void launch_target(Control* ctrl, int n_meshes, double target, long long maxsteps, const double* targets,
+                   const long long* maxsteps_each, hipStream_t stream);
+
+}  // namespace hipamr
+
+#endif
