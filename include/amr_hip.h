@@ -169,10 +169,173 @@ int amr_hip_download(amr_session* s, double* u, uint64_t u_len, amr_state_meta* 
 /* static_amr_gpu_stats / static_amr_gpu_destroy (src/static_amr2d_gpu.h:23-24). */
 void amr_hip_stats(const amr_session* s, amr_gpu_stats* out);
 void amr_hip_destroy(amr_session* s);
+/* Time and counters of the last accepted state the host has seen (no device access). */
+int amr_hip_state(const amr_session* s, amr_state_meta* state);
 /* One residual evaluation at a host state, as static_amr2d_rhs (src/static_amr2d.cpp:369-377):
  * r gets nv * n_cells values.  Uses the session's mesh; leaves the session state untouched. */
 int amr_hip_rhs(amr_session* s, const double* u, uint64_t u_len, double* r, char* err,
                 size_t err_len);
+/* Steady monitor at the accepted state (src/quick_airfoil.cpp:75-85), reduced in FP64 on the
+ * device: out[0..4] raw residual norms R_v = sum_region V |L_v(u)| / sum_region V (v < nv; the
+ * SA norm divided by nutilde_inf; 0 beyond nv), out[5..7] CL, CD, CM_z (q_inf = M^2/2, lift and
+ * drag axes rotated by the angle of attack), out[8..10] fx, fy, mz (pressure minus viscous
+ * wall shear, moment about `moment`).  region = {xmin, xmax, ymin, ymax} on cell centroids. */
+int amr_hip_monitor(amr_session* s, const double* region4, const double* moment2, double mach, double aoa_degrees,
+                    double* out11, char* err, size_t err_len);
+
+/* Steady / time-accurate driver over one session (the loop of src/quick_airfoil.cpp:66-95):
+ * advance to 0.01, then every `period`; after each target the monitor above; converged when
+ * t >= min_time, max raw residual <= raw_tol and the relative changes of CL, CD, CM_z stay
+ * <= load_tol for `hold` consecutive checks. */
+typedef struct amr_steady_options {
+  double mach, aoa_degrees;
+  double moment[2];
+  double region[4];
+  double raw_tol, load_tol, min_time, period;
+  int32_t hold;
+  int32_t mode; /* 0 steady, 1 time (run to stop_time), 2 smoke (one check at 0.01) */
+  double stop_time, budget_seconds;
+} amr_steady_options;
+typedef struct amr_steady_check {
+  double time;
+  int64_t steps, rejected;
+  double elapsed;
+  double residual[5];
+  double cl, cd, cm, dcl, dcd, dcm;
+  int32_t hold, pad;
+} amr_steady_check;
+enum {
+  AMR_STOP_CONVERGED = 0, AMR_STOP_BUDGET = 1, AMR_STOP_TIME_CAP = 2, AMR_STOP_FAILURE = 3, AMR_STOP_SHORT = 4
+};
+typedef struct amr_steady_result {
+  int32_t converged, failed, reason, hold, checks, pad;
+  double time;
+  int64_t steps, rejected;
+  double residual[5];
+  double cl, cd, cm;
+  double elapsed_seconds;
+} amr_steady_result;
+/* history (may be NULL) receives up to history_capacity checks; result->checks counts all. */
+int amr_hip_solve(amr_session* s, const amr_steady_options* options, amr_steady_result* result,
+                  amr_steady_check* history, int32_t history_capacity, char* err, size_t err_len);
+
+/* Run exactly `trials` step trials toward tend (graph replays, no intermediate poll), then
+ * read the control record.  The timing primitive of the benchmark.  0 ok, 1 fatal. */
+int amr_hip_run(amr_session* s, double tend, int32_t trials, amr_state_meta* state, char* err, size_t err_len);
+
+/* Per-kernel device time: `trials` trials with direct launches and a hipEvent pair around
+ * every kernel on the session stream, aggregated by kernel name (the solution advances). */
+typedef struct amr_kernel_time { char name[32]; double total_ms; int64_t launches; } amr_kernel_time;
+int amr_hip_profile(amr_session* s, double tend, int32_t trials, amr_kernel_time* out, int32_t capacity,
+                    int32_t* count, char* err, size_t err_len);
+
+/* STREAM-copy probe: `repeats` launches of a plain device copy of `bytes` (read + write);
+ * gbps = 2 * bytes / time.  The achievable-bandwidth reference and the HBM counter calibration. */
+int amr_hip_copy_bandwidth(uint64_t bytes, int32_t repeats, double* gbps, double* ms_per_copy, char* err,
+                           size_t err_len);
+
+/* ---- batch: many meshes of uneven size advanced together in one device arena -----------
+ * No counterpart in the reference, which advances one mesh per process
+ * (src/quick_airfoil.cpp:46-95).  The arena is a fixed set of slots; each slot owns
+ * tile-aligned ranges of the cell, face and wall index spaces and of the ragged tables, sized
+ * by its capacity.  Every slot has its own time, target, step size, counters and fatal code,
+ * so a slot's trajectory does not depend on which other meshes share the arena (same
+ * precision and tile give bitwise the same result as running it alone).  One hipGraph holds
+ * one trial of every slot; finished, failed and empty slots are no-ops inside it. */
+typedef struct amr_batch amr_batch;
+typedef struct amr_slot_capacity {
+  int32_t cells;   /* tile multiple, at least n_cells + 1 */
+  int32_t faces, walls; /* tile multiples */
+  int32_t csr, terms, groups, members; /* ragged table entries */
+  int32_t pad;
+} amr_slot_capacity;
+typedef struct amr_batch_features { int32_t laminar, sa, small, visc_group; } amr_batch_features;
+typedef struct amr_batch_options {
+  int32_t precision;    /* 64 or 32 */
+  int32_t tile;         /* launch tile: 64, 128 or 256 threads per block */
+  int32_t graph;        /* 1: replay a captured hipGraph of one trial of every slot */
+  int32_t batch_trials; /* trials per host poll, 1..1024 */
+} amr_batch_options;
+typedef struct amr_slot_state {
+  double time, target;
+  int64_t steps, rejected;
+  int32_t n_cells;  /* 0: empty slot */
+  int32_t finished; /* time >= target */
+  int32_t fatal;    /* 0, or the fatal code of the reference (2 nonphysical, 3 maxsteps, 4 dt underflow,
+                       5 rejections); the slot keeps its last accepted state */
+  int32_t pad;
+} amr_slot_state;
+
+/* Smallest slot that holds the mesh at this tile, and the kernels it needs. */
+int amr_batch_capacity(const amr_mesh_view* mesh, int32_t tile, amr_slot_capacity* capacity,
+                       amr_batch_features* features);
+/* Device bytes of an arena of n slots (its single allocation), for HBM budget admission. */
+uint64_t amr_batch_bytes(const amr_slot_capacity* capacities, int32_t n, const amr_batch_features* features,
+                         int32_t precision, int32_t tile);
+/* One slot per mesh, each sized to its mesh and admitted with state u[m] (nv * n_cells doubles)
+ * and initial[m] (NULL: t = 0, counters 0). */
+int amr_batch_create(const amr_mesh_view* meshes, const amr_params* params, const double* const* u,
+                     const amr_state_meta* initial, int32_t n, const amr_batch_options* options, amr_batch** out,
+                     char* err, size_t err_len);
+/* n empty slots of the given capacities; the kernel set is fixed by `features` (a slot accepts
+ * meshes that need no more: same SA-neg flag, laminar/small/grouped only if enabled). */
+int amr_batch_create_slots(const amr_slot_capacity* capacities, int32_t n, const amr_batch_features* features,
+                           const amr_batch_options* options, amr_batch** out, char* err, size_t err_len);
+/* Place a mesh into an empty or occupied slot (the previous mesh is dropped); the slot starts
+ * finished at initial->time until its next target. */
+int amr_batch_admit(amr_batch* b, int32_t slot, const amr_mesh_view* mesh, const amr_params* params,
+                    const double* u, uint64_t u_len, const amr_state_meta* initial, char* err, size_t err_len);
+/* Empty a slot. */
+int amr_batch_release(amr_batch* b, int32_t slot, char* err, size_t err_len);
+/* Target time and step cap of every slot (n entries each; maxsteps NULL: no cap).  Empty and
+ * failed slots ignore theirs. */
+int amr_batch_set_targets(amr_batch* b, const double* targets, const int64_t* maxsteps, char* err,
+                          size_t err_len);
+/* Replay trials until every slot is finished or failed (until = 0) or until at least one slot
+ * that was running at the call has finished or failed (until = 1), polling the control records
+ * every batch_trials trials.  states (may be NULL) receives every slot.  0 done, 2 wall budget
+ * expired (0 = no budget), 1 device error. */
+int amr_batch_advance(amr_batch* b, int32_t until, double wall_budget_seconds, amr_slot_state* states, char* err,
+                      size_t err_len);
+/* Exactly `trials` trials, then one poll. */
+int amr_batch_run(amr_batch* b, int32_t trials, amr_slot_state* states, char* err, size_t err_len);
+/* Slot states of the last poll (no device access). */
+int amr_batch_slots(const amr_batch* b, amr_slot_state* states);
+int32_t amr_batch_size(const amr_batch* b);
+/* Accepted state of one slot (nv * n_cells doubles). */
+int amr_batch_download(amr_batch* b, int32_t slot, double* u, uint64_t u_len, amr_state_meta* state, char* err,
+                       size_t err_len);
+/* Steady monitor of amr_hip_monitor for every slot with mask[m] != 0, in one residual pass:
+ * frames[6m..6m+5] = region xmin, xmax, ymin, ymax and moment point; out[11m..11m+10] as
+ * amr_hip_monitor; status[m] = 0 ok, 1 nonphysical state at the monitor, 2 empty region,
+ * 3 not evaluated (mask 0, empty or failed slot). */
+int amr_batch_monitor(amr_batch* b, const int32_t* mask, const double* frames, const double* mach,
+                      const double* aoa_degrees, double* out, int32_t* status, char* err, size_t err_len);
+/* Per-kernel device time over `trials` directly launched trials (as amr_hip_profile). */
+int amr_batch_profile(amr_batch* b, int32_t trials, amr_kernel_time* out, int32_t capacity, int32_t* count,
+                      char* err, size_t err_len);
+void amr_batch_stats(const amr_batch* b, amr_gpu_stats* out);
+void amr_batch_destroy(amr_batch* b);
+
+/* Steady driver over a batch: every slot runs its own check loop (the loop of amr_hip_solve,
+ * src/quick_airfoil.cpp:66-95) with its own options; slots finish independently. */
+typedef struct amr_batch_driver amr_batch_driver;
+int amr_driver_create(amr_batch* b, amr_batch_driver** out, char* err, size_t err_len);
+/* Start a steady run on an admitted slot from its current state (its wall budget starts now). */
+int amr_driver_begin(amr_batch_driver* d, int32_t slot, const amr_steady_options* options, char* err,
+                     size_t err_len);
+/* Advance the running slots until at least one run ends or wall_budget_seconds (0: none)
+ * passes.  Ended runs: their slot ids in done[0..*n_done-1], results[slot] filled.  0 ok,
+ * 2 budget expired with no run ended, 1 device error. */
+int amr_driver_poll(amr_batch_driver* d, double wall_budget_seconds, amr_steady_result* results, int32_t* done,
+                    int32_t* n_done, char* err, size_t err_len);
+/* Number of runs in progress. */
+int32_t amr_driver_running(const amr_batch_driver* d);
+void amr_driver_destroy(amr_batch_driver* d);
+/* Convenience: a steady run on every occupied slot (options[m] per slot), polled to the end. */
+int amr_batch_solve(amr_batch* b, const amr_steady_options* options, amr_steady_result* results, char* err,
+                    size_t err_len);
+
 /* Number of visible HIP devices (0 when no device or no driver). */
 int amr_hip_device_count(void);
 /* Host-only check of the device layout of n meshes (views[n], params[n]) at a launch tile of

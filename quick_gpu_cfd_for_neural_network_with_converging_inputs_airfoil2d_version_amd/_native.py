@@ -123,6 +123,53 @@ class amr_gpu_stats(C.Structure):
                 ("event_batches", C.c_uint64), ("graph_nodes_per_trial", C.c_uint64)]
 
 
+class amr_steady_options(C.Structure):
+    _fields_ = [("mach", C.c_double), ("aoa_degrees", C.c_double), ("moment", C.c_double * 2),
+                ("region", C.c_double * 4), ("raw_tol", C.c_double), ("load_tol", C.c_double),
+                ("min_time", C.c_double), ("period", C.c_double), ("hold", C.c_int32), ("mode", C.c_int32),
+                ("stop_time", C.c_double), ("budget_seconds", C.c_double)]
+
+
+class amr_steady_check(C.Structure):
+    _fields_ = [("time", C.c_double), ("steps", C.c_int64), ("rejected", C.c_int64), ("elapsed", C.c_double),
+                ("residual", C.c_double * 5), ("cl", C.c_double), ("cd", C.c_double), ("cm", C.c_double),
+                ("dcl", C.c_double), ("dcd", C.c_double), ("dcm", C.c_double), ("hold", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+class amr_steady_result(C.Structure):
+    _fields_ = [("converged", C.c_int32), ("failed", C.c_int32), ("reason", C.c_int32), ("hold", C.c_int32),
+                ("checks", C.c_int32), ("pad", C.c_int32), ("time", C.c_double), ("steps", C.c_int64),
+                ("rejected", C.c_int64), ("residual", C.c_double * 5), ("cl", C.c_double), ("cd", C.c_double),
+                ("cm", C.c_double), ("elapsed_seconds", C.c_double)]
+
+
+class amr_kernel_time(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("total_ms", C.c_double), ("launches", C.c_int64)]
+
+
+class amr_slot_capacity(C.Structure):
+    _fields_ = [("cells", C.c_int32), ("faces", C.c_int32), ("walls", C.c_int32), ("csr", C.c_int32),
+                ("terms", C.c_int32), ("groups", C.c_int32), ("members", C.c_int32), ("pad", C.c_int32)]
+
+
+class amr_batch_features(C.Structure):
+    _fields_ = [("laminar", C.c_int32), ("sa", C.c_int32), ("small", C.c_int32), ("visc_group", C.c_int32)]
+
+
+class amr_batch_options(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("tile", C.c_int32), ("graph", C.c_int32), ("batch_trials", C.c_int32)]
+
+
+class amr_slot_state(C.Structure):
+    _fields_ = [("time", C.c_double), ("target", C.c_double), ("steps", C.c_int64), ("rejected", C.c_int64),
+                ("n_cells", C.c_int32), ("finished", C.c_int32), ("fatal", C.c_int32), ("pad", C.c_int32)]
+
+
+STOP_REASONS = {0: "raw-residual-and-three-coefficients", 1: "time-budget", 2: "physical-time-cap",
+                3: "admissibility-or-advance-failure", 4: "short-validation"}
+
+
 def _dtype_of(struct) -> np.dtype:
     """numpy structured dtype with the exact offsets of a ctypes Structure."""
     names, formats, offsets = [], [], []
@@ -178,6 +225,48 @@ def _bind_mesh_api(lib, prefix, handle_name):
         fn.argtypes = argtypes
 
 
+def _bind_batch_api(lib):
+    """Declare the amr_batch_* and amr_driver_* entry points."""
+    H = C.c_void_p
+    E = [C.c_char_p, C.c_size_t]
+    sig = {
+        "amr_batch_capacity": (C.c_int, [C.POINTER(amr_mesh_view), C.c_int32, C.POINTER(amr_slot_capacity),
+                                         C.POINTER(amr_batch_features)]),
+        "amr_batch_bytes": (C.c_uint64, [C.POINTER(amr_slot_capacity), C.c_int32, C.POINTER(amr_batch_features),
+                                         C.c_int32, C.c_int32]),
+        "amr_batch_create": (C.c_int, [C.POINTER(amr_mesh_view), C.POINTER(amr_params), C.POINTER(c_double_p),
+                                       C.POINTER(amr_state_meta), C.c_int32, C.POINTER(amr_batch_options),
+                                       C.POINTER(H), *E]),
+        "amr_batch_create_slots": (C.c_int, [C.POINTER(amr_slot_capacity), C.c_int32, C.POINTER(amr_batch_features),
+                                             C.POINTER(amr_batch_options), C.POINTER(H), *E]),
+        "amr_batch_admit": (C.c_int, [H, C.c_int32, C.POINTER(amr_mesh_view), C.POINTER(amr_params), c_double_p,
+                                      C.c_uint64, C.POINTER(amr_state_meta), *E]),
+        "amr_batch_release": (C.c_int, [H, C.c_int32, *E]),
+        "amr_batch_set_targets": (C.c_int, [H, c_double_p, C.POINTER(C.c_int64), *E]),
+        "amr_batch_advance": (C.c_int, [H, C.c_int32, C.c_double, C.POINTER(amr_slot_state), *E]),
+        "amr_batch_run": (C.c_int, [H, C.c_int32, C.POINTER(amr_slot_state), *E]),
+        "amr_batch_slots": (C.c_int, [H, C.POINTER(amr_slot_state)]),
+        "amr_batch_size": (C.c_int32, [H]),
+        "amr_batch_download": (C.c_int, [H, C.c_int32, c_double_p, C.c_uint64, C.POINTER(amr_state_meta), *E]),
+        "amr_batch_monitor": (C.c_int, [H, c_i32_p, c_double_p, c_double_p, c_double_p, c_double_p, c_i32_p, *E]),
+        "amr_batch_profile": (C.c_int, [H, C.c_int32, C.POINTER(amr_kernel_time), C.c_int32, C.POINTER(C.c_int32),
+                                        *E]),
+        "amr_batch_stats": (None, [H, C.POINTER(amr_gpu_stats)]),
+        "amr_batch_destroy": (None, [H]),
+        "amr_driver_create": (C.c_int, [H, C.POINTER(H), *E]),
+        "amr_driver_begin": (C.c_int, [H, C.c_int32, C.POINTER(amr_steady_options), *E]),
+        "amr_driver_poll": (C.c_int, [H, C.c_double, C.POINTER(amr_steady_result), c_i32_p, C.POINTER(C.c_int32),
+                                      *E]),
+        "amr_driver_running": (C.c_int32, [H]),
+        "amr_driver_destroy": (None, [H]),
+        "amr_batch_solve": (C.c_int, [H, C.POINTER(amr_steady_options), C.POINTER(amr_steady_result), *E]),
+    }
+    for name, (restype, argtypes) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+
+
 _LIB = None
 
 
@@ -210,6 +299,22 @@ def load_library(path: os.PathLike | None = None):
     lib.amr_hip_rhs.argtypes = [H, c_double_p, C.c_uint64, c_double_p, C.c_char_p, C.c_size_t]
     lib.amr_hip_device_count.restype = C.c_int
     lib.amr_hip_device_count.argtypes = []
+    lib.amr_hip_run.restype = C.c_int
+    lib.amr_hip_run.argtypes = [H, C.c_double, C.c_int32, C.POINTER(amr_state_meta), C.c_char_p, C.c_size_t]
+    lib.amr_hip_profile.restype = C.c_int
+    lib.amr_hip_profile.argtypes = [H, C.c_double, C.c_int32, C.POINTER(amr_kernel_time), C.c_int32,
+                                    C.POINTER(C.c_int32), C.c_char_p, C.c_size_t]
+    lib.amr_hip_copy_bandwidth.restype = C.c_int
+    lib.amr_hip_copy_bandwidth.argtypes = [C.c_uint64, C.c_int32, c_double_p, c_double_p, C.c_char_p, C.c_size_t]
+    lib.amr_hip_state.restype = C.c_int
+    lib.amr_hip_state.argtypes = [H, C.POINTER(amr_state_meta)]
+    lib.amr_hip_monitor.restype = C.c_int
+    lib.amr_hip_monitor.argtypes = [H, c_double_p, c_double_p, C.c_double, C.c_double, c_double_p, C.c_char_p,
+                                    C.c_size_t]
+    lib.amr_hip_solve.restype = C.c_int
+    lib.amr_hip_solve.argtypes = [H, C.POINTER(amr_steady_options), C.POINTER(amr_steady_result),
+                                  C.POINTER(amr_steady_check), C.c_int32, C.c_char_p, C.c_size_t]
+    _bind_batch_api(lib)
     lib.amr_hip_check_layout.restype = C.c_int
     lib.amr_hip_check_layout.argtypes = [C.POINTER(amr_mesh_view), C.POINTER(amr_params), C.c_int32, C.c_int32,
                                          C.c_char_p, C.c_size_t]

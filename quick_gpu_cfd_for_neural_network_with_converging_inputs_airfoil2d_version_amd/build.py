@@ -20,8 +20,8 @@ OUT = PACKAGE / "libamr_hip.so"
 OBJ = ROOT / "_build" / "obj"
 
 HOST_SOURCES = ["host/cutcell.cpp", "host/mesher.cpp", "host/viscous_prep.cpp", "host/mesh_capi.cpp",
-                "host/cfd_api.cpp"]
-HIP_SOURCES = ["hip/amr_kernels.hip", "hip/amr_device.hip", "hip/amr_session.hip"]
+                "host/cfd_api.cpp", "host/steady.cpp", "host/batch_steady.cpp"]
+HIP_SOURCES = ["hip/amr_kernels.hip", "hip/amr_device.hip", "hip/amr_session.hip", "hip/amr_batch.hip"]
 INCLUDES = ["-I", str(ROOT / "include"), "-I", str(CSRC / "host"), "-I", str(CSRC / "hip")]
 ARCH = os.environ.get("HIPAMR_ARCH", "gfx950")
 

@@ -93,9 +93,18 @@ struct ArenaPtrs {
   const T* wall_ny;
   const T* wall_len;
   const T* wall_ddist;
-  // state and work arrays
-  T* U[2];
-  T* U2;
+  // steady monitor: FP64 centroids and fragment midpoints, per-block FP64 partial sums
+  const double* cell_cx;
+  const double* cell_cy;
+  const double* wall_mx;
+  const double* wall_my;
+  const double* mon_frame;  // n_meshes x 6: region xmin, xmax, ymin, ymax; moment point x, y
+  double* MON_CELL;  // n_cell_blocks x 6: sum V|r_v| (v < 5) and sum V over the region
+  double* MON_WALL;  // n_wall_blocks x 3: fx, fy, mz
+  // Conserved state: FP64 in both precisions (the RK update dt*r must stay above one ulp of u
+  // down to the steady tolerance); residuals and all work arrays are T.
+  double* U[2];
+  double* U2;
   T* R0;
   T* W;       // rho, u, v, p
   T* PHI;     // nutilde = rho nutilde / rho
@@ -115,19 +124,32 @@ struct ArenaPtrs {
   T* WCOEF;
   T* SWFLUX;
   T* SWCOEF;
-  T* QHAT;    // SRD group averages, 4 planes
+  double* QHAT;  // SRD group averages, 4 planes
   T* CELL_DT; // deferred local dt (SRD-grouped viscous limit only)
   T* DRATE;
   T* DGROUP;
 };
 
+// Optional observer of the launch sequence (per-kernel timing): before(name) is called right
+// before and after(name) right after each kernel launch on the stream.
+struct LaunchHook {
+  virtual void before(const char* name) = 0;
+  virtual void after(const char* name) = 0;
+  virtual ~LaunchHook() = default;
+};
+
 // Launch one full trial (one SSP-RK3 step attempt) of every mesh on `stream`.
 // Returns the number of kernel launches it issued.
 template <class T>
-int launch_trial(const ArenaPtrs<T>& a, hipStream_t stream);
+int launch_trial(const ArenaPtrs<T>& a, hipStream_t stream, LaunchHook* hook = nullptr);
 // Base pass only: primitives of U[cur] and R0 = L(U[cur]) (the residual evaluation).
 template <class T>
-int launch_residual(const ArenaPtrs<T>& a, hipStream_t stream);
+int launch_residual(const ArenaPtrs<T>& a, hipStream_t stream, LaunchHook* hook = nullptr);
+// Steady-monitor partial sums of the residual R0 (cells whose centroid lies in the mesh's
+// region) and of the wall pressure/shear force and moment about the mesh's moment point, both
+// taken from mon_frame.
+template <class T>
+void launch_monitor(const ArenaPtrs<T>& a, hipStream_t stream);
 // Set the target time and step cap of every mesh and start a trial.  targets/maxsteps are
 // device arrays of n_meshes entries, or null to use the scalar values for all meshes.
 void launch_target(Control* ctrl, int n_meshes, double target, long long maxsteps, const double* targets,

@@ -1,9 +1,14 @@
-// amr_device.hip -- arena layout, one-allocation upload and state transfer (see amr_device.h).
+// amr_device.hip -- slot layout of an arena, one-allocation upload, admission, state transfer.
 //
-// Layout: mesh m gets tile-aligned global ranges [cell_begin, cell_begin + n_cells) etc.; the
-// padding tail of a range belongs to no cell (all CSR offsets stay monotone there).  Every
-// array lives in one hipMalloc'ed slab (256-byte aligned sub-allocations), so an arena costs a
-// single allocation and frees in one call.
+// Every array lives in one hipMalloc'ed slab (256-byte aligned sub-allocations).  Host copies
+// of the geometry tables are kept so that admitting a mesh into a slot rewrites and uploads
+// only that slot's ranges.  Index spaces and their per-slot ranges:
+//   cells   [cell_begin, cell_begin + cap.cells)   (tile multiple, >= n_cells + 1)
+//   faces   [face_begin, face_begin + cap.faces)   walls [wall_begin, wall_begin + cap.walls)
+//   csr     [csr_begin, csr_begin + cap.csr)       terms [term_begin, term_begin + cap.terms)
+//   groups  [grp_begin, grp_begin + cap.groups)    members [mem_begin, mem_begin + cap.members)
+// Offsets arrays (nc + 1 entries) hold global positions; a slot's padding cells repeat the
+// end of its used range, so a real cell's range never reaches unused entries.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -16,6 +21,7 @@ namespace {
 
 constexpr int BC_TRANSMISSIVE = 1, BC_REFLECT = 2, BC_FARFIELD = 3, BC_TOTAL_INLET = 4, BC_PRESSURE_OUTLET = 5,
               BC_NOSLIP = 6;
+constexpr unsigned long long kInf = 0x7ff0000000000000ull;
 
 bool state_ok(const double* w) {
   return std::isfinite(w[0]) && w[0] > 0 && std::isfinite(w[1]) && std::isfinite(w[2]) && std::isfinite(w[3]) &&
@@ -67,20 +73,89 @@ bool tables_ready(const amr_mesh_view& v) {
 }
 
 inline int round_up(int n, int tile) { return (n + tile - 1) / tile * tile; }
+inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Element counts of an arena's index spaces.
+struct Totals {
+  size_t meshes, cells, faces, walls, csr, terms, groups, members, cell_blocks, face_blocks, wall_blocks;
+};
+
+Totals totals_of(const std::vector<SlotCapacity>& caps, int tile) {
+  Totals t{};
+  t.meshes = caps.size();
+  for (const SlotCapacity& c : caps) {
+    t.cells += c.cells;
+    t.faces += c.faces;
+    t.walls += c.walls;
+    t.csr += c.csr;
+    t.terms += c.terms;
+    t.groups += c.groups;
+    t.members += c.members;
+  }
+  t.cell_blocks = t.cells / tile;
+  t.face_blocks = t.faces / tile;
+  t.wall_blocks = t.walls / tile;
+  return t;
+}
+
+// Byte plan of the slab: geometry tables, control records, work planes.  Every array takes at
+// least one element and starts on a 256-byte boundary.
+struct SlabPlan {
+  size_t geometry, control, work, total;
+  size_t cplane, dplane, fplane, wplane, mon_cell, mon_wall;
+};
+
+template <class T>
+SlabPlan plan_slab(const Totals& t, int nv) {
+  const auto arr = [](size_t elem, size_t count) { return align256(elem * std::max<size_t>(count, 1)); };
+  SlabPlan p{};
+  p.geometry = arr(sizeof(MeshDesc<T>), t.meshes) + arr(4, t.cell_blocks) + arr(4, t.face_blocks + t.wall_blocks) +
+               5 * arr(sizeof(T), t.cells) + 2 * arr(4, t.cells) + 2 * arr(8, t.cells) + 4 * arr(4, t.cells + 1) +
+               arr(4, t.csr) + arr(4, t.terms) + 2 * arr(sizeof(T), t.terms) + arr(4, t.groups) +
+               arr(sizeof(T), t.groups) + arr(4, t.members) + 3 * arr(4, t.faces) + 6 * arr(sizeof(T), t.faces) +
+               2 * arr(4, t.walls) + 4 * arr(sizeof(T), t.walls) + 2 * arr(8, t.walls);
+  p.control = arr(sizeof(Control), t.meshes) + arr(sizeof(double), t.meshes) + arr(sizeof(long long), t.meshes) +
+              arr(6 * sizeof(double), t.meshes);
+  p.cplane = arr(sizeof(T), t.cells);
+  p.dplane = arr(sizeof(double), t.cells);
+  p.fplane = arr(sizeof(T), t.faces);
+  p.wplane = arr(sizeof(T), t.walls);
+  p.mon_cell = arr(6 * sizeof(double), t.cell_blocks);
+  p.mon_wall = arr(3 * sizeof(double), t.wall_blocks);
+  // FP64 state U[2], U2 (nv planes each) and QHAT (4); T planes R0 (nv), W (4), PHI, GRAD (4),
+  // MU, EDDY, SGRAD (2), SDIFF, SSRC, SRATE, CELL_DT, DRATE, DGROUP; faces FLUX (4), SPEED,
+  // FCOEF, SFLUX, SFCOEF; walls WFLUX (2), WCOEF, SWFLUX, SWCOEF; monitor partials.
+  const size_t state_planes = 3 * nv + 4, cell_planes = nv + 4 + 1 + 4 + 2 + 2 + 3 + 3;
+  p.work = state_planes * p.dplane + cell_planes * p.cplane + 8 * p.fplane + 5 * p.wplane + p.mon_cell + p.mon_wall;
+  p.total = p.geometry + p.control + p.work;
+  return p;
+}
 
 template <class T>
 struct HostArrays {
   std::vector<MeshDesc<T>> desc;
   std::vector<int> cell_block_mesh, face_block_mesh;
-  std::vector<T> vol, inv_vol, dt_area, wall_dist;
-  std::vector<int> csr_off, csr_face, wall_first, wall_cnt, term_off, term_src;
-  std::vector<T> term_wx, term_wy;
-  std::vector<int> grp_off, grp_mem, mem_off, mem_grp;
-  std::vector<T> grp_w, mem_inv;
+  std::vector<T> vol, inv_vol, dt_area, wall_dist, mem_inv;
+  std::vector<int> wall_first, wall_cnt, csr_off, term_off, grp_off, mem_off;
+  std::vector<double> cell_cx, cell_cy;
+  std::vector<int> csr_face, term_src, grp_mem, mem_grp;
+  std::vector<T> term_wx, term_wy, grp_w;
   std::vector<int> face_l, face_r, face_info;
   std::vector<T> face_len, face_dx, face_dy, face_ex, face_ey, face_dist;
   std::vector<int> wall_owner, wall_donor;
   std::vector<T> wall_nx, wall_ny, wall_len, wall_ddist;
+  std::vector<double> wall_mx, wall_my;
+};
+
+enum Space { CELLS, OFFSETS, FACES, WALLS, CSR, TERMS, GROUPS, MEMBERS, TABLE };
+
+// One uploaded array: where its host copy and device copy live, in which index space.
+struct Column {
+  const void* host;
+  void* dev;
+  size_t elem;
+  size_t count;
+  Space space;
 };
 
 template <class T>
@@ -89,414 +164,450 @@ class DeviceArena final : public Arena {
   ~DeviceArena() override {
     if (slab_) (void)hipFree(slab_);
     if (pinned_) (void)hipHostFree(pinned_);
+    if (pinned_frames_) (void)hipHostFree(pinned_frames_);
   }
 
-  std::string build(const std::vector<MeshInput>& in, int tile, hipStream_t stream) {
-    HostArrays<T> h;
-    std::string message = layout(in, tile, &h);
-    if (message.empty()) message = check_layout(h);
-    if (!message.empty()) return message;
-    return upload(h, in, stream);
-  }
-
-  // Host-side arrays of the arena (no device calls).
-  std::string layout(const std::vector<MeshInput>& in, int tile, HostArrays<T>* hp) {
-    HostArrays<T>& h = *hp;
+  // Slots, host tables with every slot empty; no device calls.
+  std::string layout(const std::vector<SlotCapacity>& caps, const ArenaFeatures& f, int tile) {
     tile_ = tile;
-    int nc = 0, nf = 0, nw = 0;
-    for (const MeshInput& m : in) {
+    features_ = f;
+    int nc = 0, nf = 0, nw = 0, ncsr = 0, nt = 0, ng = 0, nm = 0;
+    for (const SlotCapacity& c : caps) {
+      if (c.cells < tile || c.cells % tile || c.faces % tile || c.walls % tile || c.faces < 0 || c.walls < 0 ||
+          c.csr < 0 || c.terms < 0 || c.groups < 0 || c.members < 0)
+        return "invalid slot capacity";
       MeshSlot s{};
       s.cell_begin = nc;
-      s.n_cells = static_cast<int>(m.view->n_cells);
       s.face_begin = nf;
-      s.n_faces = static_cast<int>(m.view->n_faces);
       s.wall_begin = nw;
-      s.n_walls = static_cast<int>(m.view->n_walls);
-      s.nv = m.view->transport.sa_model ? 5 : 4;
-      nc += round_up(s.n_cells, tile);
-      nf += round_up(s.n_faces, tile);
-      nw += round_up(s.n_walls, tile);
+      s.csr_begin = ncsr;
+      s.term_begin = nt;
+      s.grp_begin = ng;
+      s.mem_begin = nm;
+      s.cap = c;
+      s.nv = f.sa ? 5 : 4;
       slots_.push_back(s);
+      nc += c.cells;
+      nf += c.faces;
+      nw += c.walls;
+      ncsr += c.csr;
+      nt += c.terms;
+      ng += c.groups;
+      nm += c.members;
+      if (nc >= (1 << 30) || nf >= (1 << 30) || nw >= (1 << 30) || ncsr >= (1 << 30) || nt >= (1 << 30))
+        return "arena exceeds 2^30 elements in one index space";
     }
+    if (slots_.empty()) return "arena without slots";
+    totals_ = totals_of(caps, tile);
     nc_ = nc;
     nf_ = nf;
     nw_ = nw;
-    h.vol.assign(nc, T(1));
-    h.inv_vol.assign(nc, T(1));
-    h.dt_area.assign(nc, T(1));
-    h.wall_dist.assign(nc, T(1));
-    h.wall_first.assign(nc, 0);
-    h.wall_cnt.assign(nc, 0);
-    h.mem_inv.assign(nc, T(1));
-    h.face_l.assign(nf, 0);
-    h.face_r.assign(nf, 0);
-    h.face_info.assign(nf, 0);
-    h.face_len.assign(nf, T(0));
-    h.face_dx.assign(nf, T(0));
-    h.face_dy.assign(nf, T(0));
-    h.face_ex.assign(nf, T(0));
-    h.face_ey.assign(nf, T(0));
-    h.face_dist.assign(nf, T(1));
-    h.wall_owner.assign(nw, 0);
-    h.wall_donor.assign(nw, 0);
-    h.wall_nx.assign(nw, T(0));
-    h.wall_ny.assign(nw, T(0));
-    h.wall_len.assign(nw, T(0));
-    h.wall_ddist.assign(nw, T(1));
-    h.csr_off.push_back(0);
-    h.term_off.push_back(0);
-    h.grp_off.push_back(0);
-    h.mem_off.push_back(0);
-    for (size_t mi = 0; mi < in.size(); ++mi) {
-      const amr_mesh_view& v = *in[mi].view;
-      const amr_params& p = in[mi].params;
-      const MeshSlot& s = slots_[mi];
-      const int cb = s.cell_begin, fb = s.face_begin, wb = s.wall_begin;
-      const bool laminar = v.transport.model == 1, sa = v.transport.sa_model != 0;
-      for (int k = 0; k < round_up(s.n_cells, tile); ++k) {
-        const int g = cb + k;
-        if (k < s.n_cells) {
-          const amr_cell& c = v.cells[k];
-          h.vol[g] = static_cast<T>(c.volume);
-          h.inv_vol[g] = static_cast<T>(1.0 / c.volume);
-          h.dt_area[g] = static_cast<T>(v.agglomerate_small ? c.volume : (c.x1 - c.x0) * (c.y1 - c.y0));
-          if (sa) h.wall_dist[g] = static_cast<T>(v.wall_distance[k]);
-          h.wall_first[g] = wb + static_cast<int>(c.wall_begin);
-          h.wall_cnt[g] = static_cast<int>(c.wall_count);
-          for (uint64_t pos = v.face_offsets[k]; pos < v.face_offsets[k + 1]; ++pos) {
-            const int q = static_cast<int>(v.cell_faces[pos]);
-            h.csr_face.push_back(v.faces[q].left == k ? fb + q : ~(fb + q));
-          }
-          if (laminar)
-            for (uint64_t pos = v.visc_offsets[k]; pos < v.visc_offsets[k + 1]; ++pos) {
-              const amr_grad_term& t = v.visc_terms[pos];
-              h.term_src.push_back(t.source >= 0 ? cb + t.source : (t.source == -1 ? -1 : -2 - t.side));
-              h.term_wx.push_back(static_cast<T>(t.x));
-              h.term_wy.push_back(static_cast<T>(t.y));
-            }
-          const uint64_t n_k = v.member_offsets[k + 1] - v.member_offsets[k];
-          h.mem_inv[g] = static_cast<T>(1.0 / static_cast<double>(n_k));
-          for (uint64_t pos = v.group_offsets[k]; pos < v.group_offsets[k + 1]; ++pos) {
-            const uint64_t member = v.group_members[pos];
-            const uint64_t n_m = v.member_offsets[member + 1] - v.member_offsets[member];
-            h.grp_mem.push_back(cb + static_cast<int>(member));
-            h.grp_w.push_back(static_cast<T>(v.cells[member].volume / static_cast<double>(n_m) / v.group_volume[k]));
-          }
-          for (uint64_t pos = v.member_offsets[k]; pos < v.member_offsets[k + 1]; ++pos)
-            h.mem_grp.push_back(cb + static_cast<int>(v.member_groups[pos]));
-        }
-        h.csr_off.push_back(static_cast<int>(h.csr_face.size()));
-        h.term_off.push_back(static_cast<int>(h.term_src.size()));
-        h.grp_off.push_back(static_cast<int>(h.grp_mem.size()));
-        h.mem_off.push_back(static_cast<int>(h.mem_grp.size()));
-      }
-      for (int q = 0; q < s.n_faces; ++q) {
-        const amr_face& f = v.faces[q];
-        const int g = fb + q;
-        h.face_l[g] = f.left >= 0 ? cb + f.left : -1;
-        h.face_r[g] = f.right >= 0 ? cb + f.right : -1;
-        h.face_info[g] = (f.axis & 1) | ((f.boundary + 1) << 1);
-        h.face_len[g] = static_cast<T>(f.length);
-        if (laminar) {
-          const amr_face_geometry& geo = v.face_geometry[q];
-          h.face_dx[g] = static_cast<T>(geo.dx);
-          h.face_dy[g] = static_cast<T>(geo.dy);
-          h.face_ex[g] = static_cast<T>(geo.ex);
-          h.face_ey[g] = static_cast<T>(geo.ey);
-          h.face_dist[g] = static_cast<T>(f.boundary >= 0 ? v.face_samples[q].distance : geo.distance);
-        }
-      }
-      for (int j = 0; j < s.n_walls; ++j) {
-        const amr_wall& w = v.walls[j];
-        const int g = wb + j;
-        h.wall_owner[g] = cb + static_cast<int>(w.owner_cell);
-        h.wall_donor[g] = laminar ? cb + v.wall_samples[j].donor : cb + static_cast<int>(w.owner_cell);
-        h.wall_nx[g] = static_cast<T>(w.nx);
-        h.wall_ny[g] = static_cast<T>(w.ny);
-        h.wall_len[g] = static_cast<T>(w.length);
-        h.wall_ddist[g] = static_cast<T>(laminar ? v.wall_samples[j].distance : 1.0);
-      }
-      MeshDesc<T> d{};
-      d.cell_begin = cb;
-      d.n_cells = s.n_cells;
-      d.face_begin = fb;
-      d.n_faces = s.n_faces;
-      d.wall_begin = wb;
-      d.n_walls = s.n_walls;
-      d.first_cell_block = cb / tile;
-      d.nv = s.nv;
-      d.laminar = laminar;
-      d.sa = sa;
-      d.flux = p.flux;
-      d.small = v.small_cells > 0;
-      d.visc_mode = (!laminar || v.transport.stabilization == 0) ? VISC_VOLUME
-                    : (d.small ? VISC_SRD_GROUP : VISC_SRD_RATE);
-      d.agglomerated = v.agglomerate_small;
-      d.dt_fixed = p.dt_fixed;
-      d.gamma = static_cast<T>(p.gamma);
-      d.cfl = static_cast<T>(p.cfl);
-      d.mco = static_cast<T>(p.ausm_mco);
-      d.transport.law = v.transport.law;
-      d.transport.mu_ref = static_cast<T>(v.transport.mu_ref);
-      d.transport.gas_constant = static_cast<T>(v.transport.gas_constant);
-      d.transport.temperature_ref = static_cast<T>(v.transport.temperature_ref);
-      d.transport.sutherland = static_cast<T>(v.transport.sutherland);
-      d.transport.viscous_cfl = static_cast<T>(v.transport.viscous_cfl);
-      d.transport.sa_nu_inf = static_cast<T>(v.transport.sa_nu_inf);
-      d.transport.sa_cfl = static_cast<T>(v.transport.sa_cfl);
-      for (int side = 0; side < 4; ++side) {
-        const amr_boundary& b = v.boundaries[side];
-        Side<T>& o = d.side[side];
-        o.kind = b.kind < 0 ? p.bc[side] : b.kind;
-        for (int q = 0; q < 4; ++q) o.reference[q] = static_cast<T>(b.reference[q]);
-        o.total_pressure = static_cast<T>(b.total_pressure);
-        o.total_temperature = static_cast<T>(b.total_temperature);
-        o.gas_constant = static_cast<T>(b.gas_constant);
-        o.direction[0] = static_cast<T>(b.direction[0]);
-        o.direction[1] = static_cast<T>(b.direction[1]);
-        o.outlet_pressure = static_cast<T>(b.outlet_pressure);
-        o.wall_velocity[0] = static_cast<T>(b.wall_velocity[0]);
-        o.wall_velocity[1] = static_cast<T>(b.wall_velocity[1]);
-      }
-      h.desc.push_back(d);
-      for (int b = 0; b < round_up(s.n_cells, tile) / tile; ++b) h.cell_block_mesh.push_back(static_cast<int>(mi));
-      any_laminar_ |= laminar;
-      any_sa_ |= sa;
-      any_small_ |= d.small != 0;
-      any_visc_group_ |= d.visc_mode == VISC_SRD_GROUP;
-    }
-    for (size_t mi = 0; mi < in.size(); ++mi)
-      for (int b = 0; b < round_up(slots_[mi].n_faces, tile) / tile; ++b) h.face_block_mesh.push_back(static_cast<int>(mi));
+    HostArrays<T>& h = h_;
+    h.desc.assign(slots_.size(), MeshDesc<T>{});
+    for (size_t m = 0; m < slots_.size(); ++m)
+      for (int b = 0; b < slots_[m].cap.cells / tile; ++b) h.cell_block_mesh.push_back(static_cast<int>(m));
+    for (size_t m = 0; m < slots_.size(); ++m)
+      for (int b = 0; b < slots_[m].cap.faces / tile; ++b) h.face_block_mesh.push_back(static_cast<int>(m));
     n_face_blocks_ = static_cast<int>(h.face_block_mesh.size());
-    for (size_t mi = 0; mi < in.size(); ++mi)
-      for (int b = 0; b < round_up(slots_[mi].n_walls, tile) / tile; ++b) h.face_block_mesh.push_back(static_cast<int>(mi));
+    for (size_t m = 0; m < slots_.size(); ++m)
+      for (int b = 0; b < slots_[m].cap.walls / tile; ++b) h.face_block_mesh.push_back(static_cast<int>(m));
     n_wall_blocks_ = static_cast<int>(h.face_block_mesh.size()) - n_face_blocks_;
     n_cell_blocks_ = static_cast<int>(h.cell_block_mesh.size());
-    // Non-empty placeholders keep every device pointer valid.
-    for (auto* v : {&h.csr_face, &h.term_src, &h.grp_mem, &h.mem_grp})
-      if (v->empty()) v->push_back(0);
-    for (auto* v : {&h.term_wx, &h.term_wy, &h.grp_w})
-      if (v->empty()) v->push_back(T(0));
     if (h.face_block_mesh.empty()) h.face_block_mesh.push_back(0);
+    const auto sized = [](auto& v, size_t n, auto value) { v.assign(std::max<size_t>(n, 1), value); };
+    sized(h.vol, nc, T(1));
+    sized(h.inv_vol, nc, T(1));
+    sized(h.dt_area, nc, T(1));
+    sized(h.wall_dist, nc, T(1));
+    sized(h.mem_inv, nc, T(1));
+    sized(h.wall_first, nc, 0);
+    sized(h.wall_cnt, nc, 0);
+    sized(h.cell_cx, nc, 0.0);
+    sized(h.cell_cy, nc, 0.0);
+    h.csr_off.assign(nc + 1, 0);
+    h.term_off.assign(nc + 1, 0);
+    h.grp_off.assign(nc + 1, 0);
+    h.mem_off.assign(nc + 1, 0);
+    sized(h.csr_face, ncsr, 0);
+    sized(h.term_src, nt, -1);
+    sized(h.term_wx, nt, T(0));
+    sized(h.term_wy, nt, T(0));
+    sized(h.grp_mem, ng, 0);
+    sized(h.grp_w, ng, T(0));
+    sized(h.mem_grp, nm, 0);
+    sized(h.face_l, nf, 0);
+    sized(h.face_r, nf, 0);
+    sized(h.face_info, nf, 0);
+    sized(h.face_len, nf, T(0));
+    sized(h.face_dx, nf, T(0));
+    sized(h.face_dy, nf, T(0));
+    sized(h.face_ex, nf, T(0));
+    sized(h.face_ey, nf, T(0));
+    sized(h.face_dist, nf, T(1));
+    sized(h.wall_owner, nw, 0);
+    sized(h.wall_donor, nw, 0);
+    sized(h.wall_nx, nw, T(0));
+    sized(h.wall_ny, nw, T(0));
+    sized(h.wall_len, nw, T(0));
+    sized(h.wall_ddist, nw, T(1));
+    sized(h.wall_mx, nw, 0.0);
+    sized(h.wall_my, nw, 0.0);
+    sa_nu_inf_.assign(slots_.size(), 0.0);
+    for (size_t m = 0; m < slots_.size(); ++m) clear_slot(static_cast<int>(m));
+    // The sentinel after the last slot.
+    h.csr_off[nc] = ncsr;
+    h.term_off[nc] = nt;
+    h.grp_off[nc] = ng;
+    h.mem_off[nc] = nm;
     return "";
   }
 
-  // Every index a kernel dereferences stays inside its array and inside its own mesh's range
-  // (the bounds the launch grid and the kernels assume).  Empty string when consistent.
-  std::string check_layout(const HostArrays<T>& h) const {
-    const int n_meshes = static_cast<int>(slots_.size());
-    if (static_cast<int>(h.desc.size()) != n_meshes) return "layout: descriptor count";
+  // Whether mesh v fits slot m's capacity and the arena's kernel set.
+  std::string fits(int m, const amr_mesh_view& v) const {
+    const MeshSlot& s = slots_[m];
+    const SlotCapacity need = capacity_of(v, tile_);
+    if (need.cells > s.cap.cells || need.faces > s.cap.faces || need.walls > s.cap.walls || need.csr > s.cap.csr ||
+        need.terms > s.cap.terms || need.groups > s.cap.groups || need.members > s.cap.members)
+      return "mesh exceeds the slot capacity";
+    const ArenaFeatures f = features_of(v);
+    if ((f.laminar && !features_.laminar) || (f.sa && !features_.sa) || (f.small && !features_.small) ||
+        (f.visc_group && !features_.visc_group))
+      return "mesh needs kernels the arena was not built with";
+    if (f.sa != features_.sa) return "every mesh of an arena has the same number of conserved variables";
+    return "";
+  }
+
+  // Host tables of slot m for mesh `in`.
+  std::string fill_slot(int m, const MeshInput& in) {
+    const std::string e = fits(m, *in.view);
+    if (!e.empty()) return e;
+    const amr_mesh_view& v = *in.view;
+    const amr_params& p = in.params;
+    MeshSlot& s = slots_[m];
+    HostArrays<T>& h = h_;
+    const int cb = s.cell_begin, fb = s.face_begin, wb = s.wall_begin;
+    const int n = static_cast<int>(v.n_cells), nf = static_cast<int>(v.n_faces), nw = static_cast<int>(v.n_walls);
+    const bool laminar = v.transport.model == 1, sa = v.transport.sa_model != 0;
+    int csr = s.csr_begin, term = s.term_begin, grp = s.grp_begin, mem = s.mem_begin;
+    for (int k = 0; k < s.cap.cells; ++k) {
+      const int g = cb + k;
+      h.csr_off[g] = csr;
+      h.term_off[g] = term;
+      h.grp_off[g] = grp;
+      h.mem_off[g] = mem;
+      if (k >= n) {
+        pad_cell(g, wb);
+        continue;
+      }
+      const amr_cell& c = v.cells[k];
+      h.vol[g] = static_cast<T>(c.volume);
+      h.inv_vol[g] = static_cast<T>(1.0 / c.volume);
+      h.dt_area[g] = static_cast<T>(v.agglomerate_small ? c.volume : (c.x1 - c.x0) * (c.y1 - c.y0));
+      h.wall_dist[g] = sa ? static_cast<T>(v.wall_distance[k]) : T(1);
+      h.wall_first[g] = wb + static_cast<int>(c.wall_begin);
+      h.wall_cnt[g] = static_cast<int>(c.wall_count);
+      h.cell_cx[g] = c.cx;
+      h.cell_cy[g] = c.cy;
+      for (uint64_t pos = v.face_offsets[k]; pos < v.face_offsets[k + 1]; ++pos) {
+        const int q = static_cast<int>(v.cell_faces[pos]);
+        h.csr_face[csr++] = v.faces[q].left == k ? fb + q : ~(fb + q);
+      }
+      if (laminar)
+        for (uint64_t pos = v.visc_offsets[k]; pos < v.visc_offsets[k + 1]; ++pos) {
+          const amr_grad_term& t = v.visc_terms[pos];
+          h.term_src[term] = t.source >= 0 ? cb + t.source : (t.source == -1 ? -1 : -2 - t.side);
+          h.term_wx[term] = static_cast<T>(t.x);
+          h.term_wy[term] = static_cast<T>(t.y);
+          ++term;
+        }
+      const uint64_t n_k = v.member_offsets[k + 1] - v.member_offsets[k];
+      h.mem_inv[g] = static_cast<T>(1.0 / static_cast<double>(n_k));
+      for (uint64_t pos = v.group_offsets[k]; pos < v.group_offsets[k + 1]; ++pos) {
+        const uint64_t member = v.group_members[pos];
+        const uint64_t n_m = v.member_offsets[member + 1] - v.member_offsets[member];
+        h.grp_mem[grp] = cb + static_cast<int>(member);
+        h.grp_w[grp] = static_cast<T>(v.cells[member].volume / static_cast<double>(n_m) / v.group_volume[k]);
+        ++grp;
+      }
+      for (uint64_t pos = v.member_offsets[k]; pos < v.member_offsets[k + 1]; ++pos)
+        h.mem_grp[mem++] = cb + static_cast<int>(v.member_groups[pos]);
+    }
+    for (int q = 0; q < s.cap.faces; ++q) {
+      const int g = fb + q;
+      if (q >= nf) {
+        pad_face(g, cb);
+        continue;
+      }
+      const amr_face& face = v.faces[q];
+      h.face_l[g] = face.left >= 0 ? cb + face.left : -1;
+      h.face_r[g] = face.right >= 0 ? cb + face.right : -1;
+      h.face_info[g] = (face.axis & 1) | ((face.boundary + 1) << 1);
+      h.face_len[g] = static_cast<T>(face.length);
+      if (laminar) {
+        const amr_face_geometry& geo = v.face_geometry[q];
+        h.face_dx[g] = static_cast<T>(geo.dx);
+        h.face_dy[g] = static_cast<T>(geo.dy);
+        h.face_ex[g] = static_cast<T>(geo.ex);
+        h.face_ey[g] = static_cast<T>(geo.ey);
+        h.face_dist[g] = static_cast<T>(face.boundary >= 0 ? v.face_samples[q].distance : geo.distance);
+      } else {
+        h.face_dx[g] = h.face_dy[g] = h.face_ex[g] = h.face_ey[g] = T(0);
+        h.face_dist[g] = T(1);
+      }
+    }
+    for (int j = 0; j < s.cap.walls; ++j) {
+      const int g = wb + j;
+      if (j >= nw) {
+        pad_wall(g, cb);
+        continue;
+      }
+      const amr_wall& w = v.walls[j];
+      h.wall_owner[g] = cb + static_cast<int>(w.owner_cell);
+      h.wall_donor[g] = laminar ? cb + v.wall_samples[j].donor : cb + static_cast<int>(w.owner_cell);
+      h.wall_nx[g] = static_cast<T>(w.nx);
+      h.wall_ny[g] = static_cast<T>(w.ny);
+      h.wall_len[g] = static_cast<T>(w.length);
+      h.wall_ddist[g] = static_cast<T>(laminar ? v.wall_samples[j].distance : 1.0);
+      h.wall_mx[g] = (w.x0 + w.x1) * .5;
+      h.wall_my[g] = (w.y0 + w.y1) * .5;
+    }
+    MeshDesc<T> d{};
+    d.cell_begin = cb;
+    d.n_cells = n;
+    d.face_begin = fb;
+    d.n_faces = nf;
+    d.wall_begin = wb;
+    d.n_walls = nw;
+    d.first_cell_block = cb / tile_;
+    d.nv = sa ? 5 : 4;
+    d.laminar = laminar;
+    d.sa = sa;
+    d.flux = p.flux;
+    d.small = v.small_cells > 0;
+    d.visc_mode = (!laminar || v.transport.stabilization == 0) ? VISC_VOLUME : (d.small ? VISC_SRD_GROUP : VISC_SRD_RATE);
+    d.agglomerated = v.agglomerate_small;
+    d.dt_fixed = p.dt_fixed;
+    d.gamma = static_cast<T>(p.gamma);
+    d.cfl = static_cast<T>(p.cfl);
+    d.mco = static_cast<T>(p.ausm_mco);
+    d.transport.law = v.transport.law;
+    d.transport.mu_ref = static_cast<T>(v.transport.mu_ref);
+    d.transport.gas_constant = static_cast<T>(v.transport.gas_constant);
+    d.transport.temperature_ref = static_cast<T>(v.transport.temperature_ref);
+    d.transport.sutherland = static_cast<T>(v.transport.sutherland);
+    d.transport.viscous_cfl = static_cast<T>(v.transport.viscous_cfl);
+    d.transport.sa_nu_inf = static_cast<T>(v.transport.sa_nu_inf);
+    d.transport.sa_cfl = static_cast<T>(v.transport.sa_cfl);
+    for (int side = 0; side < 4; ++side) {
+      const amr_boundary& b = v.boundaries[side];
+      Side<T>& o = d.side[side];
+      o.kind = b.kind < 0 ? p.bc[side] : b.kind;
+      for (int q = 0; q < 4; ++q) o.reference[q] = static_cast<T>(b.reference[q]);
+      o.total_pressure = static_cast<T>(b.total_pressure);
+      o.total_temperature = static_cast<T>(b.total_temperature);
+      o.gas_constant = static_cast<T>(b.gas_constant);
+      o.direction[0] = static_cast<T>(b.direction[0]);
+      o.direction[1] = static_cast<T>(b.direction[1]);
+      o.outlet_pressure = static_cast<T>(b.outlet_pressure);
+      o.wall_velocity[0] = static_cast<T>(b.wall_velocity[0]);
+      o.wall_velocity[1] = static_cast<T>(b.wall_velocity[1]);
+    }
+    h.desc[m] = d;
+    s.n_cells = n;
+    s.n_faces = nf;
+    s.n_walls = nw;
+    s.nv = d.nv;
+    sa_nu_inf_[m] = v.transport.sa_nu_inf;
+    return "";
+  }
+
+  // Every index a kernel dereferences for slot m stays inside its array and inside the slot.
+  std::string check_slot(int m) const {
+    const HostArrays<T>& h = h_;
+    const MeshSlot& s = slots_[m];
+    const MeshDesc<T>& d = h.desc[m];
+    if (d.n_cells != s.n_cells || d.cell_begin != s.cell_begin || d.face_begin != s.face_begin ||
+        d.wall_begin != s.wall_begin || d.first_cell_block * tile_ != s.cell_begin)
+      return "layout: descriptor ranges";
+    if (s.n_cells == 0) return "";
+    if (s.n_cells >= s.cap.cells || s.n_faces > s.cap.faces || s.n_walls > s.cap.walls) return "layout: slot overflow";
+    const auto cell_ok = [&](int k) { return k >= s.cell_begin && k < s.cell_begin + s.n_cells; };
+    for (int k = s.cell_begin; k < s.cell_begin + s.n_cells; ++k) {
+      if (h.csr_off[k] < s.csr_begin || h.csr_off[k] > h.csr_off[k + 1] || h.csr_off[k + 1] > s.csr_begin + s.cap.csr)
+        return "layout: cell-face offsets";
+      for (int j = h.csr_off[k]; j < h.csr_off[k + 1]; ++j) {
+        const int sq = h.csr_face[j], q = sq >= 0 ? sq : ~sq;
+        if (q < s.face_begin || q >= s.face_begin + s.n_faces) return "layout: cell-face id";
+        if ((sq >= 0 ? h.face_l[q] : h.face_r[q]) != k) return "layout: cell-face orientation";
+      }
+      if (h.wall_cnt[k] < 0 || h.wall_first[k] < s.wall_begin || h.wall_first[k] + h.wall_cnt[k] > s.wall_begin + s.n_walls)
+        return "layout: cell walls";
+      if (h.term_off[k] < s.term_begin || h.term_off[k] > h.term_off[k + 1] ||
+          h.term_off[k + 1] > s.term_begin + s.cap.terms)
+        return "layout: gradient term offsets";
+      for (int j = h.term_off[k]; j < h.term_off[k + 1]; ++j) {
+        const int src = h.term_src[j];
+        if (!(cell_ok(src) || (src <= -1 && src >= -5))) return "layout: gradient term source";
+      }
+      if (h.grp_off[k] < s.grp_begin || h.grp_off[k] > h.grp_off[k + 1] || h.grp_off[k + 1] > s.grp_begin + s.cap.groups ||
+          h.mem_off[k] < s.mem_begin || h.mem_off[k] > h.mem_off[k + 1] || h.mem_off[k + 1] > s.mem_begin + s.cap.members)
+        return "layout: SRD offsets";
+      for (int j = h.grp_off[k]; j < h.grp_off[k + 1]; ++j)
+        if (!cell_ok(h.grp_mem[j])) return "layout: SRD member";
+      for (int j = h.mem_off[k]; j < h.mem_off[k + 1]; ++j)
+        if (!cell_ok(h.mem_grp[j])) return "layout: SRD group";
+      if (h.mem_off[k + 1] == h.mem_off[k]) return "layout: cell in no SRD group";
+    }
+    for (int q = s.face_begin; q < s.face_begin + s.n_faces; ++q) {
+      const int l = h.face_l[q], r = h.face_r[q], side = (h.face_info[q] >> 1) - 1;
+      if (side < -1 || side > 3) return "layout: face side";
+      if (l != -1 && !cell_ok(l)) return "layout: face left";
+      if (r != -1 && !cell_ok(r)) return "layout: face right";
+      if (l < 0 && r < 0) return "layout: face without cell";
+      if (side < 0 && (l < 0 || r < 0)) return "layout: interior face missing a cell";
+    }
+    for (int j = s.wall_begin; j < s.wall_begin + s.n_walls; ++j)
+      if (!cell_ok(h.wall_owner[j]) || !cell_ok(h.wall_donor[j])) return "layout: wall owner/donor";
+    return "";
+  }
+
+  std::string check_tables() const {
+    const HostArrays<T>& h = h_;
     if (static_cast<int>(h.cell_block_mesh.size()) * tile_ != nc_) return "layout: cell block table";
-    if (n_face_blocks_ * tile_ != nf_ || n_wall_blocks_ * tile_ != nw_) return "layout: face/wall block table";
-    if (h.csr_off.size() != static_cast<size_t>(nc_) + 1 || h.term_off.size() != static_cast<size_t>(nc_) + 1 ||
-        h.grp_off.size() != static_cast<size_t>(nc_) + 1 || h.mem_off.size() != static_cast<size_t>(nc_) + 1)
-      return "layout: offset array sizes";
-    for (int b = 0; b < static_cast<int>(h.cell_block_mesh.size()); ++b) {
-      const int m = h.cell_block_mesh[b];
-      if (m < 0 || m >= n_meshes) return "layout: cell block mesh id";
-      const MeshSlot& s = slots_[m];
-      if (b * tile_ < s.cell_begin || b * tile_ >= s.cell_begin + std::max(s.n_cells, 1)) return "layout: cell block range";
+    for (int b = 0; b < n_cell_blocks_; ++b) {
+      const MeshSlot& s = slots_[h.cell_block_mesh[b]];
+      if (b * tile_ < s.cell_begin || b * tile_ >= s.cell_begin + s.cap.cells) return "layout: cell block range";
     }
     for (int b = 0; b < n_face_blocks_ + n_wall_blocks_; ++b) {
-      const int m = h.face_block_mesh[b];
-      if (m < 0 || m >= n_meshes) return "layout: face block mesh id";
-      const MeshSlot& s = slots_[m];
-      const int first = b < n_face_blocks_ ? b * tile_ : (b - n_face_blocks_) * tile_;
-      const int lo = b < n_face_blocks_ ? s.face_begin : s.wall_begin;
-      const int n = b < n_face_blocks_ ? s.n_faces : s.n_walls;
-      if (first < lo || first >= lo + n) return "layout: face/wall block range";
+      const MeshSlot& s = slots_[h.face_block_mesh[b]];
+      const bool face = b < n_face_blocks_;
+      const int first = (face ? b : b - n_face_blocks_) * tile_;
+      const int lo = face ? s.face_begin : s.wall_begin, cap = face ? s.cap.faces : s.cap.walls;
+      if (first < lo || first >= lo + cap) return "layout: face/wall block range";
     }
-    for (int m = 0; m < n_meshes; ++m) {
-      const MeshSlot& s = slots_[m];
-      const MeshDesc<T>& d = h.desc[m];
-      if (d.cell_begin != s.cell_begin || d.n_cells != s.n_cells || d.face_begin != s.face_begin ||
-          d.n_faces != s.n_faces || d.wall_begin != s.wall_begin || d.n_walls != s.n_walls ||
-          d.first_cell_block * tile_ != s.cell_begin)
-        return "layout: descriptor ranges";
-      const auto cell_ok = [&](int k) { return k >= s.cell_begin && k < s.cell_begin + s.n_cells; };
-      for (int k = s.cell_begin; k < s.cell_begin + s.n_cells; ++k) {
-        if (h.csr_off[k] > h.csr_off[k + 1] || h.csr_off[k + 1] > static_cast<int>(h.csr_face.size()))
-          return "layout: cell-face offsets";
-        for (int j = h.csr_off[k]; j < h.csr_off[k + 1]; ++j) {
-          const int sq = h.csr_face[j], q = sq >= 0 ? sq : ~sq;
-          if (q < s.face_begin || q >= s.face_begin + s.n_faces) return "layout: cell-face id";
-          if ((sq >= 0 ? h.face_l[q] : h.face_r[q]) != k) return "layout: cell-face orientation";
-        }
-        if (h.wall_cnt[k] < 0 || h.wall_first[k] < s.wall_begin || h.wall_first[k] + h.wall_cnt[k] > s.wall_begin + s.n_walls)
-          return "layout: cell walls";
-        if (h.term_off[k] > h.term_off[k + 1] || h.term_off[k + 1] > static_cast<int>(h.term_src.size()) ||
-            h.term_off[k + 1] > static_cast<int>(h.term_wx.size()))
-          return "layout: gradient term offsets";
-        for (int j = h.term_off[k]; j < h.term_off[k + 1]; ++j) {
-          const int src = h.term_src[j];
-          if (!(cell_ok(src) || (src <= -1 && src >= -5))) return "layout: gradient term source";
-        }
-        if (h.grp_off[k + 1] > static_cast<int>(h.grp_mem.size()) || h.mem_off[k + 1] > static_cast<int>(h.mem_grp.size()))
-          return "layout: SRD offsets";
-        for (int j = h.grp_off[k]; j < h.grp_off[k + 1]; ++j)
-          if (!cell_ok(h.grp_mem[j])) return "layout: SRD member";
-        for (int j = h.mem_off[k]; j < h.mem_off[k + 1]; ++j)
-          if (!cell_ok(h.mem_grp[j])) return "layout: SRD group";
-        if (h.mem_off[k + 1] == h.mem_off[k]) return "layout: cell in no SRD group";
-      }
-      for (int q = s.face_begin; q < s.face_begin + s.n_faces; ++q) {
-        const int l = h.face_l[q], r = h.face_r[q], side = (h.face_info[q] >> 1) - 1;
-        if (side < -1 || side > 3) return "layout: face side";
-        if (l != -1 && !cell_ok(l)) return "layout: face left";
-        if (r != -1 && !cell_ok(r)) return "layout: face right";
-        if (l < 0 && r < 0) return "layout: face without cell";
-        if (side < 0 && (l < 0 || r < 0)) return "layout: interior face missing a cell";
-      }
-      for (int j = s.wall_begin; j < s.wall_begin + s.n_walls; ++j)
-        if (!cell_ok(h.wall_owner[j]) || !cell_ok(h.wall_donor[j])) return "layout: wall owner/donor";
+    for (size_t m = 0; m < slots_.size(); ++m) {
+      const std::string e = check_slot(static_cast<int>(m));
+      if (!e.empty()) return e;
     }
     return "";
   }
 
-  int precision() const override { return sizeof(T) == 8 ? 64 : 32; }
-  const std::vector<MeshSlot>& slots() const override { return slots_; }
-  uint64_t device_bytes() const override { return slab_bytes_; }
-  uint64_t geometry_bytes() const override { return geometry_bytes_; }
-  uint64_t state_bytes() const override { return state_bytes_; }
-  int launch_trial(hipStream_t s) override { return hipamr::launch_trial<T>(ptrs_, s); }
-  int launch_residual(hipStream_t s) override { return hipamr::launch_residual<T>(ptrs_, s); }
-  Control* control() override { return ptrs_.ctrl; }
-  int n_meshes() const override { return static_cast<int>(slots_.size()); }
-  int n_cell_blocks() const override { return n_cell_blocks_; }
-  int n_face_blocks() const override { return n_face_blocks_; }
-  int n_wall_blocks() const override { return n_wall_blocks_; }
-  int tile() const override { return tile_; }
-
-  hipError_t read_state(int m, int buffer, double* out, hipStream_t s) override {
-    const MeshSlot& sl = slots_[m];
-    const T* src = buffer == 2 ? ptrs_.R0 : ptrs_.U[buffer];
-    hipError_t e = hipMemcpy2DAsync(pinned_, sl.n_cells * sizeof(T), src + sl.cell_begin, nc_ * sizeof(T),
-                                    sl.n_cells * sizeof(T), sl.nv, hipMemcpyDeviceToHost, s);
-    if (e != hipSuccess) return e;
-    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-    const T* p = static_cast<const T*>(pinned_);
-    for (size_t i = 0; i < static_cast<size_t>(sl.nv) * sl.n_cells; ++i) out[i] = static_cast<double>(p[i]);
-    return hipSuccess;
-  }
-
-  hipError_t write_state(int m, int buffer, const double* in, hipStream_t s) override {
-    const MeshSlot& sl = slots_[m];
-    T* p = static_cast<T*>(pinned_);
-    for (size_t i = 0; i < static_cast<size_t>(sl.nv) * sl.n_cells; ++i) p[i] = static_cast<T>(in[i]);
-    hipError_t e = hipMemcpy2DAsync(ptrs_.U[buffer] + sl.cell_begin, nc_ * sizeof(T), pinned_, sl.n_cells * sizeof(T),
-                                    sl.n_cells * sizeof(T), sl.nv, hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return e;
-    return hipStreamSynchronize(s);
-  }
-
-  std::string error_;
-
- private:
-  template <class V>
-  static size_t bytes_of(const std::vector<V>& v) {
-    return (v.size() * sizeof(V) + 255) / 256 * 256;
-  }
-
-  std::string upload(const HostArrays<T>& h, const std::vector<MeshInput>& in, hipStream_t stream) {
-    const int nc = nc_, nf = std::max(nf_, 1), nw = std::max(nw_, 1);
-    const size_t cplane = (static_cast<size_t>(nc) * sizeof(T) + 255) / 256 * 256;
-    const size_t fplane = (static_cast<size_t>(nf) * sizeof(T) + 255) / 256 * 256;
-    const size_t wplane = (static_cast<size_t>(nw) * sizeof(T) + 255) / 256 * 256;
-    // Geometry + tables.
+  // Allocate the slab, upload every table, clear the work arrays, allocate pinned staging.
+  std::string upload_all(const std::vector<amr_state_meta>& metas, hipStream_t stream) {
+    const HostArrays<T>& h = h_;
+    columns_.clear();
+    const auto col = [&](const auto& vec, Space sp) {
+      using V = typename std::decay_t<decltype(vec)>::value_type;
+      columns_.push_back({vec.data(), nullptr, sizeof(V), vec.size(), sp});
+    };
+    col(h.desc, TABLE);
+    col(h.cell_block_mesh, TABLE);
+    col(h.face_block_mesh, TABLE);
+    for (auto* v : {&h.vol, &h.inv_vol, &h.dt_area, &h.wall_dist, &h.mem_inv}) col(*v, CELLS);
+    for (auto* v : {&h.wall_first, &h.wall_cnt}) col(*v, CELLS);
+    for (auto* v : {&h.cell_cx, &h.cell_cy}) col(*v, CELLS);
+    for (auto* v : {&h.csr_off, &h.term_off, &h.grp_off, &h.mem_off}) col(*v, OFFSETS);
+    col(h.csr_face, CSR);
+    col(h.term_src, TERMS);
+    col(h.term_wx, TERMS);
+    col(h.term_wy, TERMS);
+    col(h.grp_mem, GROUPS);
+    col(h.grp_w, GROUPS);
+    col(h.mem_grp, MEMBERS);
+    for (auto* v : {&h.face_l, &h.face_r, &h.face_info}) col(*v, FACES);
+    for (auto* v : {&h.face_len, &h.face_dx, &h.face_dy, &h.face_ex, &h.face_ey, &h.face_dist}) col(*v, FACES);
+    for (auto* v : {&h.wall_owner, &h.wall_donor}) col(*v, WALLS);
+    for (auto* v : {&h.wall_nx, &h.wall_ny, &h.wall_len, &h.wall_ddist}) col(*v, WALLS);
+    for (auto* v : {&h.wall_mx, &h.wall_my}) col(*v, WALLS);
     size_t geo = 0;
-    geo += bytes_of(h.desc) + bytes_of(h.cell_block_mesh) + bytes_of(h.face_block_mesh);
-    for (auto* v : {&h.vol, &h.inv_vol, &h.dt_area, &h.wall_dist, &h.term_wx, &h.term_wy, &h.grp_w, &h.mem_inv,
-                    &h.face_len, &h.face_dx, &h.face_dy, &h.face_ex, &h.face_ey, &h.face_dist, &h.wall_nx, &h.wall_ny,
-                    &h.wall_len, &h.wall_ddist})
-      geo += bytes_of(*v);
-    for (auto* v : {&h.csr_off, &h.csr_face, &h.wall_first, &h.wall_cnt, &h.term_off, &h.term_src, &h.grp_off,
-                    &h.grp_mem, &h.mem_off, &h.mem_grp, &h.face_l, &h.face_r, &h.face_info, &h.wall_owner,
-                    &h.wall_donor})
-      geo += bytes_of(*v);
-    const int nv = any_sa_ ? 5 : 4;
-    const size_t ctrl_bytes = (in.size() * sizeof(Control) + 255) / 256 * 256;
-    // Work arrays: U[2], U2, R0 (nv planes each), W (4), PHI, GRAD (4), MU, EDDY, SGRAD (2),
-    // SDIFF, SSRC, SRATE, QHAT (4), CELL_DT, DRATE, DGROUP; faces FLUX (4), SPEED, FCOEF, SFLUX,
-    // SFCOEF; walls WFLUX (2), WCOEF, SWFLUX, SWCOEF.
-    const size_t cell_planes = 4 * nv + 4 + 1 + 4 + 2 + 2 + 3 + 4 + 3;
-    const size_t work = cell_planes * cplane + 8 * fplane + 5 * wplane;
-    slab_bytes_ = geo + ctrl_bytes + work;
+    for (const Column& c : columns_) geo += align256(c.elem * c.count);
+    const int nv = features_.sa ? 5 : 4;
+    const size_t nm = slots_.size();
+    const SlabPlan plan = plan_slab<T>(totals_, nv);
+    if (geo != plan.geometry) return "AMR arena size bookkeeping";
+    const int nc = std::max(nc_, 1), nf = std::max(nf_, 1), nw = std::max(nw_, 1);
+    const size_t cplane = plan.cplane, dplane = plan.dplane, fplane = plan.fplane, wplane = plan.wplane;
+    const size_t mon_cell = plan.mon_cell, mon_wall = plan.mon_wall;
+    slab_bytes_ = plan.total;
     geometry_bytes_ = geo;
-    state_bytes_ = static_cast<uint64_t>(nv) * nc * sizeof(T);
+    state_bytes_ = static_cast<uint64_t>(nv) * nc * sizeof(double);
     char* base = nullptr;
     if (hipMalloc(reinterpret_cast<void**>(&base), slab_bytes_) != hipSuccess) return "AMR hipMalloc of the arena failed";
     slab_ = base;
     size_t offset = 0;
-    std::vector<char> staging(geo + ctrl_bytes, 0);
-    const auto put = [&](const auto& vec) {
-      using V = typename std::decay_t<decltype(vec)>::value_type;
-      V* dst = reinterpret_cast<V*>(base + offset);
-      std::memcpy(staging.data() + offset, vec.data(), vec.size() * sizeof(V));
-      offset += bytes_of(vec);
-      return dst;
-    };
-    ArenaPtrs<T>& a = ptrs_;
-    a.desc = put(h.desc);
-    a.cell_block_mesh = put(h.cell_block_mesh);
-    a.face_block_mesh = put(h.face_block_mesh);
-    a.vol = put(h.vol);
-    a.inv_vol = put(h.inv_vol);
-    a.dt_area = put(h.dt_area);
-    a.wall_dist = put(h.wall_dist);
-    a.term_wx = put(h.term_wx);
-    a.term_wy = put(h.term_wy);
-    a.grp_w = put(h.grp_w);
-    a.mem_inv = put(h.mem_inv);
-    a.face_len = put(h.face_len);
-    a.face_dx = put(h.face_dx);
-    a.face_dy = put(h.face_dy);
-    a.face_ex = put(h.face_ex);
-    a.face_ey = put(h.face_ey);
-    a.face_dist = put(h.face_dist);
-    a.wall_nx = put(h.wall_nx);
-    a.wall_ny = put(h.wall_ny);
-    a.wall_len = put(h.wall_len);
-    a.wall_ddist = put(h.wall_ddist);
-    a.csr_off = put(h.csr_off);
-    a.csr_face = put(h.csr_face);
-    a.wall_first = put(h.wall_first);
-    a.wall_cnt = put(h.wall_cnt);
-    a.term_off = put(h.term_off);
-    a.term_src = put(h.term_src);
-    a.grp_off = put(h.grp_off);
-    a.grp_mem = put(h.grp_mem);
-    a.mem_off = put(h.mem_off);
-    a.mem_grp = put(h.mem_grp);
-    a.face_l = put(h.face_l);
-    a.face_r = put(h.face_r);
-    a.face_info = put(h.face_info);
-    a.wall_owner = put(h.wall_owner);
-    a.wall_donor = put(h.wall_donor);
-    // Controls: initial time and counters of every mesh; finished until the first target.
-    std::vector<Control> ctrl(in.size());
-    for (size_t m = 0; m < in.size(); ++m) {
-      Control& c = ctrl[m];
-      std::memset(&c, 0, sizeof(c));
-      c.time = in[m].meta.time;
-      c.target = in[m].meta.time;
-      c.steps = in[m].meta.steps;
-      c.rejected = in[m].meta.rejected;
-      c.finished = 1;
-      c.dt_min = 0x7ff0000000000000ull;
+    for (Column& c : columns_) {
+      c.dev = base + offset;
+      offset += align256(c.elem * c.count);
     }
-    a.ctrl = put(ctrl);
-    const size_t uploaded = offset;
+    ArenaPtrs<T>& a = ptrs_;
+    size_t i = 0;
+    const auto next = [&](auto* typed) { return reinterpret_cast<decltype(typed)>(columns_[i++].dev); };
+    a.desc = next(static_cast<const MeshDesc<T>*>(nullptr));
+    a.cell_block_mesh = next(static_cast<const int*>(nullptr));
+    a.face_block_mesh = next(static_cast<const int*>(nullptr));
+    a.vol = next(static_cast<const T*>(nullptr));
+    a.inv_vol = next(static_cast<const T*>(nullptr));
+    a.dt_area = next(static_cast<const T*>(nullptr));
+    a.wall_dist = next(static_cast<const T*>(nullptr));
+    a.mem_inv = next(static_cast<const T*>(nullptr));
+    a.wall_first = next(static_cast<const int*>(nullptr));
+    a.wall_cnt = next(static_cast<const int*>(nullptr));
+    a.cell_cx = next(static_cast<const double*>(nullptr));
+    a.cell_cy = next(static_cast<const double*>(nullptr));
+    a.csr_off = next(static_cast<const int*>(nullptr));
+    a.term_off = next(static_cast<const int*>(nullptr));
+    a.grp_off = next(static_cast<const int*>(nullptr));
+    a.mem_off = next(static_cast<const int*>(nullptr));
+    a.csr_face = next(static_cast<const int*>(nullptr));
+    a.term_src = next(static_cast<const int*>(nullptr));
+    a.term_wx = next(static_cast<const T*>(nullptr));
+    a.term_wy = next(static_cast<const T*>(nullptr));
+    a.grp_mem = next(static_cast<const int*>(nullptr));
+    a.grp_w = next(static_cast<const T*>(nullptr));
+    a.mem_grp = next(static_cast<const int*>(nullptr));
+    a.face_l = next(static_cast<const int*>(nullptr));
+    a.face_r = next(static_cast<const int*>(nullptr));
+    a.face_info = next(static_cast<const int*>(nullptr));
+    a.face_len = next(static_cast<const T*>(nullptr));
+    a.face_dx = next(static_cast<const T*>(nullptr));
+    a.face_dy = next(static_cast<const T*>(nullptr));
+    a.face_ex = next(static_cast<const T*>(nullptr));
+    a.face_ey = next(static_cast<const T*>(nullptr));
+    a.face_dist = next(static_cast<const T*>(nullptr));
+    a.wall_owner = next(static_cast<const int*>(nullptr));
+    a.wall_donor = next(static_cast<const int*>(nullptr));
+    a.wall_nx = next(static_cast<const T*>(nullptr));
+    a.wall_ny = next(static_cast<const T*>(nullptr));
+    a.wall_len = next(static_cast<const T*>(nullptr));
+    a.wall_ddist = next(static_cast<const T*>(nullptr));
+    a.wall_mx = next(static_cast<const double*>(nullptr));
+    a.wall_my = next(static_cast<const double*>(nullptr));
+    if (i != columns_.size()) return "AMR arena column bookkeeping";
+    a.ctrl = reinterpret_cast<Control*>(base + offset);
+    offset += align256(nm * sizeof(Control));
+    targets_ = reinterpret_cast<double*>(base + offset);
+    offset += align256(nm * sizeof(double));
+    maxsteps_ = reinterpret_cast<long long*>(base + offset);
+    offset += align256(nm * sizeof(long long));
+    frames_ = reinterpret_cast<double*>(base + offset);
+    a.mon_frame = frames_;
+    offset += align256(nm * 6 * sizeof(double));
+    const size_t tables_end = offset;
+    const auto dplanes = [&](int count) {
+      double* p = reinterpret_cast<double*>(base + offset);
+      offset += dplane * count;
+      return p;
+    };
     const auto plane = [&](size_t bytes, int count) {
       T* p = reinterpret_cast<T*>(base + offset);
       offset += bytes * count;
       return p;
     };
-    a.U[0] = plane(cplane, nv);
-    a.U[1] = plane(cplane, nv);
-    a.U2 = plane(cplane, nv);
+    a.U[0] = dplanes(nv);
+    a.U[1] = dplanes(nv);
+    a.U2 = dplanes(nv);
+    a.QHAT = dplanes(4);
     a.R0 = plane(cplane, nv);
-    // Kernels index planes as v * nc + k; nc is a multiple of the tile, so nc * sizeof(T)
-    // is already a multiple of 256 and equals the reserved plane size.
     a.W = plane(cplane, 4);
     a.PHI = plane(cplane, 1);
     a.GRAD = plane(cplane, 4);
@@ -506,7 +617,6 @@ class DeviceArena final : public Arena {
     a.SDIFF = plane(cplane, 1);
     a.SSRC = plane(cplane, 1);
     a.SRATE = plane(cplane, 1);
-    a.QHAT = plane(cplane, 4);
     a.CELL_DT = plane(cplane, 1);
     a.DRATE = plane(cplane, 1);
     a.DGROUP = plane(cplane, 1);
@@ -519,8 +629,12 @@ class DeviceArena final : public Arena {
     a.WCOEF = plane(wplane, 1);
     a.SWFLUX = plane(wplane, 1);
     a.SWCOEF = plane(wplane, 1);
-    if (offset > slab_bytes_) return "AMR arena layout overflow";
-    a.n_meshes = static_cast<int>(in.size());
+    a.MON_CELL = reinterpret_cast<double*>(base + offset);
+    offset += mon_cell;
+    a.MON_WALL = reinterpret_cast<double*>(base + offset);
+    offset += mon_wall;
+    if (offset != slab_bytes_) return "AMR arena layout overflow";
+    a.n_meshes = static_cast<int>(nm);
     a.nc = nc;
     a.nf = nf;
     a.nw = nw;
@@ -528,39 +642,268 @@ class DeviceArena final : public Arena {
     a.n_cell_blocks = n_cell_blocks_;
     a.n_face_blocks = n_face_blocks_;
     a.n_wall_blocks = n_wall_blocks_;
-    a.any_laminar = any_laminar_;
-    a.any_sa = any_sa_;
-    a.any_small = any_small_;
-    a.any_visc_group = any_visc_group_;
-    if (hipMemcpyAsync(base, staging.data(), uploaded, hipMemcpyHostToDevice, stream) != hipSuccess ||
-        hipStreamSynchronize(stream) != hipSuccess)
-      return "AMR geometry H2D failed";
+    a.any_laminar = features_.laminar;
+    a.any_sa = features_.sa;
+    a.any_small = features_.small;
+    a.any_visc_group = features_.visc_group;
+    for (const Column& c : columns_)
+      if (hipMemcpyAsync(c.dev, c.host, c.elem * c.count, hipMemcpyHostToDevice, stream) != hipSuccess)
+        return "AMR geometry H2D failed";
+    std::vector<Control> ctrl(nm);
+    for (size_t m = 0; m < nm; ++m) ctrl[m] = initial_control(m < metas.size() ? metas[m] : amr_state_meta{0, 0, 0});
+    if (hipMemcpyAsync(a.ctrl, ctrl.data(), nm * sizeof(Control), hipMemcpyHostToDevice, stream) != hipSuccess)
+      return "AMR control H2D failed";
     // Zero the work arrays once so padding lanes never hold garbage.
-    if (hipMemsetAsync(base + uploaded, 0, slab_bytes_ - uploaded, stream) != hipSuccess ||
+    if (hipMemsetAsync(base + tables_end, 0, slab_bytes_ - tables_end, stream) != hipSuccess ||
         hipStreamSynchronize(stream) != hipSuccess)
       return "AMR work clear failed";
-    size_t max_state = 0;
-    for (const MeshSlot& s : slots_) max_state = std::max(max_state, static_cast<size_t>(s.nv) * s.n_cells);
-    if (hipHostMalloc(&pinned_, std::max<size_t>(max_state, 1) * sizeof(T), hipHostMallocDefault) != hipSuccess)
+    size_t max_state = 1;
+    for (const MeshSlot& s : slots_) max_state = std::max(max_state, static_cast<size_t>(s.nv) * s.cap.cells);
+    if (hipHostMalloc(&pinned_, max_state * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc(reinterpret_cast<void**>(&pinned_frames_), nm * 6 * sizeof(double), hipHostMallocDefault) !=
+            hipSuccess)
       return "AMR pinned staging allocation failed";
-    pinned_bytes_ = std::max<size_t>(max_state, 1) * sizeof(T);
-    for (size_t m = 0; m < in.size(); ++m)
-      if (write_state(static_cast<int>(m), 0, in[m].u, stream) != hipSuccess) return "AMR initial state H2D failed";
     return "";
   }
 
+  // Upload the tables of slot m, its descriptor and a fresh control.
+  std::string upload_slot(int m, const amr_state_meta& meta, hipStream_t stream) {
+    const MeshSlot& s = slots_[m];
+    for (const Column& c : columns_) {
+      size_t begin = 0, count = 0;
+      switch (c.space) {
+        case CELLS: begin = s.cell_begin, count = s.cap.cells; break;
+        case OFFSETS: begin = s.cell_begin, count = s.cap.cells; break;
+        case FACES: begin = s.face_begin, count = s.cap.faces; break;
+        case WALLS: begin = s.wall_begin, count = s.cap.walls; break;
+        case CSR: begin = s.csr_begin, count = s.cap.csr; break;
+        case TERMS: begin = s.term_begin, count = s.cap.terms; break;
+        case GROUPS: begin = s.grp_begin, count = s.cap.groups; break;
+        case MEMBERS: begin = s.mem_begin, count = s.cap.members; break;
+        case TABLE: begin = 0, count = 0; break;
+      }
+      if (!count) continue;
+      if (hipMemcpyAsync(static_cast<char*>(c.dev) + begin * c.elem, static_cast<const char*>(c.host) + begin * c.elem,
+                         count * c.elem, hipMemcpyHostToDevice, stream) != hipSuccess)
+        return "AMR slot H2D failed";
+    }
+    if (hipMemcpyAsync(const_cast<MeshDesc<T>*>(ptrs_.desc) + m, &h_.desc[m], sizeof(MeshDesc<T>), hipMemcpyHostToDevice,
+                       stream) != hipSuccess)
+      return "AMR descriptor H2D failed";
+    const Control c = initial_control(meta);
+    if (hipMemcpyAsync(ptrs_.ctrl + m, &c, sizeof(Control), hipMemcpyHostToDevice, stream) != hipSuccess ||
+        hipStreamSynchronize(stream) != hipSuccess)
+      return "AMR slot control H2D failed";
+    return "";
+  }
+
+  std::string admit(int m, const MeshInput& in, hipStream_t stream) override {
+    if (m < 0 || m >= static_cast<int>(slots_.size())) return "slot out of range";
+    std::string e = validate_input(in);
+    if (e.empty()) e = fits(m, *in.view);
+    if (!e.empty()) return e;  // the slot is untouched
+    e = fill_slot(m, in);
+    if (e.empty()) e = check_slot(m);
+    if (e.empty()) e = upload_slot(m, in.meta, stream);
+    if (e.empty() && write_state(m, 0, in.u, stream) != hipSuccess) e = "AMR state H2D failed";
+    if (!e.empty()) release(m, stream);
+    return e;
+  }
+
+  std::string release(int m, hipStream_t stream) override {
+    if (m < 0 || m >= static_cast<int>(slots_.size())) return "slot out of range";
+    clear_slot(m);
+    return upload_slot(m, amr_state_meta{0, 0, 0}, stream);
+  }
+
+  int precision() const override { return sizeof(T) == 8 ? 64 : 32; }
+  const std::vector<MeshSlot>& slots() const override { return slots_; }
+  const ArenaFeatures& features() const override { return features_; }
+  uint64_t device_bytes() const override { return slab_bytes_; }
+  uint64_t geometry_bytes() const override { return geometry_bytes_; }
+  uint64_t state_bytes() const override { return state_bytes_; }
+  int launch_trial(hipStream_t s, LaunchHook* hook) override { return hipamr::launch_trial<T>(ptrs_, s, hook); }
+  int launch_residual(hipStream_t s, LaunchHook* hook) override { return hipamr::launch_residual<T>(ptrs_, s, hook); }
+  Control* control() override { return ptrs_.ctrl; }
+  double* target_array() override { return targets_; }
+  long long* maxsteps_array() override { return maxsteps_; }
+  hipError_t launch_monitor(hipStream_t s, const double* frames) override {
+    // The previous monitor's copy has completed: read_monitor synchronises the stream.
+    std::memcpy(pinned_frames_, frames, slots_.size() * 6 * sizeof(double));
+    const hipError_t e =
+        hipMemcpyAsync(frames_, pinned_frames_, slots_.size() * 6 * sizeof(double), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    hipamr::launch_monitor<T>(ptrs_, s);
+    return hipGetLastError();
+  }
+  hipError_t read_monitor(std::vector<double>* cells, std::vector<double>* walls, hipStream_t s) override {
+    cells->resize(static_cast<size_t>(n_cell_blocks_) * 6);
+    walls->resize(static_cast<size_t>(n_wall_blocks_) * 3);
+    hipError_t e = hipMemcpyAsync(cells->data(), ptrs_.MON_CELL, cells->size() * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && !walls->empty())
+      e = hipMemcpyAsync(walls->data(), ptrs_.MON_WALL, walls->size() * sizeof(double), hipMemcpyDeviceToHost, s);
+    return e == hipSuccess ? hipStreamSynchronize(s) : e;
+  }
+  int n_meshes() const override { return static_cast<int>(slots_.size()); }
+  int n_cell_blocks() const override { return n_cell_blocks_; }
+  int n_face_blocks() const override { return n_face_blocks_; }
+  int n_wall_blocks() const override { return n_wall_blocks_; }
+  int tile() const override { return tile_; }
+
+  hipError_t read_state(int m, int buffer, double* out, hipStream_t s) override {
+    const MeshSlot& sl = slots_[m];
+    const size_t count = static_cast<size_t>(sl.nv) * sl.n_cells;
+    if (sl.n_cells == 0) return hipSuccess;
+    if (buffer == 2) {  // residual R0 (T planes)
+      hipError_t e = hipMemcpy2DAsync(pinned_, sl.n_cells * sizeof(T), ptrs_.R0 + sl.cell_begin, nc_ * sizeof(T),
+                                      sl.n_cells * sizeof(T), sl.nv, hipMemcpyDeviceToHost, s);
+      if (e != hipSuccess || (e = hipStreamSynchronize(s)) != hipSuccess) return e;
+      const T* p = static_cast<const T*>(pinned_);
+      for (size_t i = 0; i < count; ++i) out[i] = static_cast<double>(p[i]);
+      return hipSuccess;
+    }
+    hipError_t e = hipMemcpy2DAsync(pinned_, sl.n_cells * sizeof(double), ptrs_.U[buffer] + sl.cell_begin,
+                                    nc_ * sizeof(double), sl.n_cells * sizeof(double), sl.nv, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess || (e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    std::memcpy(out, pinned_, count * sizeof(double));
+    return hipSuccess;
+  }
+
+  hipError_t write_state(int m, int buffer, const double* in, hipStream_t s) override {
+    const MeshSlot& sl = slots_[m];
+    if (sl.n_cells == 0) return hipSuccess;
+    std::memcpy(pinned_, in, static_cast<size_t>(sl.nv) * sl.n_cells * sizeof(double));
+    hipError_t e = hipMemcpy2DAsync(ptrs_.U[buffer] + sl.cell_begin, nc_ * sizeof(double), pinned_,
+                                    sl.n_cells * sizeof(double), sl.n_cells * sizeof(double), sl.nv,
+                                    hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+  }
+
+  double sa_nu_inf(int m) const override { return sa_nu_inf_[m]; }
+
+ private:
+  static Control initial_control(const amr_state_meta& meta) {
+    Control c;
+    std::memset(&c, 0, sizeof(c));
+    c.time = meta.time;
+    c.target = meta.time;
+    c.steps = meta.steps;
+    c.rejected = meta.rejected;
+    c.finished = 1;
+    c.dt_min = kInf;
+    return c;
+  }
+
+  void pad_cell(int g, int wb) {
+    HostArrays<T>& h = h_;
+    h.vol[g] = h.inv_vol[g] = h.dt_area[g] = h.wall_dist[g] = h.mem_inv[g] = T(1);
+    h.wall_first[g] = wb;
+    h.wall_cnt[g] = 0;
+    h.cell_cx[g] = h.cell_cy[g] = 0.0;
+  }
+  void pad_face(int g, int cb) {
+    HostArrays<T>& h = h_;
+    h.face_l[g] = h.face_r[g] = cb;
+    h.face_info[g] = 0;
+    h.face_len[g] = h.face_dx[g] = h.face_dy[g] = h.face_ex[g] = h.face_ey[g] = T(0);
+    h.face_dist[g] = T(1);
+  }
+  void pad_wall(int g, int cb) {
+    HostArrays<T>& h = h_;
+    h.wall_owner[g] = h.wall_donor[g] = cb;
+    h.wall_nx[g] = h.wall_ny[g] = h.wall_len[g] = T(0);
+    h.wall_ddist[g] = T(1);
+    h.wall_mx[g] = h.wall_my[g] = 0.0;
+  }
+  void clear_slot(int m) {
+    MeshSlot& s = slots_[m];
+    HostArrays<T>& h = h_;
+    for (int k = 0; k < s.cap.cells; ++k) {
+      const int g = s.cell_begin + k;
+      h.csr_off[g] = s.csr_begin;
+      h.term_off[g] = s.term_begin;
+      h.grp_off[g] = s.grp_begin;
+      h.mem_off[g] = s.mem_begin;
+      pad_cell(g, s.wall_begin);
+    }
+    for (int q = 0; q < s.cap.faces; ++q) pad_face(s.face_begin + q, s.cell_begin);
+    for (int j = 0; j < s.cap.walls; ++j) pad_wall(s.wall_begin + j, s.cell_begin);
+    MeshDesc<T> d{};
+    d.cell_begin = s.cell_begin;
+    d.face_begin = s.face_begin;
+    d.wall_begin = s.wall_begin;
+    d.first_cell_block = s.cell_begin / tile_;
+    d.nv = features_.sa ? 5 : 4;
+    d.gamma = T(1.4);
+    h.desc[m] = d;
+    s.n_cells = s.n_faces = s.n_walls = 0;
+    sa_nu_inf_[m] = 0.0;
+  }
+
+  HostArrays<T> h_;
+  std::vector<Column> columns_;
   ArenaPtrs<T> ptrs_{};
   std::vector<MeshSlot> slots_;
+  std::vector<double> sa_nu_inf_;
+  ArenaFeatures features_{};
   char* slab_ = nullptr;
   void* pinned_ = nullptr;
-  size_t pinned_bytes_ = 0;
+  double* targets_ = nullptr;
+  long long* maxsteps_ = nullptr;
+  double* frames_ = nullptr;
+  double* pinned_frames_ = nullptr;
+  Totals totals_{};
   uint64_t slab_bytes_ = 0, geometry_bytes_ = 0, state_bytes_ = 0;
   int nc_ = 0, nf_ = 0, nw_ = 0, tile_ = 256;
   int n_cell_blocks_ = 0, n_face_blocks_ = 0, n_wall_blocks_ = 0;
-  int any_laminar_ = 0, any_sa_ = 0, any_small_ = 0, any_visc_group_ = 0;
 };
 
+template <class T>
+std::unique_ptr<Arena> build_arena(const std::vector<SlotCapacity>& caps, const ArenaFeatures& f, int tile,
+                                   const std::vector<MeshInput>* inputs, hipStream_t stream, std::string* error) {
+  auto a = std::make_unique<DeviceArena<T>>();
+  std::string e = a->layout(caps, f, tile);
+  std::vector<amr_state_meta> metas;
+  if (e.empty() && inputs)
+    for (size_t m = 0; m < inputs->size() && e.empty(); ++m) {
+      e = a->fill_slot(static_cast<int>(m), (*inputs)[m]);
+      metas.push_back((*inputs)[m].meta);
+    }
+  if (e.empty()) e = a->check_tables();
+  if (e.empty()) e = a->upload_all(metas, stream);
+  if (e.empty() && inputs)
+    for (size_t m = 0; m < inputs->size() && e.empty(); ++m)
+      if (a->write_state(static_cast<int>(m), 0, (*inputs)[m].u, stream) != hipSuccess) e = "AMR initial state H2D failed";
+  if (!e.empty()) {
+    *error = e;
+    return nullptr;
+  }
+  return a;
+}
+
 }  // namespace
+
+SlotCapacity capacity_of(const amr_mesh_view& v, int tile) {
+  SlotCapacity c{};
+  c.cells = round_up(static_cast<int>(v.n_cells) + 1, tile);
+  c.faces = round_up(static_cast<int>(v.n_faces), tile);
+  c.walls = round_up(static_cast<int>(v.n_walls), tile);
+  c.csr = static_cast<int>(v.face_offsets[v.n_cells]);
+  c.terms = v.visc_offsets ? static_cast<int>(v.visc_offsets[v.n_cells]) : 0;
+  c.groups = static_cast<int>(v.group_offsets[v.n_cells]);
+  c.members = static_cast<int>(v.member_offsets[v.n_cells]);
+  return c;
+}
+
+ArenaFeatures features_of(const amr_mesh_view& v) {
+  ArenaFeatures f{};
+  f.laminar = v.transport.model == 1;
+  f.sa = v.transport.sa_model != 0;
+  f.small = v.small_cells > 0;
+  f.visc_group = f.laminar && v.transport.stabilization != 0 && f.small;
+  return f;
+}
 
 std::string validate_input(const MeshInput& in) {
   const amr_mesh_view* v = in.view;
@@ -573,8 +916,8 @@ std::string validate_input(const MeshInput& in) {
     return "invalid AMR GPU configuration";
   for (int side = 0; side < 4; ++side)
     if (!side_valid(v->boundaries[side], p.bc[side], side)) return "invalid AMR GPU boundary configuration";
-  if (v->n_cells > (1u << 30) || v->n_faces > (1u << 30) || v->n_walls > (1u << 30))
-    return "AMR mesh exceeds 2^30 elements";
+  if (v->n_cells > (1u << 29) || v->n_faces > (1u << 29) || v->n_walls > (1u << 29))
+    return "AMR mesh exceeds 2^29 elements";
   return "";
 }
 
@@ -584,37 +927,68 @@ std::unique_ptr<Arena> make_arena(const std::vector<MeshInput>& inputs, int prec
     *error = "tile must be 64, 128 or 256";
     return nullptr;
   }
-  size_t nc = 0, nf = 0, nw = 0;
-  for (const MeshInput& m : inputs) {
-    nc += round_up(static_cast<int>(m.view->n_cells), tile);
-    nf += round_up(static_cast<int>(m.view->n_faces), tile);
-    nw += round_up(static_cast<int>(m.view->n_walls), tile);
+  std::vector<SlotCapacity> caps;
+  ArenaFeatures f{};
+  for (const MeshInput& in : inputs) {
+    const std::string e = validate_input(in);
+    if (!e.empty()) {
+      *error = e;
+      return nullptr;
+    }
+    caps.push_back(capacity_of(*in.view, tile));
+    const ArenaFeatures g = features_of(*in.view);
+    f.laminar |= g.laminar;
+    f.sa |= g.sa;
+    f.small |= g.small;
+    f.visc_group |= g.visc_group;
   }
-  if (nc >= (1u << 30) || nf >= (1u << 30) || nw >= (1u << 30)) {
-    *error = "arena exceeds 2^30 cells, faces or walls";
-    return nullptr;
-  }
-  std::string message;
-  if (precision == 64) {
-    auto a = std::make_unique<DeviceArena<double>>();
-    message = a->build(inputs, tile, stream);
-    if (message.empty()) return a;
-  } else if (precision == 32) {
-    auto a = std::make_unique<DeviceArena<float>>();
-    message = a->build(inputs, tile, stream);
-    if (message.empty()) return a;
-  } else {
-    message = "precision must be 64 or 32";
-  }
-  *error = message;
+  if (precision == 64) return build_arena<double>(caps, f, tile, &inputs, stream, error);
+  if (precision == 32) return build_arena<float>(caps, f, tile, &inputs, stream, error);
+  *error = "precision must be 64 or 32";
   return nullptr;
 }
 
+std::unique_ptr<Arena> make_slot_arena(const std::vector<SlotCapacity>& caps, const ArenaFeatures& features,
+                                       int precision, int tile, hipStream_t stream, std::string* error) {
+  if (tile != 64 && tile != 128 && tile != 256) {
+    *error = "tile must be 64, 128 or 256";
+    return nullptr;
+  }
+  if (precision == 64) return build_arena<double>(caps, features, tile, nullptr, stream, error);
+  if (precision == 32) return build_arena<float>(caps, features, tile, nullptr, stream, error);
+  *error = "precision must be 64 or 32";
+  return nullptr;
+}
+
+uint64_t arena_bytes(const std::vector<SlotCapacity>& caps, const ArenaFeatures& features, int precision, int tile) {
+  const Totals t = totals_of(caps, tile);
+  const int nv = features.sa ? 5 : 4;
+  return precision == 32 ? plan_slab<float>(t, nv).total : plan_slab<double>(t, nv).total;
+}
+
 std::string check_inputs_layout(const std::vector<MeshInput>& inputs, int tile) {
+  std::vector<SlotCapacity> caps;
+  ArenaFeatures f{1, 0, 1, 1};
+  for (const MeshInput& in : inputs) {
+    caps.push_back(capacity_of(*in.view, tile));
+    f.sa = in.view->transport.sa_model != 0;
+  }
   DeviceArena<double> arena;
-  HostArrays<double> h;
-  std::string message = arena.layout(inputs, tile, &h);
-  return message.empty() ? arena.check_layout(h) : message;
+  std::string e = arena.layout(caps, f, tile);
+  for (size_t m = 0; m < inputs.size() && e.empty(); ++m) e = arena.fill_slot(static_cast<int>(m), inputs[m]);
+  return e.empty() ? arena.check_tables() : e;
+}
+
+void sum_monitor(const Arena& arena, int m, const std::vector<double>& cells, const std::vector<double>& walls,
+                 double* out) {
+  const MeshSlot& s = arena.slots()[m];
+  const int tile = arena.tile();
+  long double acc[9] = {};
+  for (int b = s.cell_begin / tile; b < (s.cell_begin + s.cap.cells) / tile; ++b)
+    for (int i = 0; i < 6; ++i) acc[i] += cells[6 * b + i];
+  for (int b = s.wall_begin / tile; b < (s.wall_begin + s.cap.walls) / tile; ++b)
+    for (int i = 0; i < 3; ++i) acc[6 + i] += walls[3 * b + i];
+  for (int i = 0; i < 9; ++i) out[i] = static_cast<double>(acc[i]);
 }
 
 }  // namespace hipamr
@@ -623,14 +997,14 @@ extern "C" int amr_hip_check_layout(const amr_mesh_view* views, const amr_params
                                     char* err, size_t err_len) {
   std::string message;
   if (!views || !params || n < 1 || (tile != 64 && tile != 128 && tile != 256)) message = "invalid layout check arguments";
-  std::vector<std::vector<double>> dummy;
+  std::vector<std::vector<double>> dummy(message.empty() ? n : 0);
   std::vector<hipamr::MeshInput> inputs;
   for (int32_t m = 0; message.empty() && m < n; ++m) {
     hipamr::MeshInput in{};
     in.view = views + m;
     in.params = params[m];
-    dummy.emplace_back((views[m].transport.sa_model ? 5 : 4) * views[m].n_cells, 1.0);
-    in.u = dummy.back().data();
+    dummy[m].assign((views[m].transport.sa_model ? 5 : 4) * views[m].n_cells, 1.0);
+    in.u = dummy[m].data();
     in.meta = amr_state_meta{0, 0, 0};
     message = hipamr::validate_input(in);
     inputs.push_back(in);

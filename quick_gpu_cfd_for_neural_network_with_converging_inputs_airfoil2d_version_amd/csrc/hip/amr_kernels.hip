@@ -36,23 +36,25 @@ __device__ __forceinline__ bool pass_on(const Control* c, int pass) {
 enum : int { BAD_BASE = 1, BAD_STAGE = 2 };
 __device__ __forceinline__ void flag_bad(Control* c, int bit) { atomicOr(&c->bad, bit); }
 
-// U -> W = (rho, u, v, p) and nutilde; false if not admissible (reference primitive rules).
+// U -> W = (rho, u, v, p) and nutilde (computed in FP64 from the FP64 state, stored as T);
+// false if not admissible (reference primitive rules, src/static_amr2d.cpp:143-154).
 template <class T>
-__device__ __forceinline__ bool to_primitive(const T* u, int nc, int k, T gamma, bool sa, T* w, T* phi) {
-  const T rho = u[k], mx = u[nc + k], my = u[2 * nc + k], energy = u[3 * nc + k];
-  const T ir = T(1) / rho;
-  const T vx = mx * ir, vy = my * ir;
-  const T p = (gamma - T(1)) * (energy - T(0.5) * (mx * vx + my * vy));
-  bool ok = isfinite(rho) && rho > T(0) && isfinite(mx) && isfinite(my) && isfinite(energy) && isfinite(p) && p > T(0);
-  w[0] = rho;
-  w[1] = vx;
-  w[2] = vy;
-  w[3] = p;
+__device__ __forceinline__ bool to_primitive(const double* u, int nc, int k, T gamma, bool sa, T* w, T* phi) {
+  const double rho = u[k], mx = u[nc + k], my = u[2 * nc + k], energy = u[3 * nc + k];
+  const double ir = 1.0 / rho;
+  const double vx = mx * ir, vy = my * ir;
+  const double p = (static_cast<double>(gamma) - 1.0) * (energy - 0.5 * (mx * vx + my * vy));
+  bool ok = isfinite(rho) && rho > 0 && isfinite(mx) && isfinite(my) && isfinite(energy) && isfinite(p) && p > 0;
+  w[0] = static_cast<T>(rho);
+  w[1] = static_cast<T>(vx);
+  w[2] = static_cast<T>(vy);
+  w[3] = static_cast<T>(p);
   if (sa) {
-    *phi = u[4 * nc + k] * ir;
-    ok = ok && isfinite(*phi);
+    const double ph = u[4 * nc + k] * ir;
+    *phi = static_cast<T>(ph);
+    ok = ok && isfinite(ph);
   }
-  return ok;
+  return ok && w[0] > T(0) && w[3] > T(0) && isfinite(w[0]) && isfinite(w[3]);
 }
 
 template <class T>
@@ -447,17 +449,17 @@ __global__ __launch_bounds__(kMaxTile) void k_collect(ArenaPtrs<T> a) {
       local_dt = static_cast<double>(dt);
     } else {
       // SSP-RK3: u2 = 3/4 u + 1/4 (u1 + dt r)  or  u1 = 1/3 u + 2/3 (u2 + dt r).
-      const T dt = static_cast<T>(c->trial_dt);
+      const double dt = c->trial_dt;
       const int cur = c->cur;
-      const T* u = a.U[cur];
-      T* out = STAGE == 1 ? a.U2 : a.U[cur ^ 1];
-      const T* prev = STAGE == 1 ? a.U[cur ^ 1] : a.U2;
-      const T wu = STAGE == 1 ? T(0.75) : T(1.0 / 3.0), wp = STAGE == 1 ? T(0.25) : T(2.0 / 3.0);
-      out[k] = wu * u[k] + wp * (prev[k] + dt * r0);
-      out[nc + k] = wu * u[nc + k] + wp * (prev[nc + k] + dt * r1);
-      out[2 * nc + k] = wu * u[2 * nc + k] + wp * (prev[2 * nc + k] + dt * r2);
-      out[3 * nc + k] = wu * u[3 * nc + k] + wp * (prev[3 * nc + k] + dt * r3);
-      if (sa) out[4 * nc + k] = wu * u[4 * nc + k] + wp * (prev[4 * nc + k] + dt * r4);
+      const double* u = a.U[cur];
+      double* out = STAGE == 1 ? a.U2 : a.U[cur ^ 1];
+      const double* prev = STAGE == 1 ? a.U[cur ^ 1] : a.U2;
+      const double wu = STAGE == 1 ? 0.75 : 1.0 / 3.0, wp = STAGE == 1 ? 0.25 : 2.0 / 3.0;
+      out[k] = wu * u[k] + wp * (prev[k] + dt * static_cast<double>(r0));
+      out[nc + k] = wu * u[nc + k] + wp * (prev[nc + k] + dt * static_cast<double>(r1));
+      out[2 * nc + k] = wu * u[2 * nc + k] + wp * (prev[2 * nc + k] + dt * static_cast<double>(r2));
+      out[3 * nc + k] = wu * u[3 * nc + k] + wp * (prev[3 * nc + k] + dt * static_cast<double>(r3));
+      if (sa) out[4 * nc + k] = wu * u[4 * nc + k] + wp * (prev[4 * nc + k] + dt * static_cast<double>(r4));
       if (!d.small) {
         T w[4], phi = T(0);
         if (!to_primitive(out, nc, k, d.gamma, sa, w, &phi)) flag_bad(c, BAD_STAGE);
@@ -539,10 +541,9 @@ __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= d.cell_begin + d.n_cells) return;
   const int nc = a.nc, cur = c->cur;
-  const T* u = a.U[cur];
-  T* out = a.U[cur ^ 1];
-  const T t = static_cast<T>(dt);
-  for (int v = 0; v < d.nv; ++v) out[v * nc + k] = u[v * nc + k] + t * a.R0[v * nc + k];
+  const double* u = a.U[cur];
+  double* out = a.U[cur ^ 1];
+  for (int v = 0; v < d.nv; ++v) out[v * nc + k] = u[v * nc + k] + dt * static_cast<double>(a.R0[v * nc + k]);
   if (!d.small) {
     T w[4], phi = T(0);
     if (!to_primitive(out, nc, k, d.gamma, d.sa != 0, w, &phi)) flag_bad(c, BAD_STAGE);
@@ -562,11 +563,11 @@ __global__ __launch_bounds__(kMaxTile) void k_group(ArenaPtrs<T> a, int stage) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= d.cell_begin + d.n_cells) return;
   const int nc = a.nc;
-  const T* u = stage == 1 ? a.U2 : a.U[c->cur ^ 1];
-  T q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+  const double* u = stage == 1 ? a.U2 : a.U[c->cur ^ 1];
+  double q0 = 0, q1 = 0, q2 = 0, q3 = 0;
   for (int j = a.grp_off[g]; j < a.grp_off[g + 1]; ++j) {
     const int k = a.grp_mem[j];
-    const T w = a.grp_w[j];
+    const double w = static_cast<double>(a.grp_w[j]);
     q0 += w * u[k];
     q1 += w * u[nc + k];
     q2 += w * u[2 * nc + k];
@@ -588,8 +589,8 @@ __global__ __launch_bounds__(kMaxTile) void k_redist(ArenaPtrs<T> a, int stage) 
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= d.cell_begin + d.n_cells) return;
   const int nc = a.nc;
-  T* u = stage == 1 ? a.U2 : a.U[c->cur ^ 1];
-  T s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  double* u = stage == 1 ? a.U2 : a.U[c->cur ^ 1];
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
   for (int j = a.mem_off[k]; j < a.mem_off[k + 1]; ++j) {
     const int g = a.mem_grp[j];
     s0 += a.QHAT[g];
@@ -597,7 +598,7 @@ __global__ __launch_bounds__(kMaxTile) void k_redist(ArenaPtrs<T> a, int stage) 
     s2 += a.QHAT[2 * nc + g];
     s3 += a.QHAT[3 * nc + g];
   }
-  const T inv = a.mem_inv[k];
+  const double inv = static_cast<double>(a.mem_inv[k]);
   u[k] = s0 * inv;
   u[nc + k] = s1 * inv;
   u[2 * nc + k] = s2 * inv;
@@ -667,59 +668,133 @@ __global__ void k_target(Control* ctrl, int n, double target, long long maxsteps
   begin_trial(c);
 }
 
+// Sum over the block of several doubles (same wave64 + LDS pattern as block_min); valid in thread 0.
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N]) {
+  __shared__ double part[N][kMaxTile / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int i = 0; i < N; ++i) {
+    for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off, 64);
+    if (lane == 0) part[i][wave] = v[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int i = 0; i < N; ++i) {
+      double t = part[i][0];
+      for (int w = 1; w < static_cast<int>(blockDim.x >> 6); ++w) t += part[i][w];
+      v[i] = t;
+    }
+}
+
+// Steady monitor, cells: volume-weighted |R0| over the residual region (FP64 accumulation;
+// reference src/quick_airfoil.cpp:76-81).
+template <class T>
+__global__ __launch_bounds__(kMaxTile) void k_monitor_cells(ArenaPtrs<T> a) {
+  const int m = a.cell_block_mesh[blockIdx.x];
+  const MeshDesc<T>& d = a.desc[m];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  if (k < d.cell_begin + d.n_cells) {
+    const double* f = a.mon_frame + 6 * m;
+    const double x0 = f[0], x1 = f[1], y0 = f[2], y1 = f[3];
+    const double cx = a.cell_cx[k], cy = a.cell_cy[k];
+    if (!(cx < x0 || cx > x1 || cy < y0 || cy > y1)) {
+      const double vol = static_cast<double>(a.vol[k]);
+      acc[5] = vol;
+      for (int v = 0; v < 5; ++v)
+        if (v < d.nv) acc[v] = vol * fabs(static_cast<double>(a.R0[v * a.nc + k]));
+    }
+  }
+  block_sum(acc);
+  if (threadIdx.x == 0)
+    for (int i = 0; i < 6; ++i) a.MON_CELL[6 * blockIdx.x + i] = acc[i];
+}
+
+// Steady monitor, walls: pressure force minus viscous shear and the moment about (mx, my)
+// (reference src/quick_airfoil.cpp:83-85).
+template <class T>
+__global__ __launch_bounds__(kMaxTile) void k_monitor_walls(ArenaPtrs<T> a) {
+  const int b = blockIdx.x;
+  const int m = a.face_block_mesh[a.n_face_blocks + b];
+  const MeshDesc<T>& d = a.desc[m];
+  const int j = b * blockDim.x + threadIdx.x;
+  double acc[3] = {0, 0, 0};
+  if (j < d.wall_begin + d.n_walls) {
+    const double mx = a.mon_frame[6 * m + 4], my = a.mon_frame[6 * m + 5];
+    const int k = a.wall_owner[j];
+    const double p = static_cast<double>(a.W[3 * a.nc + k]), len = static_cast<double>(a.wall_len[j]);
+    double fx = p * static_cast<double>(a.wall_nx[j]) * len, fy = p * static_cast<double>(a.wall_ny[j]) * len;
+    if (d.laminar) {
+      fx -= static_cast<double>(a.WFLUX[j]);
+      fy -= static_cast<double>(a.WFLUX[a.nw + j]);
+    }
+    acc[0] = fx;
+    acc[1] = fy;
+    acc[2] = (a.wall_mx[j] - mx) * fy - (a.wall_my[j] - my) * fx;
+  }
+  block_sum(acc);
+  if (threadIdx.x == 0)
+    for (int i = 0; i < 3; ++i) a.MON_WALL[3 * b + i] = acc[i];
+}
+
 inline int blocks_of(int n, int tile) { return (n + tile - 1) / tile; }
 
 }  // namespace
 
+#define HIPAMR_LAUNCH(name, kernel, grid, ...)                      \
+  do {                                                               \
+    if (hook) hook->before(name);                                    \
+    hipLaunchKernelGGL(kernel, grid, tile, 0, s, __VA_ARGS__);       \
+    if (hook) hook->after(name);                                     \
+    ++launches;                                                      \
+  } while (0)
+
 template <class T>
-int launch_residual(const ArenaPtrs<T>& a, hipStream_t s) {
-  const dim3 tile(a.tile);
+int launch_residual(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
+  const dim3 tile(a.tile), cells(a.n_cell_blocks), faces(a.n_face_blocks + a.n_wall_blocks);
   int launches = 0;
-  hipLaunchKernelGGL(k_base_prim<T>, dim3(a.n_cell_blocks), tile, 0, s, a);
-  ++launches;
-  if (a.any_laminar) {
-    hipLaunchKernelGGL((k_grad<T, PASS_BASE>), dim3(a.n_cell_blocks), tile, 0, s, a);
-    ++launches;
-  }
-  hipLaunchKernelGGL((k_face<T, PASS_BASE>), dim3(a.n_face_blocks + a.n_wall_blocks), tile, 0, s, a);
-  hipLaunchKernelGGL((k_collect<T, 0>), dim3(a.n_cell_blocks), tile, 0, s, a);
-  launches += 2;
+  HIPAMR_LAUNCH("base_primitive", k_base_prim<T>, cells, a);
+  if (a.any_laminar) HIPAMR_LAUNCH("gradient_base", (k_grad<T, PASS_BASE>), cells, a);
+  HIPAMR_LAUNCH("face_base", (k_face<T, PASS_BASE>), faces, a);
+  HIPAMR_LAUNCH("collect_base", (k_collect<T, 0>), cells, a);
   if (a.any_visc_group) {
-    hipLaunchKernelGGL(k_diffusion_group<T>, dim3(a.n_cell_blocks), tile, 0, s, a);
-    hipLaunchKernelGGL(k_dt_group<T>, dim3(a.n_cell_blocks), tile, 0, s, a);
-    launches += 2;
+    HIPAMR_LAUNCH("diffusion_group", k_diffusion_group<T>, cells, a);
+    HIPAMR_LAUNCH("dt_group", k_dt_group<T>, cells, a);
   }
   return launches;
 }
 
 template <class T>
-int launch_trial(const ArenaPtrs<T>& a, hipStream_t s) {
+int launch_trial(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
   const dim3 tile(a.tile), cells(a.n_cell_blocks), faces(a.n_face_blocks + a.n_wall_blocks);
-  int launches = launch_residual(a, s);
-  hipLaunchKernelGGL(k_stage0<T>, cells, tile, 0, s, a);
-  ++launches;
+  int launches = launch_residual(a, s, hook);
+  HIPAMR_LAUNCH("stage0", k_stage0<T>, cells, a);
   if (a.any_small) {
-    hipLaunchKernelGGL(k_group<T>, cells, tile, 0, s, a, 0);
-    hipLaunchKernelGGL(k_redist<T>, cells, tile, 0, s, a, 0);
-    launches += 2;
+    HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 0);
+    HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 0);
   }
   for (int stage = 1; stage <= 2; ++stage) {
-    if (a.any_laminar) {
-      hipLaunchKernelGGL((k_grad<T, PASS_STAGE>), cells, tile, 0, s, a);
-      ++launches;
-    }
-    hipLaunchKernelGGL((k_face<T, PASS_STAGE>), faces, tile, 0, s, a);
-    if (stage == 1) hipLaunchKernelGGL((k_collect<T, 1>), cells, tile, 0, s, a);
-    else hipLaunchKernelGGL((k_collect<T, 2>), cells, tile, 0, s, a);
-    launches += 2;
+    if (a.any_laminar) HIPAMR_LAUNCH("gradient_stage", (k_grad<T, PASS_STAGE>), cells, a);
+    HIPAMR_LAUNCH("face_stage", (k_face<T, PASS_STAGE>), faces, a);
+    if (stage == 1) HIPAMR_LAUNCH("collect_rk1", (k_collect<T, 1>), cells, a);
+    else HIPAMR_LAUNCH("collect_rk2", (k_collect<T, 2>), cells, a);
     if (a.any_small) {
-      hipLaunchKernelGGL(k_group<T>, cells, tile, 0, s, a, stage);
-      hipLaunchKernelGGL(k_redist<T>, cells, tile, 0, s, a, stage);
-      launches += 2;
+      HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, stage);
+      HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, stage);
     }
   }
-  hipLaunchKernelGGL(k_commit, dim3(blocks_of(a.n_meshes, 256)), dim3(256), 0, s, a.ctrl, a.n_meshes);
-  return launches + 1;
+  {
+    const dim3 tile(256);
+    HIPAMR_LAUNCH("commit", k_commit, dim3(blocks_of(a.n_meshes, 256)), a.ctrl, a.n_meshes);
+  }
+  return launches;
+}
+#undef HIPAMR_LAUNCH
+
+template <class T>
+void launch_monitor(const ArenaPtrs<T>& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_monitor_cells<T>, dim3(a.n_cell_blocks), dim3(a.tile), 0, s, a);
+  if (a.n_wall_blocks > 0) hipLaunchKernelGGL(k_monitor_walls<T>, dim3(a.n_wall_blocks), dim3(a.tile), 0, s, a);
 }
 
 void launch_target(Control* ctrl, int n, double target, long long maxsteps, const double* targets,
@@ -728,9 +803,11 @@ void launch_target(Control* ctrl, int n, double target, long long maxsteps, cons
                      maxsteps_each);
 }
 
-template int launch_trial<double>(const ArenaPtrs<double>&, hipStream_t);
-template int launch_trial<float>(const ArenaPtrs<float>&, hipStream_t);
-template int launch_residual<double>(const ArenaPtrs<double>&, hipStream_t);
-template int launch_residual<float>(const ArenaPtrs<float>&, hipStream_t);
+template int launch_trial<double>(const ArenaPtrs<double>&, hipStream_t, LaunchHook*);
+template int launch_trial<float>(const ArenaPtrs<float>&, hipStream_t, LaunchHook*);
+template int launch_residual<double>(const ArenaPtrs<double>&, hipStream_t, LaunchHook*);
+template int launch_residual<float>(const ArenaPtrs<float>&, hipStream_t, LaunchHook*);
+template void launch_monitor<double>(const ArenaPtrs<double>&, hipStream_t);
+template void launch_monitor<float>(const ArenaPtrs<float>&, hipStream_t);
 
 }  // namespace hipamr

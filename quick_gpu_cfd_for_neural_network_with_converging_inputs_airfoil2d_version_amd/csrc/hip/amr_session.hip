@@ -5,6 +5,7 @@
 // poll of the 104-byte control record until the target time is reached (0), a fatal code is
 // raised (1, last accepted state retained) or the wall budget expires (2); download copies
 // the accepted state; stats reports the same counters as AmrGpuStats.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -14,6 +15,7 @@
 #include <vector>
 
 #include "amr_device.h"
+#include "amr_util.h"
 
 using hipamr::Arena;
 using hipamr::Control;
@@ -30,6 +32,7 @@ struct amr_session {
   Control* host_control = nullptr;  // pinned
   int device = 0;
   uint64_t n_cells = 0, nv = 0;
+  double sa_nu_inf = 0;
 
   ~amr_session() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -46,33 +49,15 @@ struct amr_session {
 namespace {
 
 using Clock = std::chrono::steady_clock;
-
-int fail(char* err, size_t len, const std::string& message) {
-  if (err && len) {
-    const size_t k = message.size() < len - 1 ? message.size() : len - 1;
-    std::memcpy(err, message.data(), k);
-    err[k] = 0;
-  }
-  return 1;
-}
-
-bool checked(hipError_t status, char* err, size_t len, const char* operation) {
-  if (status == hipSuccess) return true;
-  fail(err, len, std::string(operation) + ": " + hipGetErrorString(status));
-  return false;
-}
+using hipamr::checked;
+using hipamr::copy_sync;
+using hipamr::fail;
 
 void to_meta(const Control& c, amr_state_meta* meta) {
   if (!meta) return;
   meta->time = c.time;
   meta->steps = c.steps;
   meta->rejected = c.rejected;
-}
-
-// Synchronous copy ordered on the session stream.
-hipError_t copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t stream) {
-  const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, stream);
-  return e != hipSuccess ? e : hipStreamSynchronize(stream);
 }
 
 int launch_one(amr_session* s) {
@@ -115,6 +100,7 @@ int amr_hip_create(const amr_mesh_view* mesh, const amr_params* params, const do
   s->options = *options;
   s->n_cells = mesh->n_cells;
   s->nv = nv;
+  s->sa_nu_inf = mesh->transport.sa_nu_inf;
   if (!checked(hipGetDevice(&s->device), err, err_len, "AMR device")) return 1;
   if (!checked(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), err, err_len, "AMR stream")) return 1;
   if (options->event_timing && (!checked(hipEventCreate(&s->start), err, err_len, "AMR timing start") ||
@@ -132,9 +118,9 @@ int amr_hip_create(const amr_mesh_view* mesh, const amr_params* params, const do
       !checked(hipStreamSynchronize(s->stream), err, err_len, "AMR setup sync"))
     return 1;
   s->stats.allocated_bytes = s->arena->device_bytes();
-  s->stats.pinned_bytes = sizeof(Control) + nv * mesh->n_cells * (options->precision == 64 ? 8 : 4);
+  s->stats.pinned_bytes = sizeof(Control) + nv * mesh->n_cells * sizeof(double);
   s->stats.geometry_h2d_bytes = s->arena->geometry_bytes();
-  s->stats.state_h2d_bytes = nv * mesh->n_cells * (options->precision == 64 ? 8 : 4);
+  s->stats.state_h2d_bytes = nv * mesh->n_cells * sizeof(double);
   s->stats.control_h2d_bytes = sizeof(Control);
   if (options->graph) {
     if (!checked(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), err, err_len, "AMR graph begin"))
@@ -211,7 +197,7 @@ int amr_hip_download(amr_session* s, double* u, uint64_t u_len, amr_state_meta* 
       !checked(s->arena->read_state(0, s->host_control->cur, u, s->stream), err, err_len, "AMR state D2H"))
     return 1;
   to_meta(*s->host_control, state);
-  s->stats.state_d2h_bytes += s->nv * s->n_cells * (s->options.precision == 64 ? 8 : 4);
+  s->stats.state_d2h_bytes += s->nv * s->n_cells * sizeof(double);
   ++s->stats.full_downloads;
   s->stats.download_seconds += std::chrono::duration<double>(Clock::now() - t0).count();
   return 0;
@@ -227,6 +213,12 @@ void amr_hip_stats(const amr_session* s, amr_gpu_stats* out) {
 }
 
 void amr_hip_destroy(amr_session* s) { delete s; }
+
+int amr_hip_state(const amr_session* s, amr_state_meta* state) {
+  if (!s || !state) return 1;
+  to_meta(*s->host_control, state);
+  return 0;
+}
 
 int amr_hip_rhs(amr_session* s, const double* u, uint64_t u_len, double* r, char* err, size_t err_len) {
   if (!s || !u || !r || u_len != s->nv * s->n_cells) return fail(err, err_len, "invalid AMR GPU residual request");
@@ -268,4 +260,133 @@ int amr_hip_rhs(amr_session* s, const double* u, uint64_t u_len, double* r, char
   return 0;
 }
 
+int amr_hip_monitor(amr_session* s, const double* region, const double* moment, double mach, double aoa_degrees,
+                    double* out, char* err, size_t err_len) {
+  if (!s || !region || !moment || !out || !(mach > 0)) return fail(err, err_len, "invalid AMR GPU monitor request");
+  // Residual pass on the accepted state with a temporary control, then the reductions.
+  Control saved{};
+  if (!checked(copy_sync(&saved, s->arena->control(), sizeof(Control), hipMemcpyDeviceToHost, s->stream), err, err_len,
+               "AMR control D2H"))
+    return 1;
+  if (saved.fatal) return fail(err, err_len, "AMR GPU monitor on a failed session");
+  Control eval = saved;
+  eval.active = 1;
+  eval.bad = 0;
+  eval.r0_valid = 0;
+  eval.dt_min = 0x7ff0000000000000ull;
+  Control after{};
+  std::vector<double> cells, walls;
+  if (!checked(copy_sync(s->arena->control(), &eval, sizeof(Control), hipMemcpyHostToDevice, s->stream), err, err_len,
+               "AMR control H2D"))
+    return 1;
+  const double frame[6] = {region[0], region[1], region[2], region[3], moment[0], moment[1]};
+  s->arena->launch_residual(s->stream);
+  if (!checked(s->arena->launch_monitor(s->stream, frame), err, err_len, "AMR monitor kernels") ||
+      !checked(copy_sync(&after, s->arena->control(), sizeof(Control), hipMemcpyDeviceToHost, s->stream), err, err_len,
+               "AMR control D2H") ||
+      !checked(s->arena->read_monitor(&cells, &walls, s->stream), err, err_len, "AMR monitor D2H"))
+    return 1;
+  saved.w_valid = after.bad ? 0 : 1;
+  saved.r0_valid = 0;
+  saved.dt_min = 0x7ff0000000000000ull;
+  if (!checked(copy_sync(s->arena->control(), &saved, sizeof(Control), hipMemcpyHostToDevice, s->stream), err, err_len,
+               "AMR control restore"))
+    return 1;
+  if (after.bad || after.fatal) return fail(err, err_len, "nonphysical state or nonfinite face flux at the monitor");
+  double sums[9];
+  hipamr::sum_monitor(*s->arena, 0, cells, walls, sums);
+  if (!(sums[5] > 0)) return fail(err, err_len, "empty residual region");
+  const int nv = static_cast<int>(s->nv);
+  for (int v = 0; v < 5; ++v) out[v] = v < nv ? sums[v] / sums[5] : 0.0;
+  if (nv == 5) out[4] /= s->sa_nu_inf;
+  const double alpha = aoa_degrees * std::acos(-1.) / 180, cs = std::cos(alpha), sn = std::sin(alpha);
+  const double qinf = .5 * mach * mach, fx = sums[6], fy = sums[7], mz = sums[8];
+  out[5] = (-fx * sn + fy * cs) / qinf;
+  out[6] = (fx * cs + fy * sn) / qinf;
+  out[7] = mz / qinf;
+  out[8] = fx;
+  out[9] = fy;
+  out[10] = mz;
+  return 0;
+}
+
+int amr_hip_run(amr_session* s, double tend, int32_t trials, amr_state_meta* state, char* err, size_t err_len) {
+  if (!s || !state || trials < 1 || !std::isfinite(tend) || tend < s->host_control->time)
+    return fail(err, err_len, "invalid AMR GPU run");
+  hipamr::launch_target(s->arena->control(), 1, tend, INT64_MAX, nullptr, nullptr, s->stream);
+  for (int i = 0; i < trials; ++i)
+    if (launch_one(s)) return fail(err, err_len, "AMR trial launch failed");
+  if (!checked(hipGetLastError(), err, err_len, "AMR trial kernels") ||
+      !checked(copy_sync(s->host_control, s->arena->control(), sizeof(Control), hipMemcpyDeviceToHost, s->stream), err,
+               err_len, "AMR control D2H"))
+    return 1;
+  to_meta(*s->host_control, state);
+  if (s->host_control->fatal)
+    return fail(err, err_len,
+                "AMR GPU stopped with status " + std::to_string(s->host_control->fatal) + "; last accepted state retained");
+  return 0;
+}
+
+int amr_hip_profile(amr_session* s, double tend, int32_t trials, amr_kernel_time* out, int32_t capacity,
+                    int32_t* count, char* err, size_t err_len) {
+  if (!s || !out || !count || trials < 1 || capacity < 1 || !std::isfinite(tend) || tend < s->host_control->time)
+    return fail(err, err_len, "invalid AMR GPU profile request");
+  hipamr::launch_target(s->arena->control(), 1, tend, INT64_MAX, nullptr, nullptr, s->stream);
+  hipamr::EventHook hook(s->stream);
+  for (int i = 0; i < trials; ++i) s->arena->launch_trial(s->stream, &hook);
+  if (!checked(hipGetLastError(), err, err_len, "AMR trial kernels") ||
+      !checked(copy_sync(s->host_control, s->arena->control(), sizeof(Control), hipMemcpyDeviceToHost, s->stream), err,
+               err_len, "AMR control D2H"))
+    return 1;
+  return hook.collect(out, capacity, count, err, err_len);
+}
+
 }  // extern "C"
+
+namespace {
+// Plain device copy, 16 bytes per lane per iteration (the STREAM "copy" kernel).
+__global__ __launch_bounds__(256) void k_stream_copy(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                     size_t n) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" int amr_hip_copy_bandwidth(uint64_t bytes, int32_t repeats, double* gbps, double* ms_per_copy, char* err,
+                                      size_t err_len) {
+  if (!gbps || !ms_per_copy || repeats < 1 || bytes < 4096 || bytes % 16) return fail(err, err_len, "invalid copy probe");
+  void *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const auto cleanup = [&]() {
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+  };
+  if (!checked(hipMalloc(&a, bytes), err, err_len, "copy probe alloc") ||
+      !checked(hipMalloc(&b, bytes), err, err_len, "copy probe alloc") ||
+      !checked(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), err, err_len, "copy probe stream") ||
+      !checked(hipEventCreate(&e0), err, err_len, "copy probe event") ||
+      !checked(hipEventCreate(&e1), err, err_len, "copy probe event") ||
+      !checked(hipMemsetAsync(a, 0, bytes, s), err, err_len, "copy probe clear")) {
+    cleanup();
+    return 1;
+  }
+  const size_t n = bytes / 16;
+  const dim3 grid(static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 256 * 64))), block(256);
+  hipLaunchKernelGGL(k_stream_copy, grid, block, 0, s, static_cast<const double2*>(a), static_cast<double2*>(b), n);
+  (void)hipEventRecord(e0, s);
+  for (int r = 0; r < repeats; ++r)
+    hipLaunchKernelGGL(k_stream_copy, grid, block, 0, s, static_cast<const double2*>(a), static_cast<double2*>(b), n);
+  (void)hipEventRecord(e1, s);
+  float ms = 0;
+  const bool ok = checked(hipEventSynchronize(e1), err, err_len, "copy probe sync") &&
+                  checked(hipEventElapsedTime(&ms, e0, e1), err, err_len, "copy probe elapsed");
+  cleanup();
+  if (!ok) return 1;
+  *ms_per_copy = ms / repeats;
+  *gbps = 2.0 * static_cast<double>(bytes) / (*ms_per_copy * 1e-3) / 1e9;
+  return 0;
+}

@@ -154,3 +154,42 @@ def test_budget_returns_2_and_resumes():
         status = s.advance(0.5, wall_budget=1e-6)
         assert status == 2 and 0 < s.meta["steps"]
         assert s.advance(0.01 if s.meta["time"] < 0.01 else s.meta["time"]) == 0
+
+
+def test_monitor_matches_oracle():
+    """Device FP64 monitor vs the oracle's steady monitor (src/quick_airfoil.cpp:75-85) at the
+    same accepted state: residual norms within 1e-10 relative, CL/CD/CM_z within 1e-10."""
+    case, mesh, params = case_mesh("G0001", "LF1")
+    u0 = mesh.initialize(params, case.freestream)
+    region, moment = case.convergence["residual_region_xy_c"], case.moment_reference
+    with Session(mesh, params, u0) as s:
+        s.advance(0.01)
+        u = s.download()
+        got = s.monitor(region, moment, case.mach, case.aoa)
+    ref = O.monitor(mesh, params, u, region, moment, case.mach, case.aoa)
+    print("monitor", got, ref, flush=True)
+    assert np.allclose(got["residual"], ref["residual"], rtol=1e-10, atol=0)
+    for key in ("cl", "cd", "cm"):
+        assert abs(got[key] - ref[key]) <= 1e-10
+
+
+def test_time_driver_to_half_matches_oracle():
+    """Device driver in time mode to t* = 0.5 (checks at 0.01 and 0.5), against the oracle run
+    with the same two targets: equal steps, state within 1e-8 relative L2 per variable, CL/CD/CM_z
+    within 1e-8."""
+    case, mesh, params = case_mesh("G0001", "LF1")
+    u0 = mesh.initialize(params, case.freestream)
+    with Session(mesh, params, u0) as s:
+        res = s.solve(case, mode="time", stop_time=0.5)
+        got = s.download()
+    assert not res["failed"] and res["checks"] == 2 and res["time"] == 0.5
+    ref, meta = O.run(mesh, params, u0, 0.01)
+    ref, meta2 = O.run(mesh, params, ref, 0.5, time=meta["time"], steps=meta["steps"])
+    assert res["steps"] == meta2["steps"]
+    errs = per_variable_l2(got, ref, 5)
+    mon = O.monitor(mesh, params, ref, case.convergence["residual_region_xy_c"], case.moment_reference, case.mach,
+                    case.aoa)
+    print("time driver", res["steps"], errs, res["cl"], mon["cl"], flush=True)
+    assert max(errs) <= 1e-8
+    for key in ("cl", "cd", "cm"):
+        assert abs(res[key] - mon[key]) <= 1e-8

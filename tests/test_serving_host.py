@@ -1,0 +1,136 @@
+"""Host logic of the batch arena and the serving loop (no GPU): slot sizing, HBM budget
+admission, FIFO continuous batching, latency percentiles."""
+import time
+
+import numpy as np
+import pytest
+
+from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import _native as N
+from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import batch as B
+from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import serve as S
+
+from .common import EULER, LAMINAR, case_mesh, merge
+
+
+def test_slot_capacity_covers_mesh_and_padding():
+    _, mesh, _ = case_mesh("G0001", "LF1", LAMINAR)
+    for tile in (64, 128, 256):
+        cap, feat = B.capacity(mesh, tile)
+        info = mesh.info
+        assert cap.cells % tile == 0 and cap.cells >= mesh.n_cells + 1 and cap.cells - tile < mesh.n_cells + 1
+        assert cap.faces % tile == 0 and cap.faces >= info.n_faces
+        assert cap.walls % tile == 0 and cap.walls >= info.n_walls
+        assert cap.csr == info.n_cell_faces and cap.terms == info.n_visc_terms
+        assert (feat.laminar, feat.sa) == (1, 0)
+
+
+def test_arena_bytes_monotone_and_budget_inverse():
+    _, mesh, _ = case_mesh("G0002", "LF1")
+    cap, feat = B.capacity(mesh)
+    sizes = [B.arena_bytes([cap] * k, feat, 32) for k in (1, 2, 3, 8, 64)]
+    assert all(a < b for a, b in zip(sizes, sizes[1:]))
+    assert B.arena_bytes([cap], feat, 64) > B.arena_bytes([cap], feat, 32)
+    for k in (1, 7, 40):
+        budget = B.arena_bytes([cap] * k, feat, 32)
+        assert B.slots_fitting(cap, feat, budget, 32) == k
+        assert B.slots_fitting(cap, feat, budget - 1, 32) == k - 1
+    # Roughly linear in the slot count: a slot costs the same however many there are.
+    per_slot = (sizes[-1] - sizes[0]) / 63
+    assert abs(per_slot - (sizes[1] - sizes[0])) < 0.01 * per_slot
+
+
+def test_envelope_and_union():
+    caps, feats = [], []
+    for src, ov in (("G0001", LAMINAR), ("G0010", EULER)):
+        _, mesh, _ = case_mesh(src, "LF1", ov)
+        c, f = B.capacity(mesh)
+        caps.append(c)
+        feats.append(f)
+    env = B.envelope(caps, margin=0.1)
+    for name in ("cells", "faces", "walls", "csr", "terms", "groups", "members"):
+        assert getattr(env, name) >= max(getattr(c, name) for c in caps) * 1.1 - 1
+    assert env.cells % 64 == 0
+    u = B.union(feats)
+    assert u.laminar == 1 and u.sa == 0
+
+
+def test_batch_without_device_raises():
+    if N.load_library().amr_hip_device_count() > 0:
+        pytest.skip("a device is present")
+    _, mesh, _ = case_mesh("G0001", "LF1", LAMINAR)
+    cap, feat = B.capacity(mesh)
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        B.Batch.with_slots([cap], feat)
+
+
+def test_percentile_and_poisson():
+    v = list(range(1, 101))
+    assert S.percentile(v, 50) == 50 and S.percentile(v, 99) == 99 and S.percentile(v, 100) == 100
+    assert S.percentile([3.0], 99) == 3.0
+    t = S.poisson_arrivals(20.0, 20000, seed=1)
+    assert np.all(np.diff(t) > 0)
+    assert abs(np.mean(np.diff(t)) - 0.05) < 0.002
+
+
+class FakeMesh:
+    def __init__(self, n_cells):
+        self.n_cells = n_cells
+
+
+class FakeBackend:
+    """Runs take `duration` seconds of wall time; records concurrency and admission order."""
+
+    def __init__(self, n_slots, max_cells, duration):
+        self.n_slots, self.max_cells, self.duration = n_slots, max_cells, duration
+        self.running_runs = {}
+        self.peak = 0
+        self.started = []
+
+    def fits(self, mesh):
+        return mesh.n_cells <= self.max_cells
+
+    def start(self, slot, mesh, params, u0, case, mode, stop_time, budget):
+        assert slot not in self.running_runs
+        self.running_runs[slot] = time.perf_counter() + self.duration
+        self.started.append(case)
+        self.peak = max(self.peak, len(self.running_runs))
+
+    def running(self):
+        return len(self.running_runs)
+
+    def poll(self, seconds):
+        end = time.perf_counter() + seconds
+        while True:
+            now = time.perf_counter()
+            done = [s for s, t in self.running_runs.items() if t <= now]
+            if done or now >= end:
+                break
+            time.sleep(0.0005)
+        out = []
+        for s in done:
+            del self.running_runs[s]
+            out.append((s, {"converged": 1, "steps": 1, "stop_reason": "fake"}))
+        return out
+
+    def finish(self, slot):
+        assert slot not in self.running_runs
+
+
+def test_server_fifo_capacity_and_rejection():
+    sizes = [100, 100, 500, 100, 100, 100, 100]  # request 2 is too large for a slot
+    reqs = [S.Request(i, None, i, arrival=0.002 * i) for i in range(len(sizes))]
+    backend = FakeBackend(n_slots=2, max_cells=200, duration=0.03)
+    server = S.Server(backend, workers=2, poll_seconds=0.01,
+                      mesher=lambda r: (FakeMesh(sizes[r.rid]), None, None))
+    rep = server.serve(reqs)
+    assert backend.peak == 2
+    assert rep["completed"] == 6 and rep["failed"] == 1
+    bad = [o for o in rep["outcomes"] if o.error]
+    assert bad[0].rid == 2 and "capacity" in bad[0].error
+    # First come, first served: admissions follow arrival order (meshing is fast here).
+    assert backend.started == [0, 1, 3, 4, 5, 6]
+    good = [o for o in rep["outcomes"] if not o.error]
+    assert all(o.latency >= 0.03 - 1e-3 for o in good)
+    # Three waves of two runs: the last request waits for two runs before it.
+    assert rep["p99_latency_seconds"] >= 0.06
+    assert 0 < rep["slot_occupancy"] <= 1
