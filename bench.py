@@ -20,7 +20,7 @@ and on-device checks included).  The plain path is measured the same way: one FP
 mesh solved in FP64 (--fp64-check meshes, FP64 batch, bitwise equal to FP64 sessions):
 max |dCL|, |dCD|, |dCM|.
 
-roofline: the stage pass (gradient + face + collect of one RK stage) of the whole batch,
+roofline: the stage pass (gradient + fused flux/gather/RK kernel of one RK stage) of the whole batch,
 per-kernel hipEvent times on the arena's stream; algorithmic bytes from roofline.py.
 cpu_baseline: the FP64 oracle on one host core, a bounded sample of steps of the single LF2
 case, scaled by that case's FP64 steps to steady.
@@ -140,7 +140,7 @@ def main():
     log(f"batch step {step_seconds * 1e6:.1f} us ({step_seconds * 1e6 / args.batch:.2f} us per mesh-step), "
         f"arena {stats['allocated_bytes'] / 1e6:.0f} MB, {stats['graph_nodes_per_trial']} graph nodes")
     per_launch = {k: v[0] / v[1] * 1e-3 for k, v in prof.items()}
-    stage_seconds = per_launch["gradient_stage"] + per_launch["face_stage"] + per_launch["collect_rk1"]
+    stage_seconds = per_launch["gradient_stage"] + per_launch["flux_rk1"]
     b_design = sum(RL.design_stage_bytes(m, 32) for m in meshes)
     b_contract = sum(RL.contract_stage_bytes(m, 32) for m in meshes)
     achieved = b_design / stage_seconds / 1e9
@@ -257,7 +257,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": RL.HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / RL.HBM_PEAK_GBPS, 4), "traffic": None,
-            "kernel": "stage pass of the batch: gradient_stage + face_stage + collect_rk1",
+            "kernel": "stage pass of the batch: gradient_stage (k_grad) + flux_rk1 (k_flux, faces + gather + RK)",
             "bytes_per_launch": int(b_design), "contract_bytes_per_launch": int(b_contract),
             "us_per_launch": round(stage_seconds * 1e6, 2), "stream_copy_gbps": round(gbps.value, 1),
         },

@@ -28,7 +28,7 @@ struct Control {
   int cur;                   // buffer holding the accepted state: U[cur]
   int w_valid;               // W/PHI hold primitives of U[cur]
   int r0_valid;              // R0 holds L(U[cur]) (set after a rejection)
-  int pad0;
+  int wbuf;                  // cell-record buffer holding the primitives of U[cur]: CREC[wbuf]
 };
 static_assert(sizeof(Control) == 104, "Control layout");
 
@@ -76,16 +76,10 @@ struct ArenaPtrs {
   const int* mem_off;    // nc + 1
   const int* mem_grp;
   const T* mem_inv;      // 1 / N_k
-  // faces
-  const int* face_l;
-  const int* face_r;
-  const int* face_info;  // bit 0 axis, bits 1..3 side + 1 (0 interior)
-  const T* face_len;
-  const T* face_dx;
-  const T* face_dy;
-  const T* face_ex;
-  const T* face_ey;
-  const T* face_dist;
+  // faces: FIDX = {left, right, info, 0} (info bit 0 axis, bits 1..3 side + 1, 0 interior),
+  // FGEO = {length, dx, dy, ex, ey, distance, 0, 0}
+  const int4* FIDX;
+  const T* FGEO;
   // walls
   const int* wall_owner;
   const int* wall_donor;
@@ -105,26 +99,14 @@ struct ArenaPtrs {
   // down to the steady tolerance); residuals and all work arrays are T.
   double* U[2];
   double* U2;
-  T* R0;
-  T* W;       // rho, u, v, p
-  T* PHI;     // nutilde = rho nutilde / rho
-  T* GRAD;    // ux, uy, vx, vy
-  T* MU;
-  T* EDDY;
-  T* SGRAD;   // nutilde_x, nutilde_y
-  T* SDIFF;
-  T* SSRC;
-  T* SRATE;
-  T* FLUX;    // 4 planes, times face length, viscous part subtracted
-  T* SPEED;
-  T* FCOEF;
-  T* SFLUX;
-  T* SFCOEF;
-  T* WFLUX;   // 2 planes (x, y); mass and energy wall fluxes vanish
-  T* WCOEF;
-  T* SWFLUX;
-  T* SWCOEF;
   double* QHAT;  // SRD group averages, 4 planes
+  T* R0;
+  // Cell records, double-buffered by stage (16 T per cell, AoS so that a face reads each of
+  // its cells in whole 16-byte vectors): primitives rho, u, v, p, nutilde, mu, eddy viscosity,
+  // -, then velocity gradient ux, uy, vx, vy, nutilde gradient x, y, SA diffusion, -.
+  T* CREC[2];
+  T* SSRC;    // SA-neg source (own cell only)
+  T* SRATE;   // SA-neg source rate
   T* CELL_DT; // deferred local dt (SRD-grouped viscous limit only)
   T* DRATE;
   T* DGROUP;

@@ -102,7 +102,7 @@ Totals totals_of(const std::vector<SlotCapacity>& caps, int tile) {
 // least one element and starts on a 256-byte boundary.
 struct SlabPlan {
   size_t geometry, control, work, total;
-  size_t cplane, dplane, fplane, wplane, mon_cell, mon_wall;
+  size_t cplane, dplane, crec, mon_cell, mon_wall;
 };
 
 template <class T>
@@ -112,21 +112,19 @@ SlabPlan plan_slab(const Totals& t, int nv) {
   p.geometry = arr(sizeof(MeshDesc<T>), t.meshes) + arr(4, t.cell_blocks) + arr(4, t.face_blocks + t.wall_blocks) +
                5 * arr(sizeof(T), t.cells) + 2 * arr(4, t.cells) + 2 * arr(8, t.cells) + 4 * arr(4, t.cells + 1) +
                arr(4, t.csr) + arr(4, t.terms) + 2 * arr(sizeof(T), t.terms) + arr(4, t.groups) +
-               arr(sizeof(T), t.groups) + arr(4, t.members) + 3 * arr(4, t.faces) + 6 * arr(sizeof(T), t.faces) +
+               arr(sizeof(T), t.groups) + arr(4, t.members) + arr(16, t.faces) + arr(8 * sizeof(T), t.faces) +
                2 * arr(4, t.walls) + 4 * arr(sizeof(T), t.walls) + 2 * arr(8, t.walls);
   p.control = arr(sizeof(Control), t.meshes) + arr(sizeof(double), t.meshes) + arr(sizeof(long long), t.meshes) +
               arr(6 * sizeof(double), t.meshes);
   p.cplane = arr(sizeof(T), t.cells);
   p.dplane = arr(sizeof(double), t.cells);
-  p.fplane = arr(sizeof(T), t.faces);
-  p.wplane = arr(sizeof(T), t.walls);
+  p.crec = arr(16 * sizeof(T), t.cells);
   p.mon_cell = arr(6 * sizeof(double), t.cell_blocks);
   p.mon_wall = arr(3 * sizeof(double), t.wall_blocks);
-  // FP64 state U[2], U2 (nv planes each) and QHAT (4); T planes R0 (nv), W (4), PHI, GRAD (4),
-  // MU, EDDY, SGRAD (2), SDIFF, SSRC, SRATE, CELL_DT, DRATE, DGROUP; faces FLUX (4), SPEED,
-  // FCOEF, SFLUX, SFCOEF; walls WFLUX (2), WCOEF, SWFLUX, SWCOEF; monitor partials.
-  const size_t state_planes = 3 * nv + 4, cell_planes = nv + 4 + 1 + 4 + 2 + 2 + 3 + 3;
-  p.work = state_planes * p.dplane + cell_planes * p.cplane + 8 * p.fplane + 5 * p.wplane + p.mon_cell + p.mon_wall;
+  // FP64 state U[2], U2 (nv planes each) and QHAT (4); two record buffers of 16 T per cell;
+  // T planes R0 (nv), SSRC, SRATE, CELL_DT, DRATE, DGROUP; monitor partials.
+  const size_t state_planes = 3 * nv + 4, cell_planes = nv + 5;
+  p.work = state_planes * p.dplane + 2 * p.crec + cell_planes * p.cplane + p.mon_cell + p.mon_wall;
   p.total = p.geometry + p.control + p.work;
   return p;
 }
@@ -140,8 +138,8 @@ struct HostArrays {
   std::vector<double> cell_cx, cell_cy;
   std::vector<int> csr_face, term_src, grp_mem, mem_grp;
   std::vector<T> term_wx, term_wy, grp_w;
-  std::vector<int> face_l, face_r, face_info;
-  std::vector<T> face_len, face_dx, face_dy, face_ex, face_ey, face_dist;
+  std::vector<int> face_idx;  // 4 per face: left, right, info, 0
+  std::vector<T> face_geo;    // 8 per face: length, dx, dy, ex, ey, distance, 0, 0
   std::vector<int> wall_owner, wall_donor;
   std::vector<T> wall_nx, wall_ny, wall_len, wall_ddist;
   std::vector<double> wall_mx, wall_my;
@@ -235,15 +233,8 @@ class DeviceArena final : public Arena {
     sized(h.grp_mem, ng, 0);
     sized(h.grp_w, ng, T(0));
     sized(h.mem_grp, nm, 0);
-    sized(h.face_l, nf, 0);
-    sized(h.face_r, nf, 0);
-    sized(h.face_info, nf, 0);
-    sized(h.face_len, nf, T(0));
-    sized(h.face_dx, nf, T(0));
-    sized(h.face_dy, nf, T(0));
-    sized(h.face_ex, nf, T(0));
-    sized(h.face_ey, nf, T(0));
-    sized(h.face_dist, nf, T(1));
+    sized(h.face_idx, 4 * static_cast<size_t>(std::max(nf, 1)), 0);
+    sized(h.face_geo, 8 * static_cast<size_t>(std::max(nf, 1)), T(0));
     sized(h.wall_owner, nw, 0);
     sized(h.wall_donor, nw, 0);
     sized(h.wall_nx, nw, T(0));
@@ -339,21 +330,25 @@ class DeviceArena final : public Arena {
         continue;
       }
       const amr_face& face = v.faces[q];
-      h.face_l[g] = face.left >= 0 ? cb + face.left : -1;
-      h.face_r[g] = face.right >= 0 ? cb + face.right : -1;
-      h.face_info[g] = (face.axis & 1) | ((face.boundary + 1) << 1);
-      h.face_len[g] = static_cast<T>(face.length);
+      int* fi = &h.face_idx[4 * static_cast<size_t>(g)];
+      T* fg = &h.face_geo[8 * static_cast<size_t>(g)];
+      fi[0] = face.left >= 0 ? cb + face.left : -1;
+      fi[1] = face.right >= 0 ? cb + face.right : -1;
+      fi[2] = (face.axis & 1) | ((face.boundary + 1) << 1);
+      fi[3] = 0;
+      fg[0] = static_cast<T>(face.length);
       if (laminar) {
         const amr_face_geometry& geo = v.face_geometry[q];
-        h.face_dx[g] = static_cast<T>(geo.dx);
-        h.face_dy[g] = static_cast<T>(geo.dy);
-        h.face_ex[g] = static_cast<T>(geo.ex);
-        h.face_ey[g] = static_cast<T>(geo.ey);
-        h.face_dist[g] = static_cast<T>(face.boundary >= 0 ? v.face_samples[q].distance : geo.distance);
+        fg[1] = static_cast<T>(geo.dx);
+        fg[2] = static_cast<T>(geo.dy);
+        fg[3] = static_cast<T>(geo.ex);
+        fg[4] = static_cast<T>(geo.ey);
+        fg[5] = static_cast<T>(face.boundary >= 0 ? v.face_samples[q].distance : geo.distance);
       } else {
-        h.face_dx[g] = h.face_dy[g] = h.face_ex[g] = h.face_ey[g] = T(0);
-        h.face_dist[g] = T(1);
+        fg[1] = fg[2] = fg[3] = fg[4] = T(0);
+        fg[5] = T(1);
       }
+      fg[6] = fg[7] = T(0);
     }
     for (int j = 0; j < s.cap.walls; ++j) {
       const int g = wb + j;
@@ -438,7 +433,7 @@ class DeviceArena final : public Arena {
       for (int j = h.csr_off[k]; j < h.csr_off[k + 1]; ++j) {
         const int sq = h.csr_face[j], q = sq >= 0 ? sq : ~sq;
         if (q < s.face_begin || q >= s.face_begin + s.n_faces) return "layout: cell-face id";
-        if ((sq >= 0 ? h.face_l[q] : h.face_r[q]) != k) return "layout: cell-face orientation";
+        if (h.face_idx[4 * static_cast<size_t>(q) + (sq >= 0 ? 0 : 1)] != k) return "layout: cell-face orientation";
       }
       if (h.wall_cnt[k] < 0 || h.wall_first[k] < s.wall_begin || h.wall_first[k] + h.wall_cnt[k] > s.wall_begin + s.n_walls)
         return "layout: cell walls";
@@ -459,7 +454,8 @@ class DeviceArena final : public Arena {
       if (h.mem_off[k + 1] == h.mem_off[k]) return "layout: cell in no SRD group";
     }
     for (int q = s.face_begin; q < s.face_begin + s.n_faces; ++q) {
-      const int l = h.face_l[q], r = h.face_r[q], side = (h.face_info[q] >> 1) - 1;
+      const int* fi = &h.face_idx[4 * static_cast<size_t>(q)];
+      const int l = fi[0], r = fi[1], side = (fi[2] >> 1) - 1;
       if (side < -1 || side > 3) return "layout: face side";
       if (l != -1 && !cell_ok(l)) return "layout: face left";
       if (r != -1 && !cell_ok(r)) return "layout: face right";
@@ -514,8 +510,8 @@ class DeviceArena final : public Arena {
     col(h.grp_mem, GROUPS);
     col(h.grp_w, GROUPS);
     col(h.mem_grp, MEMBERS);
-    for (auto* v : {&h.face_l, &h.face_r, &h.face_info}) col(*v, FACES);
-    for (auto* v : {&h.face_len, &h.face_dx, &h.face_dy, &h.face_ex, &h.face_ey, &h.face_dist}) col(*v, FACES);
+    columns_.push_back({h.face_idx.data(), nullptr, 4 * sizeof(int), h.face_idx.size() / 4, FACES});
+    columns_.push_back({h.face_geo.data(), nullptr, 8 * sizeof(T), h.face_geo.size() / 8, FACES});
     for (auto* v : {&h.wall_owner, &h.wall_donor}) col(*v, WALLS);
     for (auto* v : {&h.wall_nx, &h.wall_ny, &h.wall_len, &h.wall_ddist}) col(*v, WALLS);
     for (auto* v : {&h.wall_mx, &h.wall_my}) col(*v, WALLS);
@@ -526,7 +522,7 @@ class DeviceArena final : public Arena {
     const SlabPlan plan = plan_slab<T>(totals_, nv);
     if (geo != plan.geometry) return "AMR arena size bookkeeping";
     const int nc = std::max(nc_, 1), nf = std::max(nf_, 1), nw = std::max(nw_, 1);
-    const size_t cplane = plan.cplane, dplane = plan.dplane, fplane = plan.fplane, wplane = plan.wplane;
+    const size_t cplane = plan.cplane, dplane = plan.dplane, crec = plan.crec;
     const size_t mon_cell = plan.mon_cell, mon_wall = plan.mon_wall;
     slab_bytes_ = plan.total;
     geometry_bytes_ = geo;
@@ -565,15 +561,8 @@ class DeviceArena final : public Arena {
     a.grp_mem = next(static_cast<const int*>(nullptr));
     a.grp_w = next(static_cast<const T*>(nullptr));
     a.mem_grp = next(static_cast<const int*>(nullptr));
-    a.face_l = next(static_cast<const int*>(nullptr));
-    a.face_r = next(static_cast<const int*>(nullptr));
-    a.face_info = next(static_cast<const int*>(nullptr));
-    a.face_len = next(static_cast<const T*>(nullptr));
-    a.face_dx = next(static_cast<const T*>(nullptr));
-    a.face_dy = next(static_cast<const T*>(nullptr));
-    a.face_ex = next(static_cast<const T*>(nullptr));
-    a.face_ey = next(static_cast<const T*>(nullptr));
-    a.face_dist = next(static_cast<const T*>(nullptr));
+    a.FIDX = next(static_cast<const int4*>(nullptr));
+    a.FGEO = next(static_cast<const T*>(nullptr));
     a.wall_owner = next(static_cast<const int*>(nullptr));
     a.wall_donor = next(static_cast<const int*>(nullptr));
     a.wall_nx = next(static_cast<const T*>(nullptr));
@@ -607,28 +596,14 @@ class DeviceArena final : public Arena {
     a.U[1] = dplanes(nv);
     a.U2 = dplanes(nv);
     a.QHAT = dplanes(4);
+    a.CREC[0] = plane(crec, 1);
+    a.CREC[1] = plane(crec, 1);
     a.R0 = plane(cplane, nv);
-    a.W = plane(cplane, 4);
-    a.PHI = plane(cplane, 1);
-    a.GRAD = plane(cplane, 4);
-    a.MU = plane(cplane, 1);
-    a.EDDY = plane(cplane, 1);
-    a.SGRAD = plane(cplane, 2);
-    a.SDIFF = plane(cplane, 1);
     a.SSRC = plane(cplane, 1);
     a.SRATE = plane(cplane, 1);
     a.CELL_DT = plane(cplane, 1);
     a.DRATE = plane(cplane, 1);
     a.DGROUP = plane(cplane, 1);
-    a.FLUX = plane(fplane, 4);
-    a.SPEED = plane(fplane, 1);
-    a.FCOEF = plane(fplane, 1);
-    a.SFLUX = plane(fplane, 1);
-    a.SFCOEF = plane(fplane, 1);
-    a.WFLUX = plane(wplane, 2);
-    a.WCOEF = plane(wplane, 1);
-    a.SWFLUX = plane(wplane, 1);
-    a.SWCOEF = plane(wplane, 1);
     a.MON_CELL = reinterpret_cast<double*>(base + offset);
     offset += mon_cell;
     a.MON_WALL = reinterpret_cast<double*>(base + offset);
@@ -804,10 +779,12 @@ class DeviceArena final : public Arena {
   }
   void pad_face(int g, int cb) {
     HostArrays<T>& h = h_;
-    h.face_l[g] = h.face_r[g] = cb;
-    h.face_info[g] = 0;
-    h.face_len[g] = h.face_dx[g] = h.face_dy[g] = h.face_ex[g] = h.face_ey[g] = T(0);
-    h.face_dist[g] = T(1);
+    int* fi = &h.face_idx[4 * static_cast<size_t>(g)];
+    T* fg = &h.face_geo[8 * static_cast<size_t>(g)];
+    fi[0] = fi[1] = cb;
+    fi[2] = fi[3] = 0;
+    for (int i = 0; i < 8; ++i) fg[i] = T(0);
+    fg[5] = T(1);
   }
   void pad_wall(int g, int cb) {
     HostArrays<T>& h = h_;

@@ -1,15 +1,21 @@
 // amr_kernels.hip -- the explicit cut-cell SSP-RK3 trial as gfx950 kernels.
 //
 // One trial (reference launch_trial, src/static_amr2d_cuda.cu:241-268) is, per RHS pass:
-//   k_grad     cells: viscosity, eddy viscosity, LSQ velocity gradient, SA-neg gradients and
-//              point model (reference :71-91, src/static_amr2d_sa_cuda.cuh:2-11)
-//   k_face     faces then walls: boundary state, HLLC / AUSM+-up, viscous face flux, SA face
-//              flux; no-slip cut-wall shear (reference :92-143, src/static_amr2d_sa_cuda.cuh:12-29)
-//   k_collect  cells: CSR gather r = -(sum F + p n l - wall shear)/V; the base pass also forms
-//              the local CFL / viscous / SA step and reduces it (wave shuffles + LDS + one 64-bit
-//              atomicMin per block); stage passes apply the SSP-RK3 combination and convert the
-//              candidate to primitives in the same kernel (reference :144-163,188-194)
-// plus k_base_prim, k_stage0 (dt choice + first stage, reference :170-191), the SRD pair
+//   k_grad   cells: LSQ velocity gradient, SA-neg gradients and point model (reference
+//            :71-91, src/static_amr2d_sa_cuda.cuh:2-11) into the gradient half of the record
+//   k_flux   cells: for every face of the cell the boundary state, HLLC / AUSM+-up, viscous
+//            and SA face flux (reference :92-143, src/static_amr2d_sa_cuda.cuh:12-29), the
+//            no-slip cut-wall shear of its wall fragments, the gather r = -(sum F + p n l -
+//            wall shear)/V (reference :144-163); the base pass also forms the local CFL /
+//            viscous / SA step and reduces it (wave shuffles + LDS + one 64-bit atomicMin per
+//            block), stage passes apply the SSP-RK3 combination in FP64 (reference :188-194)
+//            and write the candidate's primitives, viscosity and eddy viscosity.
+// A face shared by two cells is evaluated by both with the same inputs and operations, so both
+// see the same flux bits; no flux array is written or re-read.  Cell data that faces read lives
+// in 16-wide records (AoS) so that one neighbour costs whole 16-byte vector loads instead of one
+// cache line per variable.  Records are double-buffered by stage: a stage reads one buffer and
+// writes the next state's primitives into the other.
+// Plus k_base_prim, k_stage0 (dt choice + first stage, reference :170-191), the SRD pair
 // k_group / k_redist (reference :195-207), the SRD-grouped viscous limit (reference :164-176)
 // and k_commit (accept / reject / begin, reference :54-59,212-216).
 // Every kernel serves every mesh of the arena; a block reads its mesh from a block table and
@@ -22,6 +28,7 @@ namespace hipamr {
 namespace {
 
 constexpr int kMaxTile = 256;
+constexpr int kRec = 16;  // record width (T elements per cell)
 constexpr unsigned long long kInfBits = 0x7ff0000000000000ull;
 
 enum : int { PASS_BASE = 0, PASS_STAGE = 1 };
@@ -36,34 +43,105 @@ __device__ __forceinline__ bool pass_on(const Control* c, int pass) {
 enum : int { BAD_BASE = 1, BAD_STAGE = 2 };
 __device__ __forceinline__ void flag_bad(Control* c, int bit) { atomicOr(&c->bad, bit); }
 
-// U -> W = (rho, u, v, p) and nutilde (computed in FP64 from the FP64 state, stored as T);
-// false if not admissible (reference primitive rules, src/static_amr2d.cpp:143-154).
+// Record buffer read by a pass: the base pass and stage 2 read CREC[wbuf], stage 1 reads the
+// other buffer (written by stage 0); each stage writes the buffer it does not read.
+__device__ __forceinline__ int read_buffer(const Control* c, int stage) { return stage == 1 ? c->wbuf ^ 1 : c->wbuf; }
+
+// ---- cell records ---------------------------------------------------------------------------
 template <class T>
-__device__ __forceinline__ bool to_primitive(const double* u, int nc, int k, T gamma, bool sa, T* w, T* phi) {
+struct CellPrim {
+  T rho, u, v, p, phi, mu, eddy;
+};
+template <class T>
+struct CellGrad {
+  T ux, uy, vx, vy, sx, sy, sdiff;
+};
+
+__device__ __forceinline__ CellPrim<float> load_prim(const float* r) {
+  const float4 a = *reinterpret_cast<const float4*>(r), b = *reinterpret_cast<const float4*>(r + 4);
+  return CellPrim<float>{a.x, a.y, a.z, a.w, b.x, b.y, b.z};
+}
+__device__ __forceinline__ CellPrim<double> load_prim(const double* r) {
+  const double2 a = *reinterpret_cast<const double2*>(r), b = *reinterpret_cast<const double2*>(r + 2);
+  const double2 c = *reinterpret_cast<const double2*>(r + 4), d = *reinterpret_cast<const double2*>(r + 6);
+  return CellPrim<double>{a.x, a.y, b.x, b.y, c.x, c.y, d.x};
+}
+__device__ __forceinline__ CellGrad<float> load_grad(const float* r) {
+  const float4 a = *reinterpret_cast<const float4*>(r + 8), b = *reinterpret_cast<const float4*>(r + 12);
+  return CellGrad<float>{a.x, a.y, a.z, a.w, b.x, b.y, b.z};
+}
+__device__ __forceinline__ CellGrad<double> load_grad(const double* r) {
+  const double2 a = *reinterpret_cast<const double2*>(r + 8), b = *reinterpret_cast<const double2*>(r + 10);
+  const double2 c = *reinterpret_cast<const double2*>(r + 12), d = *reinterpret_cast<const double2*>(r + 14);
+  return CellGrad<double>{a.x, a.y, b.x, b.y, c.x, c.y, d.x};
+}
+__device__ __forceinline__ void store_prim(float* r, const CellPrim<float>& w) {
+  *reinterpret_cast<float4*>(r) = make_float4(w.rho, w.u, w.v, w.p);
+  *reinterpret_cast<float4*>(r + 4) = make_float4(w.phi, w.mu, w.eddy, 0.f);
+}
+__device__ __forceinline__ void store_prim(double* r, const CellPrim<double>& w) {
+  *reinterpret_cast<double2*>(r) = make_double2(w.rho, w.u);
+  *reinterpret_cast<double2*>(r + 2) = make_double2(w.v, w.p);
+  *reinterpret_cast<double2*>(r + 4) = make_double2(w.phi, w.mu);
+  *reinterpret_cast<double2*>(r + 6) = make_double2(w.eddy, 0.0);
+}
+__device__ __forceinline__ void store_grad(float* r, const CellGrad<float>& g) {
+  *reinterpret_cast<float4*>(r + 8) = make_float4(g.ux, g.uy, g.vx, g.vy);
+  *reinterpret_cast<float4*>(r + 12) = make_float4(g.sx, g.sy, g.sdiff, 0.f);
+}
+__device__ __forceinline__ void store_grad(double* r, const CellGrad<double>& g) {
+  *reinterpret_cast<double2*>(r + 8) = make_double2(g.ux, g.uy);
+  *reinterpret_cast<double2*>(r + 10) = make_double2(g.vx, g.vy);
+  *reinterpret_cast<double2*>(r + 12) = make_double2(g.sx, g.sy);
+  *reinterpret_cast<double2*>(r + 14) = make_double2(g.sdiff, 0.0);
+}
+
+// Face geometry record: length, dx, dy, ex, ey, distance.
+template <class T>
+struct FaceGeo {
+  T len, dx, dy, ex, ey, dist;
+};
+__device__ __forceinline__ FaceGeo<float> load_geo(const float* g) {
+  const float4 a = *reinterpret_cast<const float4*>(g);
+  const float2 b = *reinterpret_cast<const float2*>(g + 4);
+  return FaceGeo<float>{a.x, a.y, a.z, a.w, b.x, b.y};
+}
+__device__ __forceinline__ FaceGeo<double> load_geo(const double* g) {
+  const double2 a = *reinterpret_cast<const double2*>(g), b = *reinterpret_cast<const double2*>(g + 2);
+  const double2 c = *reinterpret_cast<const double2*>(g + 4);
+  return FaceGeo<double>{a.x, a.y, b.x, b.y, c.x, c.y};
+}
+
+// U -> record primitives (computed in FP64 from the FP64 state, stored as T), plus molecular
+// and eddy viscosity for viscous meshes; false if not admissible (reference primitive rules,
+// src/static_amr2d.cpp:143-154, and a valid viscosity).
+template <class T>
+__device__ __forceinline__ bool make_prim(const double* u, int nc, int k, const MeshDesc<T>& d, CellPrim<T>& w) {
   const double rho = u[k], mx = u[nc + k], my = u[2 * nc + k], energy = u[3 * nc + k];
   const double ir = 1.0 / rho;
   const double vx = mx * ir, vy = my * ir;
-  const double p = (static_cast<double>(gamma) - 1.0) * (energy - 0.5 * (mx * vx + my * vy));
+  const double p = (static_cast<double>(d.gamma) - 1.0) * (energy - 0.5 * (mx * vx + my * vy));
   bool ok = isfinite(rho) && rho > 0 && isfinite(mx) && isfinite(my) && isfinite(energy) && isfinite(p) && p > 0;
-  w[0] = static_cast<T>(rho);
-  w[1] = static_cast<T>(vx);
-  w[2] = static_cast<T>(vy);
-  w[3] = static_cast<T>(p);
-  if (sa) {
+  w.rho = static_cast<T>(rho);
+  w.u = static_cast<T>(vx);
+  w.v = static_cast<T>(vy);
+  w.p = static_cast<T>(p);
+  w.phi = T(0);
+  if (d.sa) {
     const double ph = u[4 * nc + k] * ir;
-    *phi = static_cast<T>(ph);
+    w.phi = static_cast<T>(ph);
     ok = ok && isfinite(ph);
   }
-  return ok && w[0] > T(0) && w[3] > T(0) && isfinite(w[0]) && isfinite(w[3]);
-}
-
-template <class T>
-__device__ __forceinline__ void store_primitive(const ArenaPtrs<T>& a, int k, const T* w, T phi, bool sa) {
-  a.W[k] = w[0];
-  a.W[a.nc + k] = w[1];
-  a.W[2 * a.nc + k] = w[2];
-  a.W[3 * a.nc + k] = w[3];
-  if (sa) a.PHI[k] = phi;
+  ok = ok && w.rho > T(0) && w.p > T(0) && isfinite(w.rho) && isfinite(w.p);
+  w.mu = T(0);
+  w.eddy = T(0);
+  if (d.laminar && ok) {
+    const T mu = viscosity(d.transport, w.p / (w.rho * d.transport.gas_constant));
+    ok = mu > T(0) && isfinite(mu);
+    w.mu = mu;
+    if (d.sa) w.eddy = w.rho * sa_eddy(w.phi, mu / w.rho);
+  }
+  return ok;
 }
 
 // Minimum over the block of a double (wave64 xor-shuffles, then one LDS slot per wave).
@@ -94,38 +172,31 @@ __global__ __launch_bounds__(kMaxTile) void k_base_prim(ArenaPtrs<T> a) {
   const MeshDesc<T>& d = a.desc[m];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= d.cell_begin + d.n_cells) return;
-  T w[4], phi = T(0);
-  const bool ok = to_primitive(a.U[c->cur], a.nc, k, d.gamma, d.sa != 0, w, &phi);
-  if (!ok) {
+  CellPrim<T> w;
+  if (!make_prim(a.U[c->cur], a.nc, k, d, w)) {
     flag_bad(c, BAD_BASE);
     atomicExch(&c->fatal, 2);  // the accepted state itself is not admissible
     return;
   }
-  store_primitive(a, k, w, phi, d.sa != 0);
+  store_prim(a.CREC[c->wbuf] + kRec * k, w);
 }
 
 // ------------------------------------------------------------------------------------------
-template <class T, int PASS>
+// LSQ gradients of the stage's state (and the SA-neg point source) into the record's gradient
+// half.  STAGE 0 = base pass, 1 and 2 = RK stages.
+template <class T, int STAGE>
 __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
   const int m = a.cell_block_mesh[blockIdx.x];
   Control* c = a.ctrl + m;
-  if (!pass_on(c, PASS)) return;
+  if (!pass_on(c, STAGE == 0 ? PASS_BASE : PASS_STAGE)) return;
   const MeshDesc<T>& d = a.desc[m];
   if (!d.laminar) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= d.cell_begin + d.n_cells) return;
-  const int nc = a.nc;
+  T* rec = a.CREC[read_buffer(c, STAGE)];
   const bool sa = d.sa != 0;
-  const T rho = a.W[k], u = a.W[nc + k], v = a.W[2 * nc + k], p = a.W[3 * nc + k];
-  const T mu = viscosity(d.transport, p / (rho * d.transport.gas_constant));
-  const int bit = PASS == PASS_BASE ? BAD_BASE : BAD_STAGE;
-  if (!(mu > T(0)) || !isfinite(mu)) {
-    flag_bad(c, bit);
-    return;
-  }
-  a.MU[k] = mu;
-  const T phi = sa ? a.PHI[k] : T(0);
-  if (sa) a.EDDY[k] = rho * sa_eddy(phi, mu / rho);
+  const CellPrim<T> own = load_prim(rec + kRec * k);
+  const T rho = own.rho, u = own.u, v = own.v, phi = own.phi;
   T g0 = 0, g1 = 0, g2 = 0, g3 = 0, sx = 0, sy = 0, rx = 0, ry = 0;
   const int end = a.term_off[k + 1];
   for (int j = a.term_off[k]; j < end; ++j) {
@@ -133,11 +204,12 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
     const T wx = a.term_wx[j], wy = a.term_wy[j];
     T uu = T(0), vv = T(0), ph = T(0), rr = rho;
     if (src >= 0) {
-      uu = a.W[nc + src];
-      vv = a.W[2 * nc + src];
+      const CellPrim<T> nb = load_prim(rec + kRec * src);
+      uu = nb.u;
+      vv = nb.v;
       if (sa) {
-        ph = a.PHI[src];
-        rr = a.W[src];
+        ph = nb.phi;
+        rr = nb.rho;
       }
     } else if (src <= -2) {
       const Side<T>& s = d.side[-2 - src];
@@ -156,49 +228,66 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
       ry += wy * (rr - rho);
     }
   }
-  a.GRAD[k] = g0;
-  a.GRAD[nc + k] = g1;
-  a.GRAD[2 * nc + k] = g2;
-  a.GRAD[3 * nc + k] = g3;
+  CellGrad<T> g{g0, g1, g2, g3, sx, sy, T(0)};
   if (sa) {
-    a.SGRAD[k] = sx;
-    a.SGRAD[nc + k] = sy;
-    const SaPoint<T> pt = sa_point(phi, mu / rho, a.wall_dist[k], fabs(g2 - g1));
+    const SaPoint<T> pt = sa_point(phi, own.mu / rho, a.wall_dist[k], fabs(g2 - g1));
     const T source = rho * (pt.source + T(SaConst<T>::cb2 / SaConst<T>::sigma) * (sx * sx + sy * sy)) -
                      pt.diffusion * (rx * sx + ry * sy);
-    a.SDIFF[k] = pt.diffusion;
+    g.sdiff = pt.diffusion;
     a.SSRC[k] = source;
     a.SRATE[k] = pt.source_rate;
-    if (!(pt.diffusion > T(0)) || !isfinite(source) || !isfinite(pt.source_rate)) flag_bad(c, bit);
+    if (!(pt.diffusion > T(0)) || !isfinite(source) || !isfinite(pt.source_rate))
+      flag_bad(c, STAGE == 0 ? BAD_BASE : BAD_STAGE);
   }
+  store_grad(rec + kRec * k, g);
 }
 
 // ------------------------------------------------------------------------------------------
+// One face seen from one of its cells: boundary state, inviscid flux in the face frame, viscous
+// and SA face fluxes.  Everything stays in registers: the face-normal component and the two
+// sides are chosen with selects, never by indexing a local array or selecting between the
+// addresses of two records (either would put them in scratch memory).
 template <class T>
-__device__ __forceinline__ Prim<T> load_prim(const ArenaPtrs<T>& a, int k) {
-  return Prim<T>{a.W[k], a.W[a.nc + k], a.W[2 * a.nc + k], a.W[3 * a.nc + k]};
+struct FaceFlux {
+  T f0, fx, fy, f3, sflux, speed, fcoef, sfcoef;
+  bool ok;
+};
+
+// Field-wise choice between two register records (no address of either is taken).
+template <class T>
+__device__ __forceinline__ CellPrim<T> pick(bool first, const CellPrim<T>& a, const CellPrim<T>& b) {
+  return CellPrim<T>{first ? a.rho : b.rho, first ? a.u : b.u,     first ? a.v : b.v,      first ? a.p : b.p,
+                     first ? a.phi : b.phi, first ? a.mu : b.mu, first ? a.eddy : b.eddy};
+}
+template <class T>
+__device__ __forceinline__ CellGrad<T> pick(bool first, const CellGrad<T>& a, const CellGrad<T>& b) {
+  return CellGrad<T>{first ? a.ux : b.ux, first ? a.uy : b.uy, first ? a.vx : b.vx,         first ? a.vy : b.vy,
+                     first ? a.sx : b.sx, first ? a.sy : b.sy, first ? a.sdiff : b.sdiff};
 }
 
-// One face: boundary state, inviscid flux in the face frame, viscous and SA face fluxes.
-// Everything stays in registers: the face-normal component is chosen with selects on the
-// axis, never by indexing a local array (which would put it in scratch memory).
-template <class T, int PASS>
-__device__ __forceinline__ void face_point(const ArenaPtrs<T>& a, const MeshDesc<T>& d, Control* c, int q) {
-  const int nc = a.nc, nf = a.nf, bit = PASS == PASS_BASE ? BAD_BASE : BAD_STAGE;
-  const int l = a.face_l[q], r = a.face_r[q], info = a.face_info[q];
+// L, R (and Lg, Rg) are the records of the face's left and right cells; on a boundary face both
+// are the inside cell's.  `in`, `ing`: the evaluating cell, the inside cell of a boundary face.
+template <class T, bool BASE>
+__device__ __forceinline__ FaceFlux<T> face_flux(const MeshDesc<T>& d, int4 fi, const FaceGeo<T>& geo,
+                                                 const CellPrim<T>& in, const CellGrad<T>& ing, const CellPrim<T>& L,
+                                                 const CellPrim<T>& R, const CellGrad<T>& Lg, const CellGrad<T>& Rg) {
+  FaceFlux<T> o{};
+  o.ok = true;
+  const int l = fi.x, r = fi.y, info = fi.z;
   const int axis = info & 1, side = (info >> 1) - 1;
   const bool xaxis = axis == 0;
-  const T len = a.face_len[q];
-  const int inside = l >= 0 ? l : r;
-  Prim<T> wl = load_prim(a, l >= 0 ? l : inside);
-  Prim<T> wr = load_prim(a, r >= 0 ? r : inside);
+  const T len = geo.len;
+  Prim<T> wl{L.rho, L.u, L.v, L.p};
+  Prim<T> wr{R.rho, R.u, R.v, R.p};
   int kind = -1;
   if (side >= 0) {
+    wl = Prim<T>{in.rho, in.u, in.v, in.p};
+    wr = wl;
     kind = d.side[side].kind;
     const bool ok = l < 0 ? side_state(d.side[side], d.gamma, side, wr, wl) : side_state(d.side[side], d.gamma, side, wl, wr);
     if (!ok) {
-      flag_bad(c, bit);
-      return;
+      o.ok = false;
+      return o;
     }
   }
   // Face frame: n = normal velocity, t = tangential velocity.
@@ -207,7 +296,7 @@ __device__ __forceinline__ void face_point(const ArenaPtrs<T>& a, const MeshDesc
   T f[4];
   if (kind == KIND_NOSLIP) {
     f[0] = T(0);
-    f[1] = a.W[3 * nc + inside];
+    f[1] = in.p;
     f[2] = T(0);
     f[3] = T(0);
   } else if (d.flux == 0) {
@@ -216,40 +305,41 @@ __device__ __forceinline__ void face_point(const ArenaPtrs<T>& a, const MeshDesc
     ausm_flux(wl.r, nL, tL, wl.p, wr.r, nR, tR, wr.p, d.gamma, d.mco, f);
   }
   if (!isfinite(f[0]) || !isfinite(f[1]) || !isfinite(f[2]) || !isfinite(f[3])) {
-    flag_bad(c, bit);
-    return;
+    o.ok = false;
+    return o;
   }
   T F0 = f[0] * len, Fx = (xaxis ? f[1] : f[2]) * len, Fy = (xaxis ? f[2] : f[1]) * len, F3 = f[3] * len;
   if (d.laminar) {
-    // Viscous face flux (reference viscous_face, src/static_amr2d_cuda.cu:92-107).
+    // Viscous face flux (reference viscous_face, src/static_amr2d_cuda.cu:92-107).  For an
+    // interior face the inside cell is the left one.
+    // On an interior face every quantity taken from the inside cell below is replaced.
     const bool sa = d.sa != 0;
-    const int k = inside;
-    T g0 = a.GRAD[k], g1 = a.GRAD[nc + k], g2 = a.GRAD[2 * nc + k], g3 = a.GRAD[3 * nc + k];
-    const T rho_k = a.W[k], u_k = a.W[nc + k], v_k = a.W[2 * nc + k];
-    const T eddy_k = sa ? a.EDDY[k] : T(0);
-    T mu = a.MU[k] + eddy_k, uf = u_k, vf = v_k;
+    const bool interior = side < 0;
+    T g0 = ing.ux, g1 = ing.uy, g2 = ing.vx, g3 = ing.vy;
+    const T rho_k = in.rho, u_k = in.u, v_k = in.v;
+    const T eddy_k = sa ? in.eddy : T(0);
+    T mu = in.mu + eddy_k, uf = u_k, vf = v_k;
     T nu = mu / rho_k;
-    const T dist = a.face_dist[q];
-    if (side < 0) {
-      const T dx = a.face_dx[q], dy = a.face_dy[q], ex = a.face_ex[q], ey = a.face_ey[q];
-      const T gl[4] = {a.GRAD[l], a.GRAD[nc + l], a.GRAD[2 * nc + l], a.GRAD[3 * nc + l]};
-      const T gr[4] = {a.GRAD[r], a.GRAD[nc + r], a.GRAD[2 * nc + r], a.GRAD[3 * nc + r]};
+    const T dist = geo.dist;
+    if (interior) {
+      const T gl[4] = {Lg.ux, Lg.uy, Lg.vx, Lg.vy};
+      const T gr[4] = {Rg.ux, Rg.uy, Rg.vx, Rg.vy};
       T g[4];
-      corrected_gradient(gl, gr, dx, dy, wr.u - wl.u, wr.v - wl.v, g);
+      corrected_gradient(gl, gr, geo.dx, geo.dy, wr.u - wl.u, wr.v - wl.v, g);
       g0 = g[0];
       g1 = g[1];
       g2 = g[2];
       g3 = g[3];
-      uf = T(0.5) * (wl.u + wr.u) + g0 * ex + g1 * ey;
-      vf = T(0.5) * (wl.v + wr.v) + g2 * ex + g3 * ey;
+      uf = T(0.5) * (wl.u + wr.u) + g0 * geo.ex + g1 * geo.ey;
+      vf = T(0.5) * (wl.v + wr.v) + g2 * geo.ex + g3 * geo.ey;
       const T temp = T(0.5) * (wl.p / wl.r + wr.p / wr.r) / d.transport.gas_constant;
-      const T el = sa ? a.EDDY[l] : T(0), er = sa ? a.EDDY[r] : T(0);
+      const T el = sa ? L.eddy : T(0), er = sa ? R.eddy : T(0);
       mu = viscosity(d.transport, temp) + T(0.5) * (el + er);
-      nu = fmax((a.MU[l] + el) / wl.r, (a.MU[r] + er) / wr.r);
+      nu = fmax((L.mu + el) / wl.r, (R.mu + er) / wr.r);
     } else {
       const T sign = (side & 1) ? T(1) : T(-1);
       if (kind == KIND_NOSLIP) {
-        mu = a.MU[k];
+        mu = in.mu;
         uf = d.side[side].wall_velocity[0];
         vf = d.side[side].wall_velocity[1];
         const T du = (uf - u_k) / dist, dv = (vf - v_k) / dist;
@@ -271,33 +361,30 @@ __device__ __forceinline__ void face_point(const ArenaPtrs<T>& a, const MeshDesc
           g1 = T(0);
           g3 = T(0);
         }
-        const T ex = a.face_ex[q], ey = a.face_ey[q];
-        uf += g0 * ex + g1 * ey;
-        vf += g2 * ex + g3 * ey;
+        uf += g0 * geo.ex + g1 * geo.ey;
+        vf += g2 * geo.ex + g3 * geo.ey;
       }
     }
     const T g[4] = {g0, g1, g2, g3};
     T fv[4];
     stress_flux(mu, g, xaxis ? T(1) : T(0), xaxis ? T(0) : T(1), uf, vf, fv);
-    if (!isfinite(fv[1]) || !isfinite(fv[2]) || !isfinite(fv[3])) flag_bad(c, bit);
+    if (!isfinite(fv[1]) || !isfinite(fv[2]) || !isfinite(fv[3])) o.ok = false;
     Fx -= fv[1] * len;
     Fy -= fv[2] * len;
     F3 -= fv[3] * len;
-    if (PASS == PASS_BASE) a.FCOEF[q] = T(4.0 / 3.0) * nu / dist;
+    if (BASE) o.fcoef = T(4.0 / 3.0) * nu / dist;
     if (sa) {
       // SA face flux: upwind nutilde times the mass flux, minus corrected-gradient diffusion
       // (reference sa_face_wall_point, src/static_amr2d_sa_cuda.cuh:16-25).
       const T mass = F0;
-      const int ir = r >= 0 ? r : inside;  // always a valid cell index
-      const T pl = l >= 0 ? a.PHI[inside] : d.transport.sa_nu_inf;
-      const T pr = r >= 0 ? a.PHI[ir] : d.transport.sa_nu_inf;
+      const T pl = l >= 0 ? L.phi : d.transport.sa_nu_inf;
+      const T pr = r >= 0 ? R.phi : d.transport.sa_nu_inf;
       T diff = T(0), fc = T(0);
-      if (side < 0) {
-        const T dx = a.face_dx[q], dy = a.face_dy[q];
-        const T mx = T(0.5) * (a.SGRAD[l] + a.SGRAD[r]), my = T(0.5) * (a.SGRAD[nc + l] + a.SGRAD[nc + r]);
-        const T corr = ((pr - pl) - mx * dx - my * dy) / (dx * dx + dy * dy);
-        const T gn = xaxis ? mx + corr * dx : my + corr * dy;
-        const T coeff = T(0.5) * (wl.r * a.SDIFF[l] + wr.r * a.SDIFF[r]);
+      if (interior) {
+        const T mx = T(0.5) * (Lg.sx + Rg.sx), my = T(0.5) * (Lg.sy + Rg.sy);
+        const T corr = ((pr - pl) - mx * geo.dx - my * geo.dy) / (geo.dx * geo.dx + geo.dy * geo.dy);
+        const T gn = xaxis ? mx + corr * geo.dx : my + corr * geo.dy;
+        const T coeff = T(0.5) * (wl.r * Lg.sdiff + wr.r * Rg.sdiff);
         diff = coeff * gn * len;
         fc = coeff / dist * len;
       } else {
@@ -305,99 +392,87 @@ __device__ __forceinline__ void face_point(const ArenaPtrs<T>& a, const MeshDesc
         const T sign = (side & 1) ? T(1) : T(-1);
         if (kind == KIND_NOSLIP || inflow) {
           const T wallv = kind == KIND_NOSLIP ? T(0) : d.transport.sa_nu_inf;
-          const T coeff = kind == KIND_NOSLIP ? a.MU[inside] * T(1.0 / SaConst<T>::sigma) : rho_k * a.SDIFF[inside];
-          diff = sign * coeff * (wallv - a.PHI[inside]) / dist * len;
+          const T coeff = kind == KIND_NOSLIP ? in.mu * T(1.0 / SaConst<T>::sigma) : rho_k * ing.sdiff;
+          diff = sign * coeff * (wallv - in.phi) / dist * len;
           fc = coeff / dist * len;
         }
       }
-      const T sflux = mass * (mass >= T(0) ? pl : pr) - diff;
-      a.SFLUX[q] = sflux;
-      if (PASS == PASS_BASE) a.SFCOEF[q] = fc;
-      if (!isfinite(sflux)) flag_bad(c, bit);
+      o.sflux = mass * (mass >= T(0) ? pl : pr) - diff;
+      if (BASE) o.sfcoef = fc;
+      if (!isfinite(o.sflux)) o.ok = false;
     }
   }
-  a.FLUX[q] = F0;
-  a.FLUX[nf + q] = Fx;
-  a.FLUX[2 * nf + q] = Fy;
-  a.FLUX[3 * nf + q] = F3;
-  if (PASS == PASS_BASE)
-    a.SPEED[q] = fmax(fabs(nL) + sqrt(d.gamma * wl.p / wl.r), fabs(nR) + sqrt(d.gamma * wr.p / wr.r));
+  o.f0 = F0;
+  o.fx = Fx;
+  o.fy = Fy;
+  o.f3 = F3;
+  if (BASE) o.speed = fmax(fabs(nL) + sqrt(d.gamma * wl.p / wl.r), fabs(nR) + sqrt(d.gamma * wr.p / wr.r));
+  return o;
 }
 
 // No-slip shear on a cut-wall fragment from its donor cell (reference viscous_wall_point,
-// src/static_amr2d_cuda.cu:108-113, and the SA wall part of sa_face_wall_point).
-template <class T, int PASS>
-__device__ __forceinline__ void wall_point(const ArenaPtrs<T>& a, const MeshDesc<T>& d, Control* c, int j) {
-  const int nc = a.nc, k = a.wall_donor[j];
-  const T nx = a.wall_nx[j], ny = a.wall_ny[j], len = a.wall_len[j], dd = a.wall_ddist[j];
-  const T rho = a.W[k], mu = a.MU[k];
-  const T du = -a.W[nc + k] / dd, dv = -a.W[2 * nc + k] / dd;
+// src/static_amr2d_cuda.cu:108-113): fv[1], fv[2] per unit length.
+template <class T>
+__device__ __forceinline__ void wall_shear(const CellPrim<T>& donor, T nx, T ny, T dd, T* fv) {
+  const T du = -donor.u / dd, dv = -donor.v / dd;
   const T g[4] = {nx * du, ny * du, nx * dv, ny * dv};
-  T fv[4];
-  stress_flux(mu, g, nx, ny, T(0), T(0), fv);
-  if (!isfinite(fv[1]) || !isfinite(fv[2])) flag_bad(c, PASS == PASS_BASE ? BAD_BASE : BAD_STAGE);
-  a.WFLUX[j] = fv[1] * len;
-  a.WFLUX[a.nw + j] = fv[2] * len;
-  if (PASS == PASS_BASE) {
-    const int o = a.wall_owner[j];
-    a.WCOEF[j] = T(4.0 / 3.0) * fmax(mu / rho, a.MU[o] / a.W[o]) / dd;
-  }
-  if (d.sa) {
-    const T coeff = mu * T(1.0 / SaConst<T>::sigma) / dd * len;
-    if (PASS == PASS_BASE) a.SWCOEF[j] = coeff;
-    a.SWFLUX[j] = -coeff * a.PHI[k];
-  }
-}
-
-template <class T, int PASS>
-__global__ __launch_bounds__(kMaxTile) void k_face(ArenaPtrs<T> a) {
-  const int b = blockIdx.x;
-  const int m = a.face_block_mesh[b];
-  Control* c = a.ctrl + m;
-  if (!pass_on(c, PASS)) return;
-  const MeshDesc<T>& d = a.desc[m];
-  if (b < a.n_face_blocks) {
-    const int q = b * blockDim.x + threadIdx.x;
-    if (q < d.face_begin + d.n_faces) face_point<T, PASS>(a, d, c, q);
-  } else if (d.laminar) {
-    const int j = (b - a.n_face_blocks) * blockDim.x + threadIdx.x;
-    if (j < d.wall_begin + d.n_walls) wall_point<T, PASS>(a, d, c, j);
-  }
+  stress_flux(donor.mu, g, nx, ny, T(0), T(0), fv);
 }
 
 // ------------------------------------------------------------------------------------------
-// STAGE 0: base residual R0 and local dt.  1, 2: second and third SSP-RK3 combinations.
+// Residual of every cell from its faces and wall fragments.  STAGE 0: base residual R0 and the
+// local dt.  1, 2: second and third SSP-RK3 combinations, then the candidate's primitives.
 template <class T, int STAGE>
-__global__ __launch_bounds__(kMaxTile) void k_collect(ArenaPtrs<T> a) {
+__global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
+  constexpr bool BASE = STAGE == 0;
   const int m = a.cell_block_mesh[blockIdx.x];
   Control* c = a.ctrl + m;
-  if (!pass_on(c, STAGE == 0 ? PASS_BASE : PASS_STAGE)) return;
+  if (!pass_on(c, BASE ? PASS_BASE : PASS_STAGE)) return;
   const MeshDesc<T>& d = a.desc[m];
-  const int nc = a.nc, nf = a.nf, nw = a.nw;
+  const int nc = a.nc;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = k < d.cell_begin + d.n_cells;
   const bool sa = d.sa != 0, laminar = d.laminar != 0;
+  const int rb = read_buffer(c, STAGE);
+  const int bit = BASE ? BAD_BASE : BAD_STAGE;
   double local_dt = INFINITY;
   if (valid) {
+    const T* rec = a.CREC[rb];
+    const CellPrim<T> own = load_prim(rec + kRec * k);
+    CellGrad<T> owng{};
+    if (laminar) owng = load_grad(rec + kRec * k);
     T s0 = 0, s1 = 0, s2 = 0, s3 = 0, spectral = 0, diffusion = 0, sa_net = 0, sa_spec = 0;
+    bool ok = true;
     const int end = a.csr_off[k + 1];
     for (int j = a.csr_off[k]; j < end; ++j) {
       const int sq = a.csr_face[j];
-      const int q = sq >= 0 ? sq : ~sq;
-      const T sgn = sq >= 0 ? T(1) : T(-1);
-      s0 += sgn * a.FLUX[q];
-      s1 += sgn * a.FLUX[nf + q];
-      s2 += sgn * a.FLUX[2 * nf + q];
-      s3 += sgn * a.FLUX[3 * nf + q];
-      if (sa) sa_net += sgn * a.SFLUX[q];
-      if (STAGE == 0) {
-        const T len = a.face_len[q];
-        spectral += a.SPEED[q] * len;
-        if (laminar) diffusion += a.FCOEF[q] * len;
-        if (sa) sa_spec += a.SFCOEF[q];
+      const bool self_left = sq >= 0;
+      const int q = self_left ? sq : ~sq;
+      const int4 fi = a.FIDX[q];
+      const FaceGeo<T> geo = load_geo(a.FGEO + 8 * q);
+      const int other = self_left ? fi.y : fi.x;  // -1 on a boundary face
+      CellPrim<T> nb = own;
+      CellGrad<T> nbg = owng;
+      if (other >= 0) {
+        nb = load_prim(rec + kRec * other);
+        if (laminar) nbg = load_grad(rec + kRec * other);
+      }
+      const FaceFlux<T> f = face_flux<T, BASE>(d, fi, geo, own, owng, pick(self_left, own, nb), pick(self_left, nb, own),
+                                               pick(self_left, owng, nbg), pick(self_left, nbg, owng));
+      ok = ok && f.ok;
+      const T sgn = self_left ? T(1) : T(-1);
+      s0 += sgn * f.f0;
+      s1 += sgn * f.fx;
+      s2 += sgn * f.fy;
+      s3 += sgn * f.f3;
+      if (sa) sa_net += sgn * f.sflux;
+      if (BASE) {
+        spectral += f.speed * geo.len;
+        if (laminar) diffusion += f.fcoef * geo.len;
+        if (sa) sa_spec += f.sfcoef;
       }
     }
-    const T rho = a.W[k], p = a.W[3 * nc + k];
+    const T rho = own.rho, p = own.p;
     const int w0 = a.wall_first[k], w1 = w0 + a.wall_cnt[k];
     if (w1 > w0) {
       const T snd = sqrt(d.gamma * p / rho);
@@ -405,26 +480,34 @@ __global__ __launch_bounds__(kMaxTile) void k_collect(ArenaPtrs<T> a) {
         const T len = a.wall_len[j];
         s1 += p * a.wall_nx[j] * len;
         s2 += p * a.wall_ny[j] * len;
-        if (STAGE == 0) spectral += snd * len;
+        if (BASE) spectral += snd * len;
         if (laminar) {
-          s1 -= a.WFLUX[j];
-          s2 -= a.WFLUX[nw + j];
-          if (STAGE == 0) diffusion += a.WCOEF[j] * len;
-        }
-        if (sa) {
-          sa_net -= a.SWFLUX[j];
-          if (STAGE == 0) sa_spec += a.SWCOEF[j];
+          const int dn = a.wall_donor[j];
+          const CellPrim<T> donor = dn == k ? own : load_prim(rec + kRec * dn);
+          const T dd = a.wall_ddist[j];
+          T fv[4];
+          wall_shear(donor, a.wall_nx[j], a.wall_ny[j], dd, fv);
+          if (!isfinite(fv[1]) || !isfinite(fv[2])) ok = false;
+          s1 -= fv[1] * len;
+          s2 -= fv[2] * len;
+          if (BASE) diffusion += T(4.0 / 3.0) * fmax(donor.mu / donor.rho, own.mu / rho) / dd * len;
+          if (sa) {
+            const T coeff = donor.mu * T(1.0 / SaConst<T>::sigma) / dd * len;
+            sa_net -= -coeff * donor.phi;
+            if (BASE) sa_spec += coeff;
+          }
         }
       }
     }
+    if (!ok) flag_bad(c, bit);
     const T iv = a.inv_vol[k];
     const T r0 = -s0 * iv, r1 = -s1 * iv, r2 = -s2 * iv, r3 = -s3 * iv;
     T r4 = T(0);
     if (sa) {
       r4 = -sa_net * iv + a.SSRC[k];
-      if (!isfinite(r4)) flag_bad(c, STAGE == 0 ? BAD_BASE : BAD_STAGE);
+      if (!isfinite(r4)) flag_bad(c, bit);
     }
-    if (STAGE == 0) {
+    if (BASE) {
       a.R0[k] = r0;
       a.R0[nc + k] = r1;
       a.R0[2 * nc + k] = r2;
@@ -461,13 +544,13 @@ __global__ __launch_bounds__(kMaxTile) void k_collect(ArenaPtrs<T> a) {
       out[3 * nc + k] = wu * u[3 * nc + k] + wp * (prev[3 * nc + k] + dt * static_cast<double>(r3));
       if (sa) out[4 * nc + k] = wu * u[4 * nc + k] + wp * (prev[4 * nc + k] + dt * static_cast<double>(r4));
       if (!d.small) {
-        T w[4], phi = T(0);
-        if (!to_primitive(out, nc, k, d.gamma, sa, w, &phi)) flag_bad(c, BAD_STAGE);
-        else store_primitive(a, k, w, phi, sa);
+        CellPrim<T> w;
+        if (!make_prim(out, nc, k, d, w)) flag_bad(c, BAD_STAGE);
+        else store_prim(a.CREC[rb ^ 1] + kRec * k, w);
       }
     }
   }
-  if (STAGE == 0 && d.visc_mode != VISC_SRD_GROUP) publish_dt(c, local_dt);
+  if (BASE && d.visc_mode != VISC_SRD_GROUP) publish_dt(c, local_dt);
 }
 
 // SRD-grouped viscous limit: group means of the diffusion rate, then the per-cell limit
@@ -545,9 +628,9 @@ __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
   double* out = a.U[cur ^ 1];
   for (int v = 0; v < d.nv; ++v) out[v * nc + k] = u[v * nc + k] + dt * static_cast<double>(a.R0[v * nc + k]);
   if (!d.small) {
-    T w[4], phi = T(0);
-    if (!to_primitive(out, nc, k, d.gamma, d.sa != 0, w, &phi)) flag_bad(c, BAD_STAGE);
-    else store_primitive(a, k, w, phi, d.sa != 0);
+    CellPrim<T> w;
+    if (!make_prim(out, nc, k, d, w)) flag_bad(c, BAD_STAGE);
+    else store_prim(a.CREC[c->wbuf ^ 1] + kRec * k, w);
   }
 }
 
@@ -603,9 +686,9 @@ __global__ __launch_bounds__(kMaxTile) void k_redist(ArenaPtrs<T> a, int stage) 
   u[nc + k] = s1 * inv;
   u[2 * nc + k] = s2 * inv;
   u[3 * nc + k] = s3 * inv;
-  T w[4], phi = T(0);
-  if (!to_primitive(u, nc, k, d.gamma, d.sa != 0, w, &phi)) flag_bad(c, BAD_STAGE);
-  else store_primitive(a, k, w, phi, d.sa != 0);
+  CellPrim<T> w;
+  if (!make_prim(u, nc, k, d, w)) flag_bad(c, BAD_STAGE);
+  else store_prim(a.CREC[read_buffer(c, stage) ^ 1] + kRec * k, w);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -641,7 +724,7 @@ __global__ void k_commit(Control* ctrl, int n) {
       if (c->attempt >= 21) c->fatal = 5;
       c->w_valid = 0;
       c->r0_valid = 1;
-    } else {  // accept: the candidate buffer becomes the state
+    } else {  // accept: the candidate buffers become the state
       const double remaining = c->target - c->time;
       c->time = c->trial_dt == remaining ? c->target : c->time + c->trial_dt;
       ++c->steps;
@@ -649,6 +732,7 @@ __global__ void k_commit(Control* ctrl, int n) {
       c->retrying = 0;
       c->attempt = 0;
       c->cur ^= 1;
+      c->wbuf ^= 1;
       c->w_valid = 1;
       c->r0_valid = 0;
       c->finished = c->time >= c->target;
@@ -711,7 +795,7 @@ __global__ __launch_bounds__(kMaxTile) void k_monitor_cells(ArenaPtrs<T> a) {
 }
 
 // Steady monitor, walls: pressure force minus viscous shear and the moment about (mx, my)
-// (reference src/quick_airfoil.cpp:83-85).
+// (reference src/quick_airfoil.cpp:83-85), from the records of the accepted state.
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_monitor_walls(ArenaPtrs<T> a) {
   const int b = blockIdx.x;
@@ -720,13 +804,18 @@ __global__ __launch_bounds__(kMaxTile) void k_monitor_walls(ArenaPtrs<T> a) {
   const int j = b * blockDim.x + threadIdx.x;
   double acc[3] = {0, 0, 0};
   if (j < d.wall_begin + d.n_walls) {
+    const T* rec = a.CREC[a.ctrl[m].wbuf];
     const double mx = a.mon_frame[6 * m + 4], my = a.mon_frame[6 * m + 5];
-    const int k = a.wall_owner[j];
-    const double p = static_cast<double>(a.W[3 * a.nc + k]), len = static_cast<double>(a.wall_len[j]);
+    const CellPrim<T> own = load_prim(rec + kRec * a.wall_owner[j]);
+    const T len_t = a.wall_len[j];
+    const double p = static_cast<double>(own.p), len = static_cast<double>(len_t);
     double fx = p * static_cast<double>(a.wall_nx[j]) * len, fy = p * static_cast<double>(a.wall_ny[j]) * len;
     if (d.laminar) {
-      fx -= static_cast<double>(a.WFLUX[j]);
-      fy -= static_cast<double>(a.WFLUX[a.nw + j]);
+      const CellPrim<T> donor = load_prim(rec + kRec * a.wall_donor[j]);
+      T fv[4];
+      wall_shear(donor, a.wall_nx[j], a.wall_ny[j], a.wall_ddist[j], fv);
+      fx -= static_cast<double>(fv[1] * len_t);
+      fy -= static_cast<double>(fv[2] * len_t);
     }
     acc[0] = fx;
     acc[1] = fy;
@@ -751,12 +840,11 @@ inline int blocks_of(int n, int tile) { return (n + tile - 1) / tile; }
 
 template <class T>
 int launch_residual(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
-  const dim3 tile(a.tile), cells(a.n_cell_blocks), faces(a.n_face_blocks + a.n_wall_blocks);
+  const dim3 tile(a.tile), cells(a.n_cell_blocks);
   int launches = 0;
   HIPAMR_LAUNCH("base_primitive", k_base_prim<T>, cells, a);
-  if (a.any_laminar) HIPAMR_LAUNCH("gradient_base", (k_grad<T, PASS_BASE>), cells, a);
-  HIPAMR_LAUNCH("face_base", (k_face<T, PASS_BASE>), faces, a);
-  HIPAMR_LAUNCH("collect_base", (k_collect<T, 0>), cells, a);
+  if (a.any_laminar) HIPAMR_LAUNCH("gradient_base", (k_grad<T, 0>), cells, a);
+  HIPAMR_LAUNCH("flux_base", (k_flux<T, 0>), cells, a);
   if (a.any_visc_group) {
     HIPAMR_LAUNCH("diffusion_group", k_diffusion_group<T>, cells, a);
     HIPAMR_LAUNCH("dt_group", k_dt_group<T>, cells, a);
@@ -766,22 +854,24 @@ int launch_residual(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
 
 template <class T>
 int launch_trial(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
-  const dim3 tile(a.tile), cells(a.n_cell_blocks), faces(a.n_face_blocks + a.n_wall_blocks);
+  const dim3 tile(a.tile), cells(a.n_cell_blocks);
   int launches = launch_residual(a, s, hook);
   HIPAMR_LAUNCH("stage0", k_stage0<T>, cells, a);
   if (a.any_small) {
     HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 0);
     HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 0);
   }
-  for (int stage = 1; stage <= 2; ++stage) {
-    if (a.any_laminar) HIPAMR_LAUNCH("gradient_stage", (k_grad<T, PASS_STAGE>), cells, a);
-    HIPAMR_LAUNCH("face_stage", (k_face<T, PASS_STAGE>), faces, a);
-    if (stage == 1) HIPAMR_LAUNCH("collect_rk1", (k_collect<T, 1>), cells, a);
-    else HIPAMR_LAUNCH("collect_rk2", (k_collect<T, 2>), cells, a);
-    if (a.any_small) {
-      HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, stage);
-      HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, stage);
-    }
+  if (a.any_laminar) HIPAMR_LAUNCH("gradient_stage", (k_grad<T, 1>), cells, a);
+  HIPAMR_LAUNCH("flux_rk1", (k_flux<T, 1>), cells, a);
+  if (a.any_small) {
+    HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 1);
+    HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 1);
+  }
+  if (a.any_laminar) HIPAMR_LAUNCH("gradient_stage", (k_grad<T, 2>), cells, a);
+  HIPAMR_LAUNCH("flux_rk2", (k_flux<T, 2>), cells, a);
+  if (a.any_small) {
+    HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 2);
+    HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 2);
   }
   {
     const dim3 tile(256);
