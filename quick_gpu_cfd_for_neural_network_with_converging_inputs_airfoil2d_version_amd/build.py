@@ -17,6 +17,8 @@ PACKAGE = Path(__file__).resolve().parent
 ROOT = PACKAGE.parent
 CSRC = PACKAGE / "csrc"
 OUT = PACKAGE / "libamr_hip.so"
+DRIVER = PACKAGE / "quick_airfoil_hip"
+DRIVER_SOURCE = "host/quick_airfoil_hip.cpp"
 OBJ = ROOT / "_build" / "obj"
 
 HOST_SOURCES = ["host/cutcell.cpp", "host/mesher.cpp", "host/viscous_prep.cpp", "host/mesh_capi.cpp",
@@ -63,6 +65,14 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     tmp = OUT.with_suffix(".so.tmp")
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objects), "-o", str(tmp)], verbose)
     tmp.replace(OUT)
+    # The solver executable (the reference's quick_airfoil command line) over the library.
+    driver_obj = OBJ / "quick_airfoil_hip.o"
+    _run(["g++", "-std=c++17", "-O3", "-DNDEBUG", "-ffp-contract=off", "-Wall", *INCLUDES, "-c",
+          str(CSRC / DRIVER_SOURCE), "-o", str(driver_obj)], verbose)
+    tmp = DRIVER.with_suffix(".tmp")
+    _run(["g++", str(driver_obj), "-L", str(PACKAGE), "-l:libamr_hip.so", "-L", "/opt/rocm/lib",
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", str(tmp)], verbose)
+    tmp.replace(DRIVER)
     return OUT
 
 
