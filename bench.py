@@ -72,6 +72,88 @@ def build_one(spec: dict, preset: str = "LF2"):
     return case, mesh, params, mesh.initialize(params, case.freestream)
 
 
+SERVE_MIX = (("LF1", 0.50, 48_000), ("LF2", 0.35, 90_000), ("L7", 0.15, 180_000))  # preset, share, ~steps
+
+
+def serve_workload(args) -> dict:
+    """Continuous batching: requests (config-3 airfoils on LF1 / LF2 / level-7 meshes, 50/35/15 %)
+    arrive as a Poisson stream and are admitted into an arena of slot classes sized by the HBM
+    budget; each runs to the steady criteria.  Reports p50/p99 time-to-steady and solves/hour."""
+    from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import batch as B
+    from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import config as CF
+    from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import geometry as G
+    from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import serve as S
+
+    rng = np.random.default_rng(args.seed)
+    presets = rng.choice([p for p, _, _ in SERVE_MIX], size=args.serve, p=[w for _, w, _ in SERVE_MIX])
+    specs = draw_airfoils(args.serve, args.seed + 1)
+    requests = []
+    for i, (spec, preset) in enumerate(zip(specs, presets)):
+        case = CF.make_case(str(preset), {"freestream": {"Mach": spec["mach"], "AOA_degrees": spec["aoa"]}})
+        poly = G.reorder_points(G.naca4_points(spec["thickness"], spec["camber"], spec["position"], 100))
+        requests.append(S.Request(i, poly, case))
+    # Slot classes: capacity of each preset from sample meshes (+5 %), counts proportional to the
+    # expected residence (share x steps), within the HBM budget.
+    caps, feats = {}, []
+    for preset, _, _ in SERVE_MIX:
+        sample = [build_one(s, preset)[1] for s in draw_airfoils(4, args.seed + 7)]
+        cs, fs = zip(*[B.capacity(m, args.tile or 256) for m in sample])
+        caps[preset] = B.envelope(cs, margin=0.05, tile=args.tile or 256)
+        feats += fs
+    weight = {p: w * st for p, w, st in SERVE_MIX}
+    total = sum(weight.values())
+    counts = {p: max(1, int(round(args.serve_slots * weight[p] / total))) for p in weight}
+    slot_caps = [caps[p] for p, _, _ in SERVE_MIX for _ in range(counts[p])]
+    features = B.union(feats)
+    budget = int(args.hbm_budget_gb * 1e9)
+    while B.arena_bytes(slot_caps, features, 32, args.tile or 256) > budget and len(slot_caps) > 3:
+        slot_caps = slot_caps[: int(len(slot_caps) * 0.9)]
+    arena = B.arena_bytes(slot_caps, features, 32, args.tile or 256)
+    log(f"serving arena: {len(slot_caps)} slots {counts}, {arena / 1e9:.2f} GB of a {args.hbm_budget_gb:.0f} GB budget")
+    backend = S.DeviceBackend(slot_caps, features, precision=32, tile=args.tile or 256)
+    try:
+        # Capacity estimate: step time of the arena filled with sample meshes.
+        kinds = [p for p, _, _ in SERVE_MIX for _ in range(counts[p])][: len(slot_caps)]
+        with ThreadPoolExecutor(args.workers) as pool:
+            fill = list(pool.map(build_one, draw_airfoils(len(kinds), args.seed + 9), kinds))
+        for slot, (case, mesh, params, u0) in enumerate(fill[: len(slot_caps)]):
+            backend.batch.admit(slot, mesh, params, u0)
+        backend.batch.set_targets(1e30)
+        backend.batch.run(10)
+        t0 = time.perf_counter()
+        backend.batch.run(50)
+        step = (time.perf_counter() - t0) / 50
+        for slot in range(len(slot_caps)):
+            backend.batch.release(slot)
+        mean_steps = sum(w * st for _, w, st in SERVE_MIX)
+        capacity = len(slot_caps) / (mean_steps * step)
+        rate = args.serve_load * capacity
+        log(f"arena step {step * 1e3:.3f} ms -> capacity ~{capacity:.3f} requests/s; arrival rate {rate:.3f}/s")
+        arrivals = S.poisson_arrivals(rate, args.serve, seed=args.seed + 3)
+        for req, t in zip(requests, arrivals):
+            req.arrival = float(t)
+        server = S.Server(backend, mode="steady", budget=3600.0, workers=args.workers, poll_seconds=0.2,
+                          log=lambda m: log(m))
+        report = server.serve(requests)
+    finally:
+        backend.close()
+    outcomes = report.pop("outcomes")
+    lat = {p: [o.latency for o, r in zip(outcomes, requests) if not o.error and str(presets[r.rid]) == p]
+           for p, _, _ in SERVE_MIX}
+    return {
+        "metric": METRIC, "value": round(report["solves_per_hour"], 2), "unit": "solves/hour", "n_gpus": 1,
+        "steps": None, "warmup": None, "ms_per_step": round(step * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": f"synthetic: {args.serve} seeded airfoils (seed {args.seed}), Poisson arrivals",
+        "config": {"workload": f"serving: {args.serve} steady LF1/LF2/L7 SA-neg requests, continuous batching",
+                   "slots": len(slot_caps), "slot_classes": counts, "arena_bytes": arena,
+                   "hbm_budget_gb": args.hbm_budget_gb, "arrival_rate_per_s": rate, "parallelism": "single GPU"},
+        "detail": {**{k: (round(v, 4) if isinstance(v, float) else v) for k, v in report.items()},
+                   "p50_by_preset": {p: S.percentile(v, 50) for p, v in lat.items()},
+                   "p99_by_preset": {p: S.percentile(v, 99) for p, v in lat.items()}},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +170,11 @@ def main():
     ap.add_argument("--graph", action=argparse.BooleanOptionalAction, default=True,
                     help="replay captured hipGraphs in the timed steps (--no-graph: direct launches)")
     ap.add_argument("--tile", type=int, default=0, help="launch tile (0: autotune over 64/128/256)")
+    ap.add_argument("--serve", type=int, default=0,
+                    help="instead of the batch workload: serve this many Poisson-arriving requests (LF1/LF2/L7 mix)")
+    ap.add_argument("--serve-slots", type=int, default=192, help="slots of the serving arena")
+    ap.add_argument("--serve-load", type=float, default=0.8, help="arrival rate as a fraction of estimated capacity")
+    ap.add_argument("--hbm-budget-gb", type=float, default=200.0, help="device memory the serving arena may use")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -106,6 +193,9 @@ def main():
     if device_count() < 1:
         raise SystemExit("bench.py needs a HIP device")
     lib = N.load_library()
+    if args.serve:
+        print(json.dumps(serve_workload(args)), flush=True)
+        return
 
     # 0. Workload: seeded airfoils, LF2 meshes built on host threads (the mesher releases the GIL).
     specs = draw_airfoils(args.batch, args.seed)

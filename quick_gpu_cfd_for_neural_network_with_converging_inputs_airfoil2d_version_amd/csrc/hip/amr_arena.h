@@ -101,10 +101,13 @@ struct ArenaPtrs {
   double* U2;
   double* QHAT;  // SRD group averages, 4 planes
   T* R0;
-  // Cell records, double-buffered by stage (16 T per cell, AoS so that a face reads each of
-  // its cells in whole 16-byte vectors): primitives rho, u, v, p, nutilde, mu, eddy viscosity,
-  // -, then velocity gradient ux, uy, vx, vy, nutilde gradient x, y, SA diffusion, -.
-  T* CREC[2];
+  // Cell vectors gathered by faces and gradient stencils (AoS per purpose, so that a gather
+  // costs whole 16-byte loads): VREC = rho, u, v, nutilde and PREC = p, mu, eddy viscosity, -
+  // (double-buffered by stage: a stage reads one buffer and writes the next state's into the
+  // other), GREC = ux, uy, vx, vy, nutilde_x, nutilde_y, SA diffusion, - (per stage).
+  T* VREC[2];
+  T* PREC[2];
+  T* GREC;
   T* SSRC;    // SA-neg source (own cell only)
   T* SRATE;   // SA-neg source rate
   T* CELL_DT; // deferred local dt (SRD-grouped viscous limit only)

@@ -102,7 +102,7 @@ Totals totals_of(const std::vector<SlotCapacity>& caps, int tile) {
 // least one element and starts on a 256-byte boundary.
 struct SlabPlan {
   size_t geometry, control, work, total;
-  size_t cplane, dplane, crec, mon_cell, mon_wall;
+  size_t cplane, dplane, vec4, vec8, mon_cell, mon_wall;
 };
 
 template <class T>
@@ -118,15 +118,83 @@ SlabPlan plan_slab(const Totals& t, int nv) {
               arr(6 * sizeof(double), t.meshes);
   p.cplane = arr(sizeof(T), t.cells);
   p.dplane = arr(sizeof(double), t.cells);
-  p.crec = arr(16 * sizeof(T), t.cells);
+  p.vec4 = arr(4 * sizeof(T), t.cells);
+  p.vec8 = arr(8 * sizeof(T), t.cells);
   p.mon_cell = arr(6 * sizeof(double), t.cell_blocks);
   p.mon_wall = arr(3 * sizeof(double), t.wall_blocks);
-  // FP64 state U[2], U2 (nv planes each) and QHAT (4); two record buffers of 16 T per cell;
-  // T planes R0 (nv), SSRC, SRATE, CELL_DT, DRATE, DGROUP; monitor partials.
+  // FP64 state U[2], U2 (nv planes each) and QHAT (4); cell vectors VREC[2], PREC[2] (4 T)
+  // and GREC (8 T); T planes R0 (nv), SSRC, SRATE, CELL_DT, DRATE, DGROUP; monitor partials.
   const size_t state_planes = 3 * nv + 4, cell_planes = nv + 5;
-  p.work = state_planes * p.dplane + 2 * p.crec + cell_planes * p.cplane + p.mon_cell + p.mon_wall;
+  p.work = state_planes * p.dplane + 4 * p.vec4 + p.vec8 + cell_planes * p.cplane + p.mon_cell + p.mon_wall;
   p.total = p.geometry + p.control + p.work;
   return p;
+}
+
+// Distance of (x, y) along the Hilbert curve of a 2^16 x 2^16 grid (the classical iterative
+// xy -> d conversion: per bit, pick the quadrant, then rotate / reflect the frame).
+uint64_t hilbert_index(uint32_t x, uint32_t y) {
+  uint64_t d = 0;
+  for (uint32_t s = 1u << 15; s > 0; s >>= 1) {
+    const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+    d += static_cast<uint64_t>(s) * s * ((3u * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = 0xFFFFu - x;
+        y = 0xFFFFu - y;
+      }
+      const uint32_t t = x;
+      x = y;
+      y = t;
+    }
+  }
+  return d;
+}
+
+// Storage order of a mesh on the device: cells along the Hilbert curve of their centroids
+// (neighbours land in nearby tiles and cache lines), faces by the smaller storage index of their
+// cells.  cell_pos[old] = storage position, cell_at[pos] = old; face_pos / face_at likewise.
+struct MeshOrder {
+  std::vector<int> cell_pos, cell_at, face_pos, face_at;
+};
+
+MeshOrder order_of(const amr_mesh_view& v) {
+  const int n = static_cast<int>(v.n_cells), nf = static_cast<int>(v.n_faces);
+  double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+  for (int k = 0; k < n; ++k) {
+    x0 = std::min(x0, v.cells[k].cx);
+    x1 = std::max(x1, v.cells[k].cx);
+    y0 = std::min(y0, v.cells[k].cy);
+    y1 = std::max(y1, v.cells[k].cy);
+  }
+  const double sx = x1 > x0 ? 65535.0 / (x1 - x0) : 0.0, sy = y1 > y0 ? 65535.0 / (y1 - y0) : 0.0;
+  std::vector<std::pair<uint64_t, int>> key(n);
+  for (int k = 0; k < n; ++k) {
+    const uint32_t qx = static_cast<uint32_t>(std::min(65535.0, std::max(0.0, (v.cells[k].cx - x0) * sx)));
+    const uint32_t qy = static_cast<uint32_t>(std::min(65535.0, std::max(0.0, (v.cells[k].cy - y0) * sy)));
+    key[k] = {hilbert_index(qx, qy), k};
+  }
+  std::sort(key.begin(), key.end());
+  MeshOrder o;
+  o.cell_at.resize(n);
+  o.cell_pos.resize(n);
+  for (int i = 0; i < n; ++i) {
+    o.cell_at[i] = key[i].second;
+    o.cell_pos[key[i].second] = i;
+  }
+  std::vector<std::pair<int64_t, int>> fkey(nf);
+  for (int q = 0; q < nf; ++q) {
+    const int l = v.faces[q].left, r = v.faces[q].right;
+    const int a = l >= 0 ? o.cell_pos[l] : n, b = r >= 0 ? o.cell_pos[r] : n;
+    fkey[q] = {static_cast<int64_t>(std::min(a, b)) * (n + 1) + std::max(a, b), q};
+  }
+  std::sort(fkey.begin(), fkey.end());
+  o.face_at.resize(nf);
+  o.face_pos.resize(nf);
+  for (int i = 0; i < nf; ++i) {
+    o.face_at[i] = fkey[i].second;
+    o.face_pos[fkey[i].second] = i;
+  }
+  return o;
 }
 
 template <class T>
@@ -244,6 +312,7 @@ class DeviceArena final : public Arena {
     sized(h.wall_mx, nw, 0.0);
     sized(h.wall_my, nw, 0.0);
     sa_nu_inf_.assign(slots_.size(), 0.0);
+    cell_pos_.assign(slots_.size(), {});
     for (size_t m = 0; m < slots_.size(); ++m) clear_slot(static_cast<int>(m));
     // The sentinel after the last slot.
     h.csr_off[nc] = ncsr;
@@ -279,17 +348,21 @@ class DeviceArena final : public Arena {
     const int cb = s.cell_begin, fb = s.face_begin, wb = s.wall_begin;
     const int n = static_cast<int>(v.n_cells), nf = static_cast<int>(v.n_faces), nw = static_cast<int>(v.n_walls);
     const bool laminar = v.transport.model == 1, sa = v.transport.sa_model != 0;
+    const MeshOrder order = order_of(v);
+    const std::vector<int>& at = order.cell_at;
+    const std::vector<int>& pos_of = order.cell_pos;
     int csr = s.csr_begin, term = s.term_begin, grp = s.grp_begin, mem = s.mem_begin;
-    for (int k = 0; k < s.cap.cells; ++k) {
-      const int g = cb + k;
+    for (int kk = 0; kk < s.cap.cells; ++kk) {
+      const int g = cb + kk;
       h.csr_off[g] = csr;
       h.term_off[g] = term;
       h.grp_off[g] = grp;
       h.mem_off[g] = mem;
-      if (k >= n) {
+      if (kk >= n) {
         pad_cell(g, wb);
         continue;
       }
+      const int k = at[kk];  // the mesh's own index of the cell stored at kk
       const amr_cell& c = v.cells[k];
       h.vol[g] = static_cast<T>(c.volume);
       h.inv_vol[g] = static_cast<T>(1.0 / c.volume);
@@ -301,12 +374,13 @@ class DeviceArena final : public Arena {
       h.cell_cy[g] = c.cy;
       for (uint64_t pos = v.face_offsets[k]; pos < v.face_offsets[k + 1]; ++pos) {
         const int q = static_cast<int>(v.cell_faces[pos]);
-        h.csr_face[csr++] = v.faces[q].left == k ? fb + q : ~(fb + q);
+        const int stored = fb + order.face_pos[q];
+        h.csr_face[csr++] = v.faces[q].left == k ? stored : ~stored;
       }
       if (laminar)
         for (uint64_t pos = v.visc_offsets[k]; pos < v.visc_offsets[k + 1]; ++pos) {
           const amr_grad_term& t = v.visc_terms[pos];
-          h.term_src[term] = t.source >= 0 ? cb + t.source : (t.source == -1 ? -1 : -2 - t.side);
+          h.term_src[term] = t.source >= 0 ? cb + pos_of[t.source] : (t.source == -1 ? -1 : -2 - t.side);
           h.term_wx[term] = static_cast<T>(t.x);
           h.term_wy[term] = static_cast<T>(t.y);
           ++term;
@@ -316,24 +390,25 @@ class DeviceArena final : public Arena {
       for (uint64_t pos = v.group_offsets[k]; pos < v.group_offsets[k + 1]; ++pos) {
         const uint64_t member = v.group_members[pos];
         const uint64_t n_m = v.member_offsets[member + 1] - v.member_offsets[member];
-        h.grp_mem[grp] = cb + static_cast<int>(member);
+        h.grp_mem[grp] = cb + pos_of[member];
         h.grp_w[grp] = static_cast<T>(v.cells[member].volume / static_cast<double>(n_m) / v.group_volume[k]);
         ++grp;
       }
       for (uint64_t pos = v.member_offsets[k]; pos < v.member_offsets[k + 1]; ++pos)
-        h.mem_grp[mem++] = cb + static_cast<int>(v.member_groups[pos]);
+        h.mem_grp[mem++] = cb + pos_of[v.member_groups[pos]];
     }
-    for (int q = 0; q < s.cap.faces; ++q) {
-      const int g = fb + q;
-      if (q >= nf) {
+    for (int qq = 0; qq < s.cap.faces; ++qq) {
+      const int g = fb + qq;
+      if (qq >= nf) {
         pad_face(g, cb);
         continue;
       }
+      const int q = order.face_at[qq];
       const amr_face& face = v.faces[q];
       int* fi = &h.face_idx[4 * static_cast<size_t>(g)];
       T* fg = &h.face_geo[8 * static_cast<size_t>(g)];
-      fi[0] = face.left >= 0 ? cb + face.left : -1;
-      fi[1] = face.right >= 0 ? cb + face.right : -1;
+      fi[0] = face.left >= 0 ? cb + pos_of[face.left] : -1;
+      fi[1] = face.right >= 0 ? cb + pos_of[face.right] : -1;
       fi[2] = (face.axis & 1) | ((face.boundary + 1) << 1);
       fi[3] = 0;
       fg[0] = static_cast<T>(face.length);
@@ -357,8 +432,8 @@ class DeviceArena final : public Arena {
         continue;
       }
       const amr_wall& w = v.walls[j];
-      h.wall_owner[g] = cb + static_cast<int>(w.owner_cell);
-      h.wall_donor[g] = laminar ? cb + v.wall_samples[j].donor : cb + static_cast<int>(w.owner_cell);
+      h.wall_owner[g] = cb + pos_of[w.owner_cell];
+      h.wall_donor[g] = cb + pos_of[laminar ? v.wall_samples[j].donor : static_cast<int>(w.owner_cell)];
       h.wall_nx[g] = static_cast<T>(w.nx);
       h.wall_ny[g] = static_cast<T>(w.ny);
       h.wall_len[g] = static_cast<T>(w.length);
@@ -413,6 +488,7 @@ class DeviceArena final : public Arena {
     s.n_walls = nw;
     s.nv = d.nv;
     sa_nu_inf_[m] = v.transport.sa_nu_inf;
+    cell_pos_[m] = order.cell_pos;
     return "";
   }
 
@@ -522,7 +598,7 @@ class DeviceArena final : public Arena {
     const SlabPlan plan = plan_slab<T>(totals_, nv);
     if (geo != plan.geometry) return "AMR arena size bookkeeping";
     const int nc = std::max(nc_, 1), nf = std::max(nf_, 1), nw = std::max(nw_, 1);
-    const size_t cplane = plan.cplane, dplane = plan.dplane, crec = plan.crec;
+    const size_t cplane = plan.cplane, dplane = plan.dplane, vec4 = plan.vec4, vec8 = plan.vec8;
     const size_t mon_cell = plan.mon_cell, mon_wall = plan.mon_wall;
     slab_bytes_ = plan.total;
     geometry_bytes_ = geo;
@@ -596,8 +672,11 @@ class DeviceArena final : public Arena {
     a.U[1] = dplanes(nv);
     a.U2 = dplanes(nv);
     a.QHAT = dplanes(4);
-    a.CREC[0] = plane(crec, 1);
-    a.CREC[1] = plane(crec, 1);
+    a.VREC[0] = plane(vec4, 1);
+    a.VREC[1] = plane(vec4, 1);
+    a.PREC[0] = plane(vec4, 1);
+    a.PREC[1] = plane(vec4, 1);
+    a.GREC = plane(vec8, 1);
     a.R0 = plane(cplane, nv);
     a.SSRC = plane(cplane, 1);
     a.SRATE = plane(cplane, 1);
@@ -729,25 +808,34 @@ class DeviceArena final : public Arena {
     const MeshSlot& sl = slots_[m];
     const size_t count = static_cast<size_t>(sl.nv) * sl.n_cells;
     if (sl.n_cells == 0) return hipSuccess;
+    const std::vector<int>& pos = cell_pos_[m];
+    const size_t n = static_cast<size_t>(sl.n_cells);
     if (buffer == 2) {  // residual R0 (T planes)
       hipError_t e = hipMemcpy2DAsync(pinned_, sl.n_cells * sizeof(T), ptrs_.R0 + sl.cell_begin, nc_ * sizeof(T),
                                       sl.n_cells * sizeof(T), sl.nv, hipMemcpyDeviceToHost, s);
       if (e != hipSuccess || (e = hipStreamSynchronize(s)) != hipSuccess) return e;
       const T* p = static_cast<const T*>(pinned_);
-      for (size_t i = 0; i < count; ++i) out[i] = static_cast<double>(p[i]);
+      for (size_t v = 0; v < count / n; ++v)
+        for (size_t k = 0; k < n; ++k) out[v * n + k] = static_cast<double>(p[v * n + pos[k]]);
       return hipSuccess;
     }
     hipError_t e = hipMemcpy2DAsync(pinned_, sl.n_cells * sizeof(double), ptrs_.U[buffer] + sl.cell_begin,
                                     nc_ * sizeof(double), sl.n_cells * sizeof(double), sl.nv, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess || (e = hipStreamSynchronize(s)) != hipSuccess) return e;
-    std::memcpy(out, pinned_, count * sizeof(double));
+    const double* p = static_cast<const double*>(pinned_);
+    for (size_t v = 0; v < count / n; ++v)
+      for (size_t k = 0; k < n; ++k) out[v * n + k] = p[v * n + pos[k]];
     return hipSuccess;
   }
 
   hipError_t write_state(int m, int buffer, const double* in, hipStream_t s) override {
     const MeshSlot& sl = slots_[m];
     if (sl.n_cells == 0) return hipSuccess;
-    std::memcpy(pinned_, in, static_cast<size_t>(sl.nv) * sl.n_cells * sizeof(double));
+    const std::vector<int>& pos = cell_pos_[m];
+    const size_t n = static_cast<size_t>(sl.n_cells);
+    double* p = static_cast<double*>(pinned_);
+    for (size_t v = 0; v < static_cast<size_t>(sl.nv); ++v)
+      for (size_t k = 0; k < n; ++k) p[v * n + pos[k]] = in[v * n + k];
     hipError_t e = hipMemcpy2DAsync(ptrs_.U[buffer] + sl.cell_begin, nc_ * sizeof(double), pinned_,
                                     sl.n_cells * sizeof(double), sl.n_cells * sizeof(double), sl.nv,
                                     hipMemcpyHostToDevice, s);
@@ -816,9 +904,11 @@ class DeviceArena final : public Arena {
     h.desc[m] = d;
     s.n_cells = s.n_faces = s.n_walls = 0;
     sa_nu_inf_[m] = 0.0;
+    cell_pos_[m].clear();
   }
 
   HostArrays<T> h_;
+  std::vector<std::vector<int>> cell_pos_;  // per slot: storage position of each mesh cell
   std::vector<Column> columns_;
   ArenaPtrs<T> ptrs_{};
   std::vector<MeshSlot> slots_;

@@ -11,10 +11,11 @@
 //            block), stage passes apply the SSP-RK3 combination in FP64 (reference :188-194)
 //            and write the candidate's primitives, viscosity and eddy viscosity.
 // A face shared by two cells is evaluated by both with the same inputs and operations, so both
-// see the same flux bits; no flux array is written or re-read.  Cell data that faces read lives
-// in 16-wide records (AoS) so that one neighbour costs whole 16-byte vector loads instead of one
-// cache line per variable.  Records are double-buffered by stage: a stage reads one buffer and
-// writes the next state's primitives into the other.
+// see the same flux bits; no flux array is written or re-read.  Cell data that faces and
+// gradient stencils gather lives in 4- and 8-wide vectors per cell (AoS per purpose), so one
+// neighbour costs whole 16-byte loads instead of one cache line per variable.  Primitive vectors
+// are double-buffered by stage: a stage reads one buffer and writes the next state's into the
+// other.
 // Plus k_base_prim, k_stage0 (dt choice + first stage, reference :170-191), the SRD pair
 // k_group / k_redist (reference :195-207), the SRD-grouped viscous limit (reference :164-176)
 // and k_commit (accept / reject / begin, reference :54-59,212-216).
@@ -28,7 +29,6 @@ namespace hipamr {
 namespace {
 
 constexpr int kMaxTile = 256;
-constexpr int kRec = 16;  // record width (T elements per cell)
 constexpr unsigned long long kInfBits = 0x7ff0000000000000ull;
 
 enum : int { PASS_BASE = 0, PASS_STAGE = 1 };
@@ -43,9 +43,17 @@ __device__ __forceinline__ bool pass_on(const Control* c, int pass) {
 enum : int { BAD_BASE = 1, BAD_STAGE = 2 };
 __device__ __forceinline__ void flag_bad(Control* c, int bit) { atomicOr(&c->bad, bit); }
 
-// Record buffer read by a pass: the base pass and stage 2 read CREC[wbuf], stage 1 reads the
+// Primitive buffer read by a pass: the base pass and stage 2 read buffer wbuf, stage 1 reads the
 // other buffer (written by stage 0); each stage writes the buffer it does not read.
 __device__ __forceinline__ int read_buffer(const Control* c, int stage) { return stage == 1 ? c->wbuf ^ 1 : c->wbuf; }
+
+// Cell tile of this block.  The dispatcher hands consecutive blocks to the 8 XCDs in turn, each
+// with its own L2; this permutation gives every XCD one contiguous run of tiles, so the
+// neighbours a tile gathers mostly sit in the same L2.  A bijection on [0, gridDim.x).
+__device__ __forceinline__ int cell_tile() {
+  const int n = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x), per = n >> 3;
+  return b >= (per << 3) ? b : (b & 7) * per + (b >> 3);
+}
 
 // ---- cell records ---------------------------------------------------------------------------
 template <class T>
@@ -57,43 +65,59 @@ struct CellGrad {
   T ux, uy, vx, vy, sx, sy, sdiff;
 };
 
-__device__ __forceinline__ CellPrim<float> load_prim(const float* r) {
-  const float4 a = *reinterpret_cast<const float4*>(r), b = *reinterpret_cast<const float4*>(r + 4);
-  return CellPrim<float>{a.x, a.y, a.z, a.w, b.x, b.y, b.z};
+template <class T>
+struct CellVel {
+  T rho, u, v, phi;
+};
+
+// 16-byte vector loads and stores of 4 T (float4, or two double2).
+__device__ __forceinline__ void load4(const float* p, float& a, float& b, float& c, float& d) {
+  const float4 v = *reinterpret_cast<const float4*>(p);
+  a = v.x, b = v.y, c = v.z, d = v.w;
 }
-__device__ __forceinline__ CellPrim<double> load_prim(const double* r) {
-  const double2 a = *reinterpret_cast<const double2*>(r), b = *reinterpret_cast<const double2*>(r + 2);
-  const double2 c = *reinterpret_cast<const double2*>(r + 4), d = *reinterpret_cast<const double2*>(r + 6);
-  return CellPrim<double>{a.x, a.y, b.x, b.y, c.x, c.y, d.x};
+__device__ __forceinline__ void load4(const double* p, double& a, double& b, double& c, double& d) {
+  const double2 v = *reinterpret_cast<const double2*>(p), w = *reinterpret_cast<const double2*>(p + 2);
+  a = v.x, b = v.y, c = w.x, d = w.y;
 }
-__device__ __forceinline__ CellGrad<float> load_grad(const float* r) {
-  const float4 a = *reinterpret_cast<const float4*>(r + 8), b = *reinterpret_cast<const float4*>(r + 12);
-  return CellGrad<float>{a.x, a.y, a.z, a.w, b.x, b.y, b.z};
+__device__ __forceinline__ void store4(float* p, float a, float b, float c, float d) {
+  *reinterpret_cast<float4*>(p) = make_float4(a, b, c, d);
 }
-__device__ __forceinline__ CellGrad<double> load_grad(const double* r) {
-  const double2 a = *reinterpret_cast<const double2*>(r + 8), b = *reinterpret_cast<const double2*>(r + 10);
-  const double2 c = *reinterpret_cast<const double2*>(r + 12), d = *reinterpret_cast<const double2*>(r + 14);
-  return CellGrad<double>{a.x, a.y, b.x, b.y, c.x, c.y, d.x};
+__device__ __forceinline__ void store4(double* p, double a, double b, double c, double d) {
+  *reinterpret_cast<double2*>(p) = make_double2(a, b);
+  *reinterpret_cast<double2*>(p + 2) = make_double2(c, d);
 }
-__device__ __forceinline__ void store_prim(float* r, const CellPrim<float>& w) {
-  *reinterpret_cast<float4*>(r) = make_float4(w.rho, w.u, w.v, w.p);
-  *reinterpret_cast<float4*>(r + 4) = make_float4(w.phi, w.mu, w.eddy, 0.f);
+
+template <class T>
+__device__ __forceinline__ CellVel<T> load_vel(const ArenaPtrs<T>& a, int buf, int k) {
+  CellVel<T> v;
+  load4(a.VREC[buf] + 4 * k, v.rho, v.u, v.v, v.phi);
+  return v;
 }
-__device__ __forceinline__ void store_prim(double* r, const CellPrim<double>& w) {
-  *reinterpret_cast<double2*>(r) = make_double2(w.rho, w.u);
-  *reinterpret_cast<double2*>(r + 2) = make_double2(w.v, w.p);
-  *reinterpret_cast<double2*>(r + 4) = make_double2(w.phi, w.mu);
-  *reinterpret_cast<double2*>(r + 6) = make_double2(w.eddy, 0.0);
+template <class T>
+__device__ __forceinline__ CellPrim<T> load_prim(const ArenaPtrs<T>& a, int buf, int k) {
+  CellPrim<T> w;
+  T pad;
+  load4(a.VREC[buf] + 4 * k, w.rho, w.u, w.v, w.phi);
+  load4(a.PREC[buf] + 4 * k, w.p, w.mu, w.eddy, pad);
+  return w;
 }
-__device__ __forceinline__ void store_grad(float* r, const CellGrad<float>& g) {
-  *reinterpret_cast<float4*>(r + 8) = make_float4(g.ux, g.uy, g.vx, g.vy);
-  *reinterpret_cast<float4*>(r + 12) = make_float4(g.sx, g.sy, g.sdiff, 0.f);
+template <class T>
+__device__ __forceinline__ CellGrad<T> load_grad(const ArenaPtrs<T>& a, int k) {
+  CellGrad<T> g;
+  T pad;
+  load4(a.GREC + 8 * k, g.ux, g.uy, g.vx, g.vy);
+  load4(a.GREC + 8 * k + 4, g.sx, g.sy, g.sdiff, pad);
+  return g;
 }
-__device__ __forceinline__ void store_grad(double* r, const CellGrad<double>& g) {
-  *reinterpret_cast<double2*>(r + 8) = make_double2(g.ux, g.uy);
-  *reinterpret_cast<double2*>(r + 10) = make_double2(g.vx, g.vy);
-  *reinterpret_cast<double2*>(r + 12) = make_double2(g.sx, g.sy);
-  *reinterpret_cast<double2*>(r + 14) = make_double2(g.sdiff, 0.0);
+template <class T>
+__device__ __forceinline__ void store_prim(const ArenaPtrs<T>& a, int buf, int k, const CellPrim<T>& w) {
+  store4(a.VREC[buf] + 4 * k, w.rho, w.u, w.v, w.phi);
+  store4(a.PREC[buf] + 4 * k, w.p, w.mu, w.eddy, T(0));
+}
+template <class T>
+__device__ __forceinline__ void store_grad(const ArenaPtrs<T>& a, int k, const CellGrad<T>& g) {
+  store4(a.GREC + 8 * k, g.ux, g.uy, g.vx, g.vy);
+  store4(a.GREC + 8 * k + 4, g.sx, g.sy, g.sdiff, T(0));
 }
 
 // Face geometry record: length, dx, dy, ex, ey, distance.
@@ -166,11 +190,12 @@ __device__ __forceinline__ void publish_dt(Control* c, double local) {
 // ------------------------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_base_prim(ArenaPtrs<T> a) {
-  const int m = a.cell_block_mesh[blockIdx.x];
+  const int tile = cell_tile();
+  const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
   if (!pass_on(c, PASS_BASE) || c->w_valid) return;
   const MeshDesc<T>& d = a.desc[m];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
   if (k >= d.cell_begin + d.n_cells) return;
   CellPrim<T> w;
   if (!make_prim(a.U[c->cur], a.nc, k, d, w)) {
@@ -178,7 +203,7 @@ __global__ __launch_bounds__(kMaxTile) void k_base_prim(ArenaPtrs<T> a) {
     atomicExch(&c->fatal, 2);  // the accepted state itself is not admissible
     return;
   }
-  store_prim(a.CREC[c->wbuf] + kRec * k, w);
+  store_prim(a, c->wbuf, k, w);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -186,16 +211,17 @@ __global__ __launch_bounds__(kMaxTile) void k_base_prim(ArenaPtrs<T> a) {
 // half.  STAGE 0 = base pass, 1 and 2 = RK stages.
 template <class T, int STAGE>
 __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
-  const int m = a.cell_block_mesh[blockIdx.x];
+  const int tile = cell_tile();
+  const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
   if (!pass_on(c, STAGE == 0 ? PASS_BASE : PASS_STAGE)) return;
   const MeshDesc<T>& d = a.desc[m];
   if (!d.laminar) return;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
   if (k >= d.cell_begin + d.n_cells) return;
-  T* rec = a.CREC[read_buffer(c, STAGE)];
+  const int rb = read_buffer(c, STAGE);
   const bool sa = d.sa != 0;
-  const CellPrim<T> own = load_prim(rec + kRec * k);
+  const CellPrim<T> own = load_prim(a, rb, k);
   const T rho = own.rho, u = own.u, v = own.v, phi = own.phi;
   T g0 = 0, g1 = 0, g2 = 0, g3 = 0, sx = 0, sy = 0, rx = 0, ry = 0;
   const int end = a.term_off[k + 1];
@@ -204,7 +230,7 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
     const T wx = a.term_wx[j], wy = a.term_wy[j];
     T uu = T(0), vv = T(0), ph = T(0), rr = rho;
     if (src >= 0) {
-      const CellPrim<T> nb = load_prim(rec + kRec * src);
+      const CellVel<T> nb = load_vel(a, rb, src);
       uu = nb.u;
       vv = nb.v;
       if (sa) {
@@ -239,7 +265,7 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
     if (!(pt.diffusion > T(0)) || !isfinite(source) || !isfinite(pt.source_rate))
       flag_bad(c, STAGE == 0 ? BAD_BASE : BAD_STAGE);
   }
-  store_grad(rec + kRec * k, g);
+  store_grad(a, k, g);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -425,22 +451,22 @@ __device__ __forceinline__ void wall_shear(const CellPrim<T>& donor, T nx, T ny,
 template <class T, int STAGE>
 __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
   constexpr bool BASE = STAGE == 0;
-  const int m = a.cell_block_mesh[blockIdx.x];
+  const int tile = cell_tile();
+  const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
   if (!pass_on(c, BASE ? PASS_BASE : PASS_STAGE)) return;
   const MeshDesc<T>& d = a.desc[m];
   const int nc = a.nc;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
   const bool valid = k < d.cell_begin + d.n_cells;
   const bool sa = d.sa != 0, laminar = d.laminar != 0;
   const int rb = read_buffer(c, STAGE);
   const int bit = BASE ? BAD_BASE : BAD_STAGE;
   double local_dt = INFINITY;
   if (valid) {
-    const T* rec = a.CREC[rb];
-    const CellPrim<T> own = load_prim(rec + kRec * k);
+    const CellPrim<T> own = load_prim(a, rb, k);
     CellGrad<T> owng{};
-    if (laminar) owng = load_grad(rec + kRec * k);
+    if (laminar) owng = load_grad(a, k);
     T s0 = 0, s1 = 0, s2 = 0, s3 = 0, spectral = 0, diffusion = 0, sa_net = 0, sa_spec = 0;
     bool ok = true;
     const int end = a.csr_off[k + 1];
@@ -454,8 +480,8 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
       CellPrim<T> nb = own;
       CellGrad<T> nbg = owng;
       if (other >= 0) {
-        nb = load_prim(rec + kRec * other);
-        if (laminar) nbg = load_grad(rec + kRec * other);
+        nb = load_prim(a, rb, other);
+        if (laminar) nbg = load_grad(a, other);
       }
       const FaceFlux<T> f = face_flux<T, BASE>(d, fi, geo, own, owng, pick(self_left, own, nb), pick(self_left, nb, own),
                                                pick(self_left, owng, nbg), pick(self_left, nbg, owng));
@@ -483,7 +509,7 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
         if (BASE) spectral += snd * len;
         if (laminar) {
           const int dn = a.wall_donor[j];
-          const CellPrim<T> donor = dn == k ? own : load_prim(rec + kRec * dn);
+          const CellPrim<T> donor = dn == k ? own : load_prim(a, rb, dn);
           const T dd = a.wall_ddist[j];
           T fv[4];
           wall_shear(donor, a.wall_nx[j], a.wall_ny[j], dd, fv);
@@ -546,7 +572,7 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
       if (!d.small) {
         CellPrim<T> w;
         if (!make_prim(out, nc, k, d, w)) flag_bad(c, BAD_STAGE);
-        else store_prim(a.CREC[rb ^ 1] + kRec * k, w);
+        else store_prim(a, rb ^ 1, k, w);
       }
     }
   }
@@ -557,12 +583,13 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
 // (reference diffusion_group_kernel and choose_dt SRD branch, src/static_amr2d_cuda.cu:164-176).
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_diffusion_group(ArenaPtrs<T> a) {
-  const int m = a.cell_block_mesh[blockIdx.x];
+  const int tile = cell_tile();
+  const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
   if (!pass_on(c, PASS_BASE)) return;
   const MeshDesc<T>& d = a.desc[m];
   if (d.visc_mode != VISC_SRD_GROUP) return;
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
   if (g >= d.cell_begin + d.n_cells) return;
   T sum = 0;
   for (int j = a.grp_off[g]; j < a.grp_off[g + 1]; ++j) sum += a.grp_w[j] * a.DRATE[a.grp_mem[j]];
@@ -571,12 +598,13 @@ __global__ __launch_bounds__(kMaxTile) void k_diffusion_group(ArenaPtrs<T> a) {
 
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_dt_group(ArenaPtrs<T> a) {
-  const int m = a.cell_block_mesh[blockIdx.x];
+  const int tile = cell_tile();
+  const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
   if (!pass_on(c, PASS_BASE)) return;
   const MeshDesc<T>& d = a.desc[m];
   if (d.visc_mode != VISC_SRD_GROUP) return;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
   double local = INFINITY;
   if (k < d.cell_begin + d.n_cells) {
     T sum = 0;
@@ -603,11 +631,12 @@ __device__ __forceinline__ bool choose_dt(const Control* c, double fixed, double
 
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
-  const int m = a.cell_block_mesh[blockIdx.x];
+  const int tile = cell_tile();
+  const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
   if (!running(c)) return;
   const MeshDesc<T>& d = a.desc[m];
-  const bool leader = blockIdx.x == static_cast<unsigned>(d.first_cell_block) && threadIdx.x == 0;
+  const bool leader = tile == d.first_cell_block && threadIdx.x == 0;
   if (c->bad & BAD_BASE) {  // the base residual failed: nonphysical base state
     if (leader) c->fatal = 2;
     return;
@@ -621,7 +650,7 @@ __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
     c->trial_dt = dt;
     if (!c->retrying) c->attempt = 0;
   }
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
   if (k >= d.cell_begin + d.n_cells) return;
   const int nc = a.nc, cur = c->cur;
   const double* u = a.U[cur];
@@ -630,7 +659,7 @@ __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
   if (!d.small) {
     CellPrim<T> w;
     if (!make_prim(out, nc, k, d, w)) flag_bad(c, BAD_STAGE);
-    else store_prim(a.CREC[c->wbuf ^ 1] + kRec * k, w);
+    else store_prim(a, c->wbuf ^ 1, k, w);
   }
 }
 
@@ -638,12 +667,13 @@ __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
 // `stage` selects the candidate: 0 and 2 write U[cur ^ 1], 1 writes U2.
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_group(ArenaPtrs<T> a, int stage) {
-  const int m = a.cell_block_mesh[blockIdx.x];
+  const int tile = cell_tile();
+  const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
   if (!healthy(c)) return;
   const MeshDesc<T>& d = a.desc[m];
   if (!d.small) return;
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
   if (g >= d.cell_begin + d.n_cells) return;
   const int nc = a.nc;
   const double* u = stage == 1 ? a.U2 : a.U[c->cur ^ 1];
@@ -664,12 +694,13 @@ __global__ __launch_bounds__(kMaxTile) void k_group(ArenaPtrs<T> a, int stage) {
 
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_redist(ArenaPtrs<T> a, int stage) {
-  const int m = a.cell_block_mesh[blockIdx.x];
+  const int tile = cell_tile();
+  const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
   if (!healthy(c)) return;
   const MeshDesc<T>& d = a.desc[m];
   if (!d.small) return;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
   if (k >= d.cell_begin + d.n_cells) return;
   const int nc = a.nc;
   double* u = stage == 1 ? a.U2 : a.U[c->cur ^ 1];
@@ -688,7 +719,7 @@ __global__ __launch_bounds__(kMaxTile) void k_redist(ArenaPtrs<T> a, int stage) 
   u[3 * nc + k] = s3 * inv;
   CellPrim<T> w;
   if (!make_prim(u, nc, k, d, w)) flag_bad(c, BAD_STAGE);
-  else store_prim(a.CREC[read_buffer(c, stage) ^ 1] + kRec * k, w);
+  else store_prim(a, read_buffer(c, stage) ^ 1, k, w);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -710,7 +741,7 @@ __device__ __forceinline__ void begin_trial(Control* c) {
 }
 
 __global__ void k_commit(Control* ctrl, int n) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   if (m >= n) return;
   Control* c = ctrl + m;
   if (c->active) {
@@ -743,7 +774,7 @@ __global__ void k_commit(Control* ctrl, int n) {
 
 __global__ void k_target(Control* ctrl, int n, double target, long long maxsteps, const double* targets,
                          const long long* maxsteps_each) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   if (m >= n) return;
   Control* c = ctrl + m;
   c->target = targets ? targets[m] : target;
@@ -774,9 +805,10 @@ __device__ __forceinline__ void block_sum(double (&v)[N]) {
 // reference src/quick_airfoil.cpp:76-81).
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_monitor_cells(ArenaPtrs<T> a) {
-  const int m = a.cell_block_mesh[blockIdx.x];
+  const int tile = cell_tile();
+  const int m = a.cell_block_mesh[tile];
   const MeshDesc<T>& d = a.desc[m];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
   double acc[6] = {0, 0, 0, 0, 0, 0};
   if (k < d.cell_begin + d.n_cells) {
     const double* f = a.mon_frame + 6 * m;
@@ -791,7 +823,7 @@ __global__ __launch_bounds__(kMaxTile) void k_monitor_cells(ArenaPtrs<T> a) {
   }
   block_sum(acc);
   if (threadIdx.x == 0)
-    for (int i = 0; i < 6; ++i) a.MON_CELL[6 * blockIdx.x + i] = acc[i];
+    for (int i = 0; i < 6; ++i) a.MON_CELL[6 * tile + i] = acc[i];
 }
 
 // Steady monitor, walls: pressure force minus viscous shear and the moment about (mx, my)
@@ -804,14 +836,14 @@ __global__ __launch_bounds__(kMaxTile) void k_monitor_walls(ArenaPtrs<T> a) {
   const int j = b * blockDim.x + threadIdx.x;
   double acc[3] = {0, 0, 0};
   if (j < d.wall_begin + d.n_walls) {
-    const T* rec = a.CREC[a.ctrl[m].wbuf];
+    const int wb = a.ctrl[m].wbuf;
     const double mx = a.mon_frame[6 * m + 4], my = a.mon_frame[6 * m + 5];
-    const CellPrim<T> own = load_prim(rec + kRec * a.wall_owner[j]);
+    const CellPrim<T> own = load_prim(a, wb, a.wall_owner[j]);
     const T len_t = a.wall_len[j];
     const double p = static_cast<double>(own.p), len = static_cast<double>(len_t);
     double fx = p * static_cast<double>(a.wall_nx[j]) * len, fy = p * static_cast<double>(a.wall_ny[j]) * len;
     if (d.laminar) {
-      const CellPrim<T> donor = load_prim(rec + kRec * a.wall_donor[j]);
+      const CellPrim<T> donor = load_prim(a, wb, a.wall_donor[j]);
       T fv[4];
       wall_shear(donor, a.wall_nx[j], a.wall_ny[j], a.wall_ddist[j], fv);
       fx -= static_cast<double>(fv[1] * len_t);

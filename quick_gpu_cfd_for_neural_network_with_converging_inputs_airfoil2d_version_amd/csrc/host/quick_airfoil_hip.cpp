@@ -9,7 +9,8 @@
 // check.  Outputs: PREFIX.history.csv, PREFIX.result.json, PREFIX.gpu.json, the mesh arrays
 // (PREFIX.cells/faces/walls.npy, PREFIX.mesh.json) and the initial / final snapshots
 // (PREFIX.initial.state.npy + .json, PREFIX.final.state.npy + .json; PREFIX.failed_last.* after a
-// failure), in the reference's column orders.
+// failure) with their viscous parts (wall_stress.npy, viscous_loads.json, sa.npy), in the
+// reference's column orders.
 // Accepted backends: "hip" (this executable is the GPU backend; the reference's own CPU build
 // remains the "cpu" backend).  Extra optional keys: model (laminar | euler; the reference
 // executable is always laminar) and precision (64 | 32).
@@ -101,8 +102,11 @@ bool write_mesh_files(const StaticAmrMesh2D& m, const Params2D& p, const std::st
   return static_cast<bool>(out);
 }
 
+bool write_viscous_snapshot(const StaticAmrMesh2D& m, const Params2D& p, const StaticAmrState2D& s,
+                            const std::string& prefix);
+
 // State snapshot: PREFIX.state.npy (nv x n) and PREFIX.json with bounds, pressure loads and
-// conserved totals (the reference's snapshot summary fields).
+// conserved totals (the reference's snapshot summary fields), plus the viscous parts.
 bool write_snapshot(const StaticAmrMesh2D& m, const Params2D& p, const StaticAmrState2D& s, const std::string& prefix,
                     std::string* error) {
   const size_t n = m.cells.size(), nv = m.transport.sa_model ? 5 : 4;
@@ -137,6 +141,10 @@ bool write_snapshot(const StaticAmrMesh2D& m, const Params2D& p, const StaticAmr
     *error = "NPY write failed: " + prefix + ".state.npy";
     return false;
   }
+  if (!write_viscous_snapshot(m, p, s, prefix)) {
+    *error = "viscous snapshot write failed: " + prefix;
+    return false;
+  }
   std::ofstream out(prefix + ".json");
   out << std::setprecision(17) << "{\"backend\":\"hip\",\"time\":" << s.time << ",\"steps\":" << s.steps
       << ",\"rejected\":" << s.rejected << ",\"rho_min\":" << rhomin << ",\"rho_max\":" << rhomax << ",\"p_min\":" << pmin
@@ -148,6 +156,97 @@ bool write_snapshot(const StaticAmrMesh2D& m, const Params2D& p, const StaticAmr
   if (!out) {
     *error = "snapshot JSON write failed";
     return false;
+  }
+  return true;
+}
+
+// Molecular viscosity at a temperature (constant or Sutherland), as the device path evaluates it.
+double viscosity_of(const AmrTransport2D& t, double temperature) {
+  if (t.law == AMR_MU_CONSTANT) return t.mu_ref;
+  const double ratio = temperature / t.temperature_ref;
+  return t.mu_ref * ratio * std::sqrt(ratio) * (t.temperature_ref + t.sutherland) / (temperature + t.sutherland);
+}
+
+// Wall stress and body loads of a viscous snapshot, in FP64 from the snapshot state:
+// PREFIX.wall_stress.npy (14 columns per wall fragment), PREFIX.viscous_loads.json (pressure and
+// viscous force and moment about the origin, in total and per patch) and, with SA-neg,
+// PREFIX.sa.npy (nutilde, eddy viscosity, wall distance per cell) - the files the pipeline's
+// surface and load exports read.  The no-slip shear of a fragment comes from its donor cell:
+// tau n with the velocity gradient (0 - u_donor) n / d (the shear the time step applies).
+bool write_viscous_snapshot(const StaticAmrMesh2D& m, const Params2D& p, const StaticAmrState2D& s,
+                            const std::string& prefix) {
+  if (m.transport.model == AMR_MODEL_EULER) return true;
+  const size_t n = m.cells.size(), nw = m.walls.size();
+  const AmrTransport2D& t = m.transport;
+  std::vector<double> rho(n), vx(n), vy(n), pr(n), mu(n);
+  for (size_t k = 0; k < n; ++k) {
+    rho[k] = s.u[k];
+    vx[k] = s.u[n + k] / rho[k];
+    vy[k] = s.u[2 * n + k] / rho[k];
+    pr[k] = (p.gamma - 1) * (s.u[3 * n + k] - .5 * rho[k] * (vx[k] * vx[k] + vy[k] * vy[k]));
+    mu[k] = viscosity_of(t, pr[k] / (rho[k] * t.gas_constant));
+  }
+  std::vector<double> rows;
+  rows.reserve(14 * nw);
+  std::map<int, std::array<double, 6>> patches;
+  std::array<double, 6> total{};
+  for (size_t q = 0; q < nw; ++q) {
+    const WallFragment2D& w = m.walls[q];
+    const AmrWallSample2D& sample = m.viscous.wall_samples[q];
+    const size_t k = w.owner_cell, dn = static_cast<size_t>(sample.donor);
+    const double dd = sample.distance;
+    const double du = -vx[dn] / dd, dv = -vy[dn] / dd;
+    const double g0 = w.nx * du, g1 = w.ny * du, g2 = w.nx * dv, g3 = w.ny * dv;
+    const double dil = 2.0 / 3.0 * (g0 + g3);
+    const double txx = mu[dn] * (2 * g0 - dil), txy = mu[dn] * (g1 + g2), tyy = mu[dn] * (2 * g3 - dil);
+    const double flux_x = (txx * w.nx + txy * w.ny) * w.length, flux_y = (txy * w.nx + tyy * w.ny) * w.length;
+    const double x = .5 * (w.x0 + w.x1), y = .5 * (w.y0 + w.y1);
+    const double px = pr[k] * w.nx * w.length, py = pr[k] * w.ny * w.length, fvx = -flux_x, fvy = -flux_y;
+    const std::array<double, 6> loads = {px, py, x * py - y * px, fvx, fvy, x * fvy - y * fvx};
+    std::array<double, 6>& patch = patches[w.patch_id];
+    for (int v = 0; v < 6; ++v) {
+      patch[v] += loads[v];
+      total[v] += loads[v];
+    }
+    for (double value : {double(q), double(k), double(dn), dd, mu[dn], flux_x / w.length, flux_y / w.length,
+                         (-w.ny * flux_x + w.nx * flux_y) / w.length, pr[k], px, py, fvx, fvy, loads[5]})
+      rows.push_back(value);
+  }
+  if (!write_npy(prefix + ".wall_stress.npy", rows.data(), nw, 14)) return false;
+  std::ofstream f(prefix + ".viscous_loads.json");
+  f << std::setprecision(17) << "{\"time\":" << s.time << ",\"model\":\""
+    << (t.sa_model ? "sa-neg-mechanical" : "laminar")
+    << "\",\"conductivity\":0,\"load_columns\":[\"fx_pressure\",\"fy_pressure\",\"mz_pressure\",\"fx_viscous\","
+       "\"fy_viscous\",\"mz_viscous\"],\"total\":[";
+  for (int v = 0; v < 6; ++v) f << (v ? "," : "") << total[v];
+  f << "],\"patches\":[";
+  bool first = true;
+  for (const auto& [id, value] : patches) {
+    f << (first ? "" : ",") << "{\"patch\":" << id << ",\"loads\":[";
+    for (int v = 0; v < 6; ++v) f << (v ? "," : "") << value[v];
+    f << "]}";
+    first = false;
+  }
+  f << "],\"wall_stress_columns\":[\"wall_id\",\"owner\",\"donor\",\"normal_distance\",\"mu\",\"tau_n_x\","
+       "\"tau_n_y\",\"tau_t_signed\",\"p\",\"fx_pressure\",\"fy_pressure\",\"fx_viscous\",\"fy_viscous\","
+       "\"mz_viscous\"],\"moment_reference\":[0,0],\"wall_normal\":\"fluid to solid\",\"wall_energy_flux\":0}\n";
+  if (!f) return false;
+  if (t.sa_model) {
+    std::vector<double> sa;
+    sa.reserve(3 * n);
+    for (size_t k = 0; k < n; ++k) {
+      const double phi = s.u[4 * n + k] / rho[k], nu = mu[k] / rho[k];
+      const double chi = phi / nu, cube = chi * chi * chi;
+      const double eddy = phi > 0 ? rho[k] * phi * cube / (cube + 7.1 * 7.1 * 7.1) : 0.0;
+      sa.push_back(phi);
+      sa.push_back(eddy);
+      sa.push_back(m.viscous.wall_distance[k]);
+    }
+    if (!write_npy(prefix + ".sa.npy", sa.data(), n, 3)) return false;
+    std::ofstream d(prefix + ".sa.json");
+    d << std::setprecision(17) << "{\"model\":\"SA-neg-ft2-Note1c-mechanical\",\"columns\":[\"nutilde\",\"mu_t\","
+      << "\"wall_distance\"],\"nu_inf\":" << t.sa_nu_inf << ",\"wall_nutilde\":0,\"thermal_conductivity\":0}\n";
+    if (!d) return false;
   }
   return true;
 }

@@ -72,22 +72,38 @@ def build_request_mesh(request: Request):
     return mesh, params, mesh.initialize(params, request.case.freestream)
 
 
-class DeviceBackend:
-    """The slot arena and its steady driver (batch.Batch + batch.Driver)."""
+CAPACITY_FIELDS = ("cells", "faces", "walls", "csr", "terms", "groups", "members")
 
-    def __init__(self, cap, features, n_slots: int, *, precision: int = 32, tile: int = 64, batch_trials: int = 32):
+
+class DeviceBackend:
+    """The slot arena and its steady driver (batch.Batch + batch.Driver).
+
+    `caps` is one capacity per slot (slots may differ: e.g. classes sized for LF1, LF2 and
+    level-7 meshes); or one capacity with `n_slots` copies."""
+
+    def __init__(self, caps, features, n_slots: int | None = None, *, precision: int = 32, tile: int = 64,
+                 batch_trials: int = 32):
         from . import batch as B
-        self.batch = B.Batch.with_slots([cap] * n_slots, features, precision=precision, tile=tile,
-                                        batch_trials=batch_trials)
+        caps = list(caps) if isinstance(caps, (list, tuple)) else [caps] * int(n_slots)
+        self.batch = B.Batch.with_slots(caps, features, precision=precision, tile=tile, batch_trials=batch_trials)
         self.driver = B.Driver(self.batch)
-        self.cap = cap
-        self.n_slots = n_slots
+        self.caps = caps
+        self.n_slots = len(caps)
+
+    def need(self, mesh):
+        from . import batch as B
+        return B.capacity(mesh, self.batch.tile)[0]
+
+    def fits_slot(self, mesh, slot: int, need=None) -> bool:
+        need = need if need is not None else self.need(mesh)
+        return all(getattr(need, k) <= getattr(self.caps[slot], k) for k in CAPACITY_FIELDS)
+
+    def slot_size(self, slot: int) -> int:
+        return int(self.caps[slot].cells)
 
     def fits(self, mesh) -> bool:
-        from . import batch as B
-        need, _ = B.capacity(mesh, self.batch.tile)
-        return all(getattr(need, k) <= getattr(self.cap, k)
-                   for k in ("cells", "faces", "walls", "csr", "terms", "groups", "members"))
+        need = self.need(mesh)
+        return any(self.fits_slot(mesh, s, need) for s in range(self.n_slots))
 
     def start(self, slot: int, mesh, params, u0, case, mode: str, stop_time, budget: float):
         self.batch.admit(slot, mesh, params, u0)
@@ -108,7 +124,8 @@ class DeviceBackend:
 
 
 class Server:
-    """Continuous batching over a backend with `n_slots` slots (see DeviceBackend)."""
+    """Continuous batching over a backend with `n_slots` slots (see DeviceBackend): the backend
+    provides fits(mesh), fits_slot(mesh, slot), slot_size(slot), start, running, poll, finish."""
 
     def __init__(self, backend, *, mode: str = "steady", stop_time: float | None = None, budget: float = 300.0,
                  workers: int = 8, poll_seconds: float = 0.05, mesher=build_request_mesh,
@@ -162,8 +179,15 @@ class Server:
                 meshing = still
                 ready.sort(key=lambda item: (item[0].arrival, item[0].rid))  # first come, first served
                 while free and ready:
-                    req, (mesh, params, u0) = ready.pop(0)
-                    slot = free.pop(0)
+                    # First come, first served; the head request takes the smallest free slot
+                    # that holds it, or waits for one.
+                    req, (mesh, params, u0) = ready[0]
+                    fitting = [s for s in free if self.backend.fits_slot(mesh, s)]
+                    if not fitting:
+                        break
+                    slot = min(fitting, key=lambda s: (self.backend.slot_size(s), s))
+                    ready.pop(0)
+                    free.remove(slot)
                     out = outcomes[req.rid]
                     out.slot, out.n_cells = slot, mesh.n_cells
                     out.admitted = self.clock() - t0

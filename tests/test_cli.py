@@ -57,3 +57,12 @@ def test_mesh_mode_writes_reference_artefacts(tmp_path, profile):
     assert mesh["state_shape"] == [5, expected] and mesh["levels"] == 5
     gpu = json.loads(prefix.with_suffix(".gpu.json").read_text())
     assert gpu["allocated_device_bytes"] == 0
+    # Viscous snapshot of the uniform initial state: 14 wall-stress columns per fragment; the
+    # uniform pressure integrates to zero force on the closed body.
+    stress = np.load(tmp_path / profile / "flow.initial.wall_stress.npy")
+    assert stress.shape == (walls.shape[0], 14) and np.array_equal(stress[:, 1], walls[:, 0])
+    loads = json.loads((tmp_path / profile / "flow.initial.viscous_loads.json").read_text())
+    assert abs(loads["total"][0]) < 1e-12 and abs(loads["total"][1]) < 1e-12
+    assert {p["patch"] for p in loads["patches"]} == {1, 2, 3}
+    sa = np.load(tmp_path / profile / "flow.initial.sa.npy")
+    assert sa.shape == (expected, 3) and np.all(sa[:, 2] > 0)

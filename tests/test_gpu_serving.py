@@ -23,7 +23,7 @@ def test_serving_mix_equals_alone_bitwise():
     reqs = [S.Request(i, polys[i], CF.make_case(presets[i], LAMINAR), float(arrivals[i])) for i in range(4)]
     built = [S.build_request_mesh(r) for r in reqs]
     caps, feats = zip(*[B.capacity(m) for m, _, _ in built])
-    backend = S.DeviceBackend(B.envelope(caps), B.union(feats), 2, precision=32)
+    backend = S.DeviceBackend([B.envelope(caps)] * 2, B.union(feats), precision=32)
     try:
         rep = S.Server(backend, mode="time", stop_time=0.03, budget=120, workers=2).serve(reqs)
     finally:

@@ -78,21 +78,30 @@ class FakeMesh:
 
 
 class FakeBackend:
-    """Runs take `duration` seconds of wall time; records concurrency and admission order."""
+    """Runs take `duration` seconds of wall time; records concurrency, admission order and slots."""
 
-    def __init__(self, n_slots, max_cells, duration):
-        self.n_slots, self.max_cells, self.duration = n_slots, max_cells, duration
+    def __init__(self, n_slots, max_cells, duration, slot_cells=None):
+        self.slot_cells = list(slot_cells) if slot_cells else [max_cells] * n_slots
+        self.n_slots, self.duration = len(self.slot_cells), duration
         self.running_runs = {}
         self.peak = 0
         self.started = []
+        self.placed = {}
+
+    def fits_slot(self, mesh, slot):
+        return mesh.n_cells <= self.slot_cells[slot]
+
+    def slot_size(self, slot):
+        return self.slot_cells[slot]
 
     def fits(self, mesh):
-        return mesh.n_cells <= self.max_cells
+        return any(self.fits_slot(mesh, s) for s in range(self.n_slots))
 
     def start(self, slot, mesh, params, u0, case, mode, stop_time, budget):
-        assert slot not in self.running_runs
+        assert slot not in self.running_runs and self.fits_slot(mesh, slot)
         self.running_runs[slot] = time.perf_counter() + self.duration
         self.started.append(case)
+        self.placed[case] = slot
         self.peak = max(self.peak, len(self.running_runs))
 
     def running(self):
@@ -134,3 +143,16 @@ def test_server_fifo_capacity_and_rejection():
     # Three waves of two runs: the last request waits for two runs before it.
     assert rep["p99_latency_seconds"] >= 0.06
     assert 0 < rep["slot_occupancy"] <= 1
+
+
+def test_server_best_fit_slot_classes():
+    # Slots 0-1 hold 100 cells, slot 2 holds 500: small requests take small slots first, a large
+    # request waits (first come, first served) until the large slot is free.
+    sizes = [90, 400, 80, 450, 70]
+    reqs = [S.Request(i, None, i, arrival=0.001 * i) for i in range(len(sizes))]
+    backend = FakeBackend(0, 0, duration=0.02, slot_cells=[100, 100, 500])
+    server = S.Server(backend, workers=2, poll_seconds=0.005, mesher=lambda r: (FakeMesh(sizes[r.rid]), None, None))
+    rep = server.serve(reqs)
+    assert rep["completed"] == 5 and rep["failed"] == 0
+    assert backend.placed[0] in (0, 1) and backend.placed[1] == 2 and backend.placed[3] == 2
+    assert backend.started.index(3) > backend.started.index(1)
