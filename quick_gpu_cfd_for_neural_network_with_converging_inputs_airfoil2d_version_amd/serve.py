@@ -63,11 +63,68 @@ def percentile(values, q: float) -> float:
     return float(v[rank - 1])
 
 
-def build_request_mesh(request: Request):
-    """(mesh, params, u0) of a request: the case's mesh around its airfoil, freestream start."""
+class MeshCache:
+    """Least-recently-used cache of built meshes keyed by (polygon bytes, patches, mesh options,
+    physics and freestream: the side boundaries carry the freestream state).  A repeated request
+    reuses its mesh and viscous tables; only the initial state is rebuilt.  Thread-safe; a cached
+    mesh is only read (admission copies it to the device)."""
+
+    def __init__(self, capacity: int = 64):
+        import threading
+        from collections import OrderedDict
+        self.capacity = capacity
+        self._items = OrderedDict()
+        self._lock = threading.Lock()
+        self.hits = self.misses = 0
+
+    @staticmethod
+    def key(request: Request) -> str:
+        import hashlib
+        import json
+        points, patches = request.polygon
+        c = request.case
+        physics = dict(model=c.model, sa=c.sa_model, law=c.law, mu=c.mu_star, s=c.sutherland_star,
+                       nu=c.sa_nu_inf, vcfl=c.viscous_cfl, sacfl=c.sa_cfl, probe=c.wall_probe_min_h, cfl=c.cfl,
+                       flux=c.flux, gamma=c.gamma, bc=list(c.bc_kinds), outlet=c.outlet_p_star, mach=c.mach,
+                       aoa=c.aoa)
+        h = hashlib.sha256(np.ascontiguousarray(points, dtype=np.float64).tobytes())
+        h.update(np.ascontiguousarray(patches, dtype=np.int64).tobytes())
+        h.update(json.dumps([c.mesh_options, physics], sort_keys=True, default=str).encode())
+        return h.hexdigest()
+
+    def get(self, key):
+        with self._lock:
+            if key in self._items:
+                self._items.move_to_end(key)
+                self.hits += 1
+                return self._items[key]
+            self.misses += 1
+            return None
+
+    def put(self, key, value):
+        with self._lock:
+            self._items[key] = value
+            self._items.move_to_end(key)
+            while len(self._items) > self.capacity:
+                self._items.popitem(last=False)
+
+
+_CACHE = MeshCache()
+
+
+def build_request_mesh(request: Request, cache: MeshCache | None = _CACHE):
+    """(mesh, params, u0) of a request: the case's mesh around its airfoil (from the cache when
+    the same airfoil, mesh and physics came before), freestream start at the request's Mach
+    number and angle of attack."""
     from . import mesh as M
     points, patches = request.polygon
-    mesh = M.build_case_mesh(request.case, points, patches)
+    key = MeshCache.key(request) if cache is not None else None
+    built = cache.get(key) if cache is not None else None
+    if built is None:
+        built = M.build_case_mesh(request.case, points, patches)
+        if cache is not None:
+            cache.put(key, built)
+    mesh = built
     params = M.params_of(request.case)
     return mesh, params, mesh.initialize(params, request.case.freestream)
 

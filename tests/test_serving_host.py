@@ -156,3 +156,19 @@ def test_server_best_fit_slot_classes():
     assert rep["completed"] == 5 and rep["failed"] == 0
     assert backend.placed[0] in (0, 1) and backend.placed[1] == 2 and backend.placed[3] == 2
     assert backend.started.index(3) > backend.started.index(1)
+
+
+def test_mesh_cache_reuses_identical_requests_only():
+    from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import config as CF
+    from .common import naca_polygon
+    cache = S.MeshCache(capacity=2)
+    poly = naca_polygon(0.12)
+    a = S.Request(0, poly, CF.make_case("LF1"))
+    b = S.Request(1, poly, CF.make_case("LF1"))
+    c = S.Request(2, poly, CF.make_case("LF1", {"freestream": {"Mach": 1.3}}))
+    m_a, _, u_a = S.build_request_mesh(a, cache)
+    m_b, _, u_b = S.build_request_mesh(b, cache)
+    m_c, _, u_c = S.build_request_mesh(c, cache)
+    assert m_a is m_b and m_c is not m_a
+    assert cache.hits == 1 and cache.misses == 2
+    assert np.array_equal(u_a, u_b) and not np.array_equal(u_a, u_c)
