@@ -62,7 +62,8 @@ struct ArenaPtrs {
   const T* dt_area;
   const T* wall_dist;
   const int* csr_off;    // nc + 1
-  const int* csr_face;   // face id q (cell is the face's left) or ~q (right)
+  const int4* CSR;       // per cell-face entry: {q (cell is the face's left) or ~q (right), the other
+                         // cell or -1 on a boundary face, face info, 0}
   const int* wall_first; // first wall of the cell
   const int* wall_cnt;
   const int* term_off;   // nc + 1
@@ -76,9 +77,8 @@ struct ArenaPtrs {
   const int* mem_off;    // nc + 1
   const int* mem_grp;
   const T* mem_inv;      // 1 / N_k
-  // faces: FIDX = {left, right, info, 0} (info bit 0 axis, bits 1..3 side + 1, 0 interior),
-  // FGEO = {length, dx, dy, ex, ey, distance, 0, 0}
-  const int4* FIDX;
+  // faces: FGEO = {length, dx, dy, ex, ey, distance, 0, 0}; face info: bit 0 axis, bits 1..3
+  // side + 1 (0 interior)
   const T* FGEO;
   // walls
   const int* wall_owner;

@@ -111,8 +111,8 @@ SlabPlan plan_slab(const Totals& t, int nv) {
   SlabPlan p{};
   p.geometry = arr(sizeof(MeshDesc<T>), t.meshes) + arr(4, t.cell_blocks) + arr(4, t.face_blocks + t.wall_blocks) +
                5 * arr(sizeof(T), t.cells) + 2 * arr(4, t.cells) + 2 * arr(8, t.cells) + 4 * arr(4, t.cells + 1) +
-               arr(4, t.csr) + arr(4, t.terms) + 2 * arr(sizeof(T), t.terms) + arr(4, t.groups) +
-               arr(sizeof(T), t.groups) + arr(4, t.members) + arr(16, t.faces) + arr(8 * sizeof(T), t.faces) +
+               arr(16, t.csr) + arr(4, t.terms) + 2 * arr(sizeof(T), t.terms) + arr(4, t.groups) +
+               arr(sizeof(T), t.groups) + arr(4, t.members) + arr(8 * sizeof(T), t.faces) +
                2 * arr(4, t.walls) + 4 * arr(sizeof(T), t.walls) + 2 * arr(8, t.walls);
   p.control = arr(sizeof(Control), t.meshes) + arr(sizeof(double), t.meshes) + arr(sizeof(long long), t.meshes) +
               arr(6 * sizeof(double), t.meshes);
@@ -204,7 +204,7 @@ struct HostArrays {
   std::vector<T> vol, inv_vol, dt_area, wall_dist, mem_inv;
   std::vector<int> wall_first, wall_cnt, csr_off, term_off, grp_off, mem_off;
   std::vector<double> cell_cx, cell_cy;
-  std::vector<int> csr_face, term_src, grp_mem, mem_grp;
+  std::vector<int> csr_ent, term_src, grp_mem, mem_grp;  // csr_ent: 4 per entry
   std::vector<T> term_wx, term_wy, grp_w;
   std::vector<int> face_idx;  // 4 per face: left, right, info, 0
   std::vector<T> face_geo;    // 8 per face: length, dx, dy, ex, ey, distance, 0, 0
@@ -294,7 +294,7 @@ class DeviceArena final : public Arena {
     h.term_off.assign(nc + 1, 0);
     h.grp_off.assign(nc + 1, 0);
     h.mem_off.assign(nc + 1, 0);
-    sized(h.csr_face, ncsr, 0);
+    sized(h.csr_ent, 4 * static_cast<size_t>(std::max(ncsr, 1)), 0);
     sized(h.term_src, nt, -1);
     sized(h.term_wx, nt, T(0));
     sized(h.term_wy, nt, T(0));
@@ -375,7 +375,14 @@ class DeviceArena final : public Arena {
       for (uint64_t pos = v.face_offsets[k]; pos < v.face_offsets[k + 1]; ++pos) {
         const int q = static_cast<int>(v.cell_faces[pos]);
         const int stored = fb + order.face_pos[q];
-        h.csr_face[csr++] = v.faces[q].left == k ? stored : ~stored;
+        const amr_face& face = v.faces[q];
+        const bool left = face.left == k;
+        const int other = left ? face.right : face.left;
+        int* e = &h.csr_ent[4 * static_cast<size_t>(csr++)];
+        e[0] = left ? stored : ~stored;
+        e[1] = other >= 0 ? cb + pos_of[other] : -1;
+        e[2] = (face.axis & 1) | ((face.boundary + 1) << 1);
+        e[3] = 0;
       }
       if (laminar)
         for (uint64_t pos = v.visc_offsets[k]; pos < v.visc_offsets[k + 1]; ++pos) {
@@ -507,9 +514,12 @@ class DeviceArena final : public Arena {
       if (h.csr_off[k] < s.csr_begin || h.csr_off[k] > h.csr_off[k + 1] || h.csr_off[k + 1] > s.csr_begin + s.cap.csr)
         return "layout: cell-face offsets";
       for (int j = h.csr_off[k]; j < h.csr_off[k + 1]; ++j) {
-        const int sq = h.csr_face[j], q = sq >= 0 ? sq : ~sq;
+        const int* e = &h.csr_ent[4 * static_cast<size_t>(j)];
+        const int sq = e[0], q = sq >= 0 ? sq : ~sq;
         if (q < s.face_begin || q >= s.face_begin + s.n_faces) return "layout: cell-face id";
-        if (h.face_idx[4 * static_cast<size_t>(q) + (sq >= 0 ? 0 : 1)] != k) return "layout: cell-face orientation";
+        const int* fi = &h.face_idx[4 * static_cast<size_t>(q)];
+        if (fi[sq >= 0 ? 0 : 1] != k) return "layout: cell-face orientation";
+        if (e[1] != fi[sq >= 0 ? 1 : 0] || e[2] != fi[2]) return "layout: cell-face entry";
       }
       if (h.wall_cnt[k] < 0 || h.wall_first[k] < s.wall_begin || h.wall_first[k] + h.wall_cnt[k] > s.wall_begin + s.n_walls)
         return "layout: cell walls";
@@ -579,14 +589,13 @@ class DeviceArena final : public Arena {
     for (auto* v : {&h.wall_first, &h.wall_cnt}) col(*v, CELLS);
     for (auto* v : {&h.cell_cx, &h.cell_cy}) col(*v, CELLS);
     for (auto* v : {&h.csr_off, &h.term_off, &h.grp_off, &h.mem_off}) col(*v, OFFSETS);
-    col(h.csr_face, CSR);
+    columns_.push_back({h.csr_ent.data(), nullptr, 4 * sizeof(int), h.csr_ent.size() / 4, CSR});
     col(h.term_src, TERMS);
     col(h.term_wx, TERMS);
     col(h.term_wy, TERMS);
     col(h.grp_mem, GROUPS);
     col(h.grp_w, GROUPS);
     col(h.mem_grp, MEMBERS);
-    columns_.push_back({h.face_idx.data(), nullptr, 4 * sizeof(int), h.face_idx.size() / 4, FACES});
     columns_.push_back({h.face_geo.data(), nullptr, 8 * sizeof(T), h.face_geo.size() / 8, FACES});
     for (auto* v : {&h.wall_owner, &h.wall_donor}) col(*v, WALLS);
     for (auto* v : {&h.wall_nx, &h.wall_ny, &h.wall_len, &h.wall_ddist}) col(*v, WALLS);
@@ -630,14 +639,13 @@ class DeviceArena final : public Arena {
     a.term_off = next(static_cast<const int*>(nullptr));
     a.grp_off = next(static_cast<const int*>(nullptr));
     a.mem_off = next(static_cast<const int*>(nullptr));
-    a.csr_face = next(static_cast<const int*>(nullptr));
+    a.CSR = next(static_cast<const int4*>(nullptr));
     a.term_src = next(static_cast<const int*>(nullptr));
     a.term_wx = next(static_cast<const T*>(nullptr));
     a.term_wy = next(static_cast<const T*>(nullptr));
     a.grp_mem = next(static_cast<const int*>(nullptr));
     a.grp_w = next(static_cast<const T*>(nullptr));
     a.mem_grp = next(static_cast<const int*>(nullptr));
-    a.FIDX = next(static_cast<const int4*>(nullptr));
     a.FGEO = next(static_cast<const T*>(nullptr));
     a.wall_owner = next(static_cast<const int*>(nullptr));
     a.wall_donor = next(static_cast<const int*>(nullptr));

@@ -471,12 +471,12 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
     bool ok = true;
     const int end = a.csr_off[k + 1];
     for (int j = a.csr_off[k]; j < end; ++j) {
-      const int sq = a.csr_face[j];
-      const bool self_left = sq >= 0;
-      const int q = self_left ? sq : ~sq;
-      const int4 fi = a.FIDX[q];
+      const int4 e = a.CSR[j];
+      const bool self_left = e.x >= 0;
+      const int q = self_left ? e.x : ~e.x;
+      const int other = e.y;  // -1 on a boundary face
+      const int4 fi = make_int4(self_left ? k : other, self_left ? other : k, e.z, 0);
       const FaceGeo<T> geo = load_geo(a.FGEO + 8 * q);
-      const int other = self_left ? fi.y : fi.x;  // -1 on a boundary face
       CellPrim<T> nb = own;
       CellGrad<T> nbg = owng;
       if (other >= 0) {

@@ -18,13 +18,13 @@ ROOT = Path(__file__).resolve().parents[1]
 
 def declared_c_symbols():
     text = (ROOT / "include" / "amr_hip.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|void)\s+(amr_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|int32_t|uint64_t)\s+(amr_\w+)\s*\(", text, re.M)))
 
 
 def test_every_declared_symbol_is_exported():
     lib = N.load_library()
     names = declared_c_symbols()
-    assert len(names) >= 15
+    assert len(names) >= 45
     for name in names:
         assert hasattr(lib, name), name
     out = subprocess.run(["nm", "-D", "--defined-only", str(N.LIBRARY_PATH)], capture_output=True, text=True).stdout
@@ -58,3 +58,24 @@ def test_create_rejects_invalid_configuration():
     params.cfl = 0.15
     with pytest.raises(BackendError, match="invalid AMR GPU configuration"):
         Session(mesh, params, u0[:-1])
+
+
+def check_layout(items, tile):
+    lib = N.load_library()
+    views = (N.amr_mesh_view * len(items))(*[m.view for _, m, _ in items])
+    params = (N.amr_params * len(items))(*[p for _, _, p in items])
+    err = N.err_buffer()
+    return lib.amr_hip_check_layout(views, params, len(items), tile, err, N.ERR_LEN), err.value.decode()
+
+
+@pytest.mark.parametrize("tile", [64, 128, 256])
+def test_device_layout_is_consistent(tile):
+    """Host-side audit of the device layout (storage order, slot ranges, CSR entries, gradient
+    terms, SRD tables, wall owners/donors) for a mixed set of meshes at every launch tile."""
+    from .common import EULER, LAMINAR, merge, naca_polygon
+    items = [case_mesh("G0001", "LF1"), case_mesh("G0002", "LF1", LAMINAR),
+             case_mesh(naca_polygon(0.12), "LF1", merge(EULER, {"mesh": {"agglomerate_small": False}}))]
+    status, message = check_layout(items[:1], tile)
+    assert status == 0, message
+    status, message = check_layout(items[1:], tile)
+    assert status == 0, message
