@@ -1,4 +1,6 @@
-// amr_kernels.hip -- the explicit cut-cell SSP-RK3 trial as gfx950 kernels.
+// amr_kernels_impl.h -- the explicit cut-cell SSP-RK3 trial as gfx950 kernels, templated on
+// the work precision; amr_kernels_f64.hip and amr_kernels_f32.hip instantiate it (the FP32 unit
+// is compiled with reciprocal-based division, the FP64 unit with IEEE division).
 //
 // One trial (reference launch_trial, src/static_amr2d_cuda.cu:241-268) is, per RHS pass:
 //   k_grad   cells: LSQ velocity gradient, SA-neg gradients and point model (reference
@@ -21,6 +23,9 @@
 // and k_commit (accept / reject / begin, reference :54-59,212-216).
 // Every kernel serves every mesh of the arena; a block reads its mesh from a block table and
 // skips the whole tile when that mesh is not running (block-uniform branch).
+#ifndef HIPAMR_AMR_KERNELS_IMPL_H
+#define HIPAMR_AMR_KERNELS_IMPL_H
+
 #include <cfloat>
 
 #include "amr_arena.h"
@@ -919,17 +924,6 @@ void launch_monitor(const ArenaPtrs<T>& a, hipStream_t s) {
   if (a.n_wall_blocks > 0) hipLaunchKernelGGL(k_monitor_walls<T>, dim3(a.n_wall_blocks), dim3(a.tile), 0, s, a);
 }
 
-void launch_target(Control* ctrl, int n, double target, long long maxsteps, const double* targets,
-                   const long long* maxsteps_each, hipStream_t s) {
-  hipLaunchKernelGGL(k_target, dim3(blocks_of(n, 256)), dim3(256), 0, s, ctrl, n, target, maxsteps, targets,
-                     maxsteps_each);
-}
-
-template int launch_trial<double>(const ArenaPtrs<double>&, hipStream_t, LaunchHook*);
-template int launch_trial<float>(const ArenaPtrs<float>&, hipStream_t, LaunchHook*);
-template int launch_residual<double>(const ArenaPtrs<double>&, hipStream_t, LaunchHook*);
-template int launch_residual<float>(const ArenaPtrs<float>&, hipStream_t, LaunchHook*);
-template void launch_monitor<double>(const ArenaPtrs<double>&, hipStream_t);
-template void launch_monitor<float>(const ArenaPtrs<float>&, hipStream_t);
-
 }  // namespace hipamr
+
+#endif
