@@ -9,8 +9,10 @@
  *                src/static_amr2d_gpu.h:17-24 (create/advance/download/stats/destroy) plus a
  *                single residual evaluation that mirrors static_amr2d_rhs
  *                (src/static_amr2d.h:80-82).
- *   amr_batch_*  many meshes of uneven size advanced in lockstep in one device arena (no
- *                counterpart in the reference).
+ *   amr_batch_*  many meshes of uneven size advanced in lockstep in one device arena, slots
+ *                admitted and released while others run (no counterpart in the reference).
+ *   amr_driver_* the steady check loop of src/quick_airfoil.cpp:66-95 run over every slot of
+ *                a batch on a host thread (begin / poll / running), for continuous batching.
  *
  * Return codes follow the reference API: 0 ok, 1 error (message copied into err when err is
  * not NULL), 2 wall-clock budget expired with a resumable accepted state.

@@ -125,6 +125,53 @@ __device__ __forceinline__ void store_grad(const ArenaPtrs<T>& a, int k, const C
   store4(a.GREC + 8 * k + 4, g.sx, g.sy, g.sdiff, T(0));
 }
 
+// Cell vectors of the block's own tile staged in LDS: a neighbour inside the tile (most of them,
+// with cells in Hilbert order) is read from LDS instead of being gathered from L2 / HBM again.
+// The arrays live in the launch's dynamic LDS (tile_lds_bytes per block), so a block holds only
+// what its tile size needs and small tiles keep their occupancy.
+extern __shared__ __align__(16) unsigned char tile_lds[];
+
+template <class T>
+constexpr size_t tile_lds_bytes(int tile) {
+  return static_cast<size_t>(tile) * 16 * sizeof(T);
+}
+
+template <class T>
+struct TileCells {
+  T* v;  // 4 per cell: rho, u, v, phi
+  T* p;  // 4 per cell: p, mu, eddy, 0
+  T* g;  // 8 per cell: the gradient record
+  __device__ explicit TileCells(int tile)
+      : v(reinterpret_cast<T*>(tile_lds)), p(v + 4 * tile), g(v + 8 * tile) {}
+};
+
+template <class T>
+__device__ __forceinline__ void stage_cell(const TileCells<T>& t, int i, const CellPrim<T>& w, const CellGrad<T>& g,
+                                           bool grads) {
+  store4(t.v + 4 * i, w.rho, w.u, w.v, w.phi);
+  store4(t.p + 4 * i, w.p, w.mu, w.eddy, T(0));
+  if (grads) {
+    store4(t.g + 8 * i, g.ux, g.uy, g.vx, g.vy);
+    store4(t.g + 8 * i + 4, g.sx, g.sy, g.sdiff, T(0));
+  }
+}
+template <class T>
+__device__ __forceinline__ CellPrim<T> tile_prim(const TileCells<T>& t, int i) {
+  CellPrim<T> w;
+  T pad;
+  load4(t.v + 4 * i, w.rho, w.u, w.v, w.phi);
+  load4(t.p + 4 * i, w.p, w.mu, w.eddy, pad);
+  return w;
+}
+template <class T>
+__device__ __forceinline__ CellGrad<T> tile_grad(const TileCells<T>& t, int i) {
+  CellGrad<T> g;
+  T pad;
+  load4(t.g + 8 * i, g.ux, g.uy, g.vx, g.vy);
+  load4(t.g + 8 * i + 4, g.sx, g.sy, g.sdiff, pad);
+  return g;
+}
+
 // Face geometry record: length, dx, dy, ex, ey, distance.
 template <class T>
 struct FaceGeo {
@@ -222,11 +269,19 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
   if (!pass_on(c, STAGE == 0 ? PASS_BASE : PASS_STAGE)) return;
   const MeshDesc<T>& d = a.desc[m];
   if (!d.laminar) return;
-  const int k = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
-  if (k >= d.cell_begin + d.n_cells) return;
+  const int first = tile * static_cast<int>(blockDim.x);
+  const int k = first + static_cast<int>(threadIdx.x);
+  const int staged = min(static_cast<int>(blockDim.x), d.cell_begin + d.n_cells - first);  // valid cells of the tile
   const int rb = read_buffer(c, STAGE);
   const bool sa = d.sa != 0;
-  const CellPrim<T> own = load_prim(a, rb, k);
+  T* const vel = reinterpret_cast<T*>(tile_lds);  // the tile's own velocity vectors (4 per cell)
+  CellPrim<T> own{};
+  if (k < d.cell_begin + d.n_cells) {
+    own = load_prim(a, rb, k);
+    store4(vel + 4 * threadIdx.x, own.rho, own.u, own.v, own.phi);
+  }
+  __syncthreads();
+  if (k >= d.cell_begin + d.n_cells) return;
   const T rho = own.rho, u = own.u, v = own.v, phi = own.phi;
   T g0 = 0, g1 = 0, g2 = 0, g3 = 0, sx = 0, sy = 0, rx = 0, ry = 0;
   const int end = a.term_off[k + 1];
@@ -235,7 +290,12 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
     const T wx = a.term_wx[j], wy = a.term_wy[j];
     T uu = T(0), vv = T(0), ph = T(0), rr = rho;
     if (src >= 0) {
-      const CellVel<T> nb = load_vel(a, rb, src);
+      CellVel<T> nb;
+      const int local = src - first;
+      if (local >= 0 && local < staged)
+        load4(vel + 4 * local, nb.rho, nb.u, nb.v, nb.phi);
+      else
+        nb = load_vel(a, rb, src);
       uu = nb.u;
       vv = nb.v;
       if (sa) {
@@ -467,11 +527,19 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
   const bool sa = d.sa != 0, laminar = d.laminar != 0;
   const int rb = read_buffer(c, STAGE);
   const int bit = BASE ? BAD_BASE : BAD_STAGE;
+  const int first = tile * static_cast<int>(blockDim.x);
+  const int staged = min(static_cast<int>(blockDim.x), d.cell_begin + d.n_cells - first);  // valid cells of the tile
+  const TileCells<T> lds(static_cast<int>(blockDim.x));
+  CellPrim<T> own{};
+  CellGrad<T> owng{};
+  if (valid) {
+    own = load_prim(a, rb, k);
+    if (laminar) owng = load_grad(a, k);
+    stage_cell(lds, static_cast<int>(threadIdx.x), own, owng, laminar);
+  }
+  __syncthreads();
   double local_dt = INFINITY;
   if (valid) {
-    const CellPrim<T> own = load_prim(a, rb, k);
-    CellGrad<T> owng{};
-    if (laminar) owng = load_grad(a, k);
     T s0 = 0, s1 = 0, s2 = 0, s3 = 0, spectral = 0, diffusion = 0, sa_net = 0, sa_spec = 0;
     bool ok = true;
     const int end = a.csr_off[k + 1];
@@ -485,8 +553,14 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
       CellPrim<T> nb = own;
       CellGrad<T> nbg = owng;
       if (other >= 0) {
-        nb = load_prim(a, rb, other);
-        if (laminar) nbg = load_grad(a, other);
+        const int local = other - first;
+        if (local >= 0 && local < staged) {
+          nb = tile_prim(lds, local);
+          if (laminar) nbg = tile_grad(lds, local);
+        } else {
+          nb = load_prim(a, rb, other);
+          if (laminar) nbg = load_grad(a, other);
+        }
       }
       const FaceFlux<T> f = face_flux<T, BASE>(d, fi, geo, own, owng, pick(self_left, own, nb), pick(self_left, nb, own),
                                                pick(self_left, owng, nbg), pick(self_left, nbg, owng));
@@ -514,7 +588,8 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
         if (BASE) spectral += snd * len;
         if (laminar) {
           const int dn = a.wall_donor[j];
-          const CellPrim<T> donor = dn == k ? own : load_prim(a, rb, dn);
+          const int dl = dn - first;
+          const CellPrim<T> donor = dn == k ? own : (dl >= 0 && dl < staged ? tile_prim(lds, dl) : load_prim(a, rb, dn));
           const T dd = a.wall_ddist[j];
           T fv[4];
           wall_shear(donor, a.wall_nx[j], a.wall_ny[j], dd, fv);
@@ -867,21 +942,23 @@ inline int blocks_of(int n, int tile) { return (n + tile - 1) / tile; }
 
 }  // namespace
 
-#define HIPAMR_LAUNCH(name, kernel, grid, ...)                      \
+#define HIPAMR_LAUNCH_LDS(name, kernel, grid, lds, ...)             \
   do {                                                               \
     if (hook) hook->before(name);                                    \
-    hipLaunchKernelGGL(kernel, grid, tile, 0, s, __VA_ARGS__);       \
+    hipLaunchKernelGGL(kernel, grid, tile, lds, s, __VA_ARGS__);     \
     if (hook) hook->after(name);                                     \
     ++launches;                                                      \
   } while (0)
+#define HIPAMR_LAUNCH(name, kernel, grid, ...) HIPAMR_LAUNCH_LDS(name, kernel, grid, 0, __VA_ARGS__)
 
 template <class T>
 int launch_residual(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
   const dim3 tile(a.tile), cells(a.n_cell_blocks);
+  const size_t cell_lds = tile_lds_bytes<T>(a.tile), vel_lds = cell_lds / 4;
   int launches = 0;
   HIPAMR_LAUNCH("base_primitive", k_base_prim<T>, cells, a);
-  if (a.any_laminar) HIPAMR_LAUNCH("gradient_base", (k_grad<T, 0>), cells, a);
-  HIPAMR_LAUNCH("flux_base", (k_flux<T, 0>), cells, a);
+  if (a.any_laminar) HIPAMR_LAUNCH_LDS("gradient_base", (k_grad<T, 0>), cells, vel_lds, a);
+  HIPAMR_LAUNCH_LDS("flux_base", (k_flux<T, 0>), cells, cell_lds, a);
   if (a.any_visc_group) {
     HIPAMR_LAUNCH("diffusion_group", k_diffusion_group<T>, cells, a);
     HIPAMR_LAUNCH("dt_group", k_dt_group<T>, cells, a);
@@ -892,20 +969,21 @@ int launch_residual(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
 template <class T>
 int launch_trial(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
   const dim3 tile(a.tile), cells(a.n_cell_blocks);
+  const size_t cell_lds = tile_lds_bytes<T>(a.tile), vel_lds = cell_lds / 4;
   int launches = launch_residual(a, s, hook);
   HIPAMR_LAUNCH("stage0", k_stage0<T>, cells, a);
   if (a.any_small) {
     HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 0);
     HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 0);
   }
-  if (a.any_laminar) HIPAMR_LAUNCH("gradient_stage", (k_grad<T, 1>), cells, a);
-  HIPAMR_LAUNCH("flux_rk1", (k_flux<T, 1>), cells, a);
+  if (a.any_laminar) HIPAMR_LAUNCH_LDS("gradient_stage", (k_grad<T, 1>), cells, vel_lds, a);
+  HIPAMR_LAUNCH_LDS("flux_rk1", (k_flux<T, 1>), cells, cell_lds, a);
   if (a.any_small) {
     HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 1);
     HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 1);
   }
-  if (a.any_laminar) HIPAMR_LAUNCH("gradient_stage", (k_grad<T, 2>), cells, a);
-  HIPAMR_LAUNCH("flux_rk2", (k_flux<T, 2>), cells, a);
+  if (a.any_laminar) HIPAMR_LAUNCH_LDS("gradient_stage", (k_grad<T, 2>), cells, vel_lds, a);
+  HIPAMR_LAUNCH_LDS("flux_rk2", (k_flux<T, 2>), cells, cell_lds, a);
   if (a.any_small) {
     HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 2);
     HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 2);
@@ -917,6 +995,7 @@ int launch_trial(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
   return launches;
 }
 #undef HIPAMR_LAUNCH
+#undef HIPAMR_LAUNCH_LDS
 
 template <class T>
 void launch_monitor(const ArenaPtrs<T>& a, hipStream_t s) {
