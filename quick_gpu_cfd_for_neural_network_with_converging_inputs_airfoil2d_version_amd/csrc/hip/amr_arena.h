@@ -34,6 +34,26 @@ static_assert(sizeof(Control) == 104, "Control layout");
 
 enum : int { VISC_VOLUME = 0, VISC_SRD_RATE = 1, VISC_SRD_GROUP = 2 };
 
+// Face items pack (right cell + 1) into the low bits of one int and the face info above them.
+constexpr int kItemRightBits = 27;
+
+// Face items held in LDS at once per block, per cell of the tile (a tile of an LF1 / LF2 mesh
+// touches 2.2 - 2.7 faces per cell; a tile with more items is processed in several chunks).
+constexpr int kItemChunkNum = 5, kItemChunkDen = 2;
+__host__ __device__ constexpr int item_chunk(int tile) { return tile * kItemChunkNum / kItemChunkDen; }
+
+// Dynamic LDS of a flux launch: 16 T per cell, then per item of a chunk 4 + 1 T (convective and
+// viscous flux, SA flux) and, in the base pass, 4 T more (dt terms).  Gradient launches: 4 T
+// per cell.
+template <class T>
+constexpr size_t flux_lds_bytes(int tile, bool base) {
+  return (static_cast<size_t>(tile) * 16 + static_cast<size_t>(item_chunk(tile)) * (base ? 9 : 5)) * sizeof(T);
+}
+template <class T>
+constexpr size_t grad_lds_bytes(int tile) {
+  return static_cast<size_t>(tile) * 4 * sizeof(T);
+}
+
 template <class T>
 struct MeshDesc {
   int cell_begin, n_cells, face_begin, n_faces, wall_begin, n_walls;
@@ -62,8 +82,8 @@ struct ArenaPtrs {
   const T* dt_area;
   const T* wall_dist;
   const int* csr_off;    // nc + 1
-  const int4* CSR;       // per cell-face entry: {q (cell is the face's left) or ~q (right), the other
-                         // cell or -1 on a boundary face, face info, 0}
+  const uint16_t* SLOT;  // per cell-face entry: index of the face among its tile's items << 1,
+                         // | 1 when the cell is the face's right cell
   const int* wall_first; // first wall of the cell
   const int* wall_cnt;
   const int* term_off;   // nc + 1
@@ -77,9 +97,11 @@ struct ArenaPtrs {
   const int* mem_off;    // nc + 1
   const int* mem_grp;
   const T* mem_inv;      // 1 / N_k
-  // faces: FGEO = {length, dx, dy, ex, ey, distance, 0, 0}; face info: bit 0 axis, bits 1..3
-  // side + 1 (0 interior)
-  const T* FGEO;
+  // Face items: per cell block (tile) the faces with at least one cell in the tile, [x, y) in
+  // ITEM; 8 T words per item: ints {left or -1, (right + 1) | info << kItemRightBits} in words 0-1, then
+  // {length, dx, dy, ex, ey, distance}.  Face info: bit 0 axis, bits 1..3 side + 1 (0 interior).
+  const int2* item_range;
+  const T* ITEM;
   // walls
   const int* wall_owner;
   const int* wall_donor;

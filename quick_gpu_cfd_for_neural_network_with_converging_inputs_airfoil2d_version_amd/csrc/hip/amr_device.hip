@@ -111,8 +111,8 @@ SlabPlan plan_slab(const Totals& t, int nv) {
   SlabPlan p{};
   p.geometry = arr(sizeof(MeshDesc<T>), t.meshes) + arr(4, t.cell_blocks) + arr(4, t.face_blocks + t.wall_blocks) +
                5 * arr(sizeof(T), t.cells) + 2 * arr(4, t.cells) + 2 * arr(8, t.cells) + 4 * arr(4, t.cells + 1) +
-               arr(16, t.csr) + arr(4, t.terms) + 2 * arr(sizeof(T), t.terms) + arr(4, t.groups) +
-               arr(sizeof(T), t.groups) + arr(4, t.members) + arr(8 * sizeof(T), t.faces) +
+               arr(2, t.csr) + arr(8 * sizeof(T), t.csr) + arr(4, t.terms) + 2 * arr(sizeof(T), t.terms) +
+               arr(4, t.groups) + arr(sizeof(T), t.groups) + arr(4, t.members) + arr(8, t.cell_blocks) +
                2 * arr(4, t.walls) + 4 * arr(sizeof(T), t.walls) + 2 * arr(8, t.walls);
   p.control = arr(sizeof(Control), t.meshes) + arr(sizeof(double), t.meshes) + arr(sizeof(long long), t.meshes) +
               arr(6 * sizeof(double), t.meshes);
@@ -204,16 +204,18 @@ struct HostArrays {
   std::vector<T> vol, inv_vol, dt_area, wall_dist, mem_inv;
   std::vector<int> wall_first, wall_cnt, csr_off, term_off, grp_off, mem_off;
   std::vector<double> cell_cx, cell_cy;
-  std::vector<int> csr_ent, term_src, grp_mem, mem_grp;  // csr_ent: 4 per entry
+  std::vector<int> term_src, grp_mem, mem_grp;
+  std::vector<uint16_t> slot_ent;  // per cell-face entry: item index in the cell's tile << 1 | right
+  std::vector<T> items;            // 8 per item (in the cell-face index space)
+  std::vector<int> item_range;     // 2 per cell block
   std::vector<T> term_wx, term_wy, grp_w;
-  std::vector<int> face_idx;  // 4 per face: left, right, info, 0
-  std::vector<T> face_geo;    // 8 per face: length, dx, dy, ex, ey, distance, 0, 0
+  std::vector<int> face_idx;  // host only, for the layout check: 4 per face: left, right, info, 0
   std::vector<int> wall_owner, wall_donor;
   std::vector<T> wall_nx, wall_ny, wall_len, wall_ddist;
   std::vector<double> wall_mx, wall_my;
 };
 
-enum Space { CELLS, OFFSETS, FACES, WALLS, CSR, TERMS, GROUPS, MEMBERS, TABLE };
+enum Space { CELLS, OFFSETS, FACES, WALLS, CSR, TERMS, GROUPS, MEMBERS, TILES, TABLE };
 
 // One uploaded array: where its host copy and device copy live, in which index space.
 struct Column {
@@ -262,6 +264,7 @@ class DeviceArena final : public Arena {
       nm += c.members;
       if (nc >= (1 << 30) || nf >= (1 << 30) || nw >= (1 << 30) || ncsr >= (1 << 30) || nt >= (1 << 30))
         return "arena exceeds 2^30 elements in one index space";
+      if (nc >= (1 << kItemRightBits) - 1) return "arena exceeds 2^27 cells (face item packing)";
     }
     if (slots_.empty()) return "arena without slots";
     totals_ = totals_of(caps, tile);
@@ -294,7 +297,9 @@ class DeviceArena final : public Arena {
     h.term_off.assign(nc + 1, 0);
     h.grp_off.assign(nc + 1, 0);
     h.mem_off.assign(nc + 1, 0);
-    sized(h.csr_ent, 4 * static_cast<size_t>(std::max(ncsr, 1)), 0);
+    sized(h.slot_ent, static_cast<size_t>(ncsr), uint16_t(0));
+    sized(h.items, 8 * static_cast<size_t>(std::max(ncsr, 1)), T(0));
+    h.item_range.assign(2 * static_cast<size_t>(std::max(n_cell_blocks_, 1)), 0);
     sized(h.term_src, nt, -1);
     sized(h.term_wx, nt, T(0));
     sized(h.term_wy, nt, T(0));
@@ -302,7 +307,6 @@ class DeviceArena final : public Arena {
     sized(h.grp_w, ng, T(0));
     sized(h.mem_grp, nm, 0);
     sized(h.face_idx, 4 * static_cast<size_t>(std::max(nf, 1)), 0);
-    sized(h.face_geo, 8 * static_cast<size_t>(std::max(nf, 1)), T(0));
     sized(h.wall_owner, nw, 0);
     sized(h.wall_donor, nw, 0);
     sized(h.wall_nx, nw, T(0));
@@ -352,6 +356,8 @@ class DeviceArena final : public Arena {
     const std::vector<int>& at = order.cell_at;
     const std::vector<int>& pos_of = order.cell_pos;
     int csr = s.csr_begin, term = s.term_begin, grp = s.grp_begin, mem = s.mem_begin;
+    std::vector<int> entry_face;  // mesh face of each cell-face entry of the slot, in entry order
+    entry_face.reserve(static_cast<size_t>(v.face_offsets[n]));
     for (int kk = 0; kk < s.cap.cells; ++kk) {
       const int g = cb + kk;
       h.csr_off[g] = csr;
@@ -373,16 +379,8 @@ class DeviceArena final : public Arena {
       h.cell_cx[g] = c.cx;
       h.cell_cy[g] = c.cy;
       for (uint64_t pos = v.face_offsets[k]; pos < v.face_offsets[k + 1]; ++pos) {
-        const int q = static_cast<int>(v.cell_faces[pos]);
-        const int stored = fb + order.face_pos[q];
-        const amr_face& face = v.faces[q];
-        const bool left = face.left == k;
-        const int other = left ? face.right : face.left;
-        int* e = &h.csr_ent[4 * static_cast<size_t>(csr++)];
-        e[0] = left ? stored : ~stored;
-        e[1] = other >= 0 ? cb + pos_of[other] : -1;
-        e[2] = (face.axis & 1) | ((face.boundary + 1) << 1);
-        e[3] = 0;
+        entry_face.push_back(static_cast<int>(v.cell_faces[pos]));
+        ++csr;
       }
       if (laminar)
         for (uint64_t pos = v.visc_offsets[k]; pos < v.visc_offsets[k + 1]; ++pos) {
@@ -413,25 +411,12 @@ class DeviceArena final : public Arena {
       const int q = order.face_at[qq];
       const amr_face& face = v.faces[q];
       int* fi = &h.face_idx[4 * static_cast<size_t>(g)];
-      T* fg = &h.face_geo[8 * static_cast<size_t>(g)];
       fi[0] = face.left >= 0 ? cb + pos_of[face.left] : -1;
       fi[1] = face.right >= 0 ? cb + pos_of[face.right] : -1;
-      fi[2] = (face.axis & 1) | ((face.boundary + 1) << 1);
+      fi[2] = face_info(face);
       fi[3] = 0;
-      fg[0] = static_cast<T>(face.length);
-      if (laminar) {
-        const amr_face_geometry& geo = v.face_geometry[q];
-        fg[1] = static_cast<T>(geo.dx);
-        fg[2] = static_cast<T>(geo.dy);
-        fg[3] = static_cast<T>(geo.ex);
-        fg[4] = static_cast<T>(geo.ey);
-        fg[5] = static_cast<T>(face.boundary >= 0 ? v.face_samples[q].distance : geo.distance);
-      } else {
-        fg[1] = fg[2] = fg[3] = fg[4] = T(0);
-        fg[5] = T(1);
-      }
-      fg[6] = fg[7] = T(0);
     }
+    fill_items(m, v, order, entry_face);
     for (int j = 0; j < s.cap.walls; ++j) {
       const int g = wb + j;
       if (j >= nw) {
@@ -513,13 +498,15 @@ class DeviceArena final : public Arena {
     for (int k = s.cell_begin; k < s.cell_begin + s.n_cells; ++k) {
       if (h.csr_off[k] < s.csr_begin || h.csr_off[k] > h.csr_off[k + 1] || h.csr_off[k + 1] > s.csr_begin + s.cap.csr)
         return "layout: cell-face offsets";
+      const int b = k / tile_, i0 = h.item_range[2 * b], i1 = h.item_range[2 * b + 1];
       for (int j = h.csr_off[k]; j < h.csr_off[k + 1]; ++j) {
-        const int* e = &h.csr_ent[4 * static_cast<size_t>(j)];
-        const int sq = e[0], q = sq >= 0 ? sq : ~sq;
-        if (q < s.face_begin || q >= s.face_begin + s.n_faces) return "layout: cell-face id";
-        const int* fi = &h.face_idx[4 * static_cast<size_t>(q)];
-        if (fi[sq >= 0 ? 0 : 1] != k) return "layout: cell-face orientation";
-        if (e[1] != fi[sq >= 0 ? 1 : 0] || e[2] != fi[2]) return "layout: cell-face entry";
+        const int e = h.slot_ent[j], i = i0 + (e >> 1);
+        if (i >= i1) return "layout: cell-face slot";
+        int ids[2], l, r;
+        std::memcpy(ids, &h.items[8 * static_cast<size_t>(i)], sizeof(ids));
+        l = ids[0];
+        r = (ids[1] & ((1 << kItemRightBits) - 1)) - 1;
+        if ((e & 1 ? r : l) != k) return "layout: cell-face orientation";
       }
       if (h.wall_cnt[k] < 0 || h.wall_first[k] < s.wall_begin || h.wall_first[k] + h.wall_cnt[k] > s.wall_begin + s.n_walls)
         return "layout: cell walls";
@@ -539,6 +526,35 @@ class DeviceArena final : public Arena {
         if (!cell_ok(h.mem_grp[j])) return "layout: SRD group";
       if (h.mem_off[k + 1] == h.mem_off[k]) return "layout: cell in no SRD group";
     }
+    // Face items: tile ranges inside the slot's item space, in order; every item names cells of
+    // the slot with a valid side, one of them in the tile; each face of the slot appears in
+    // the tiles of its cells (as many items as the face_idx table implies).
+    long long expected = 0;
+    for (int q = s.face_begin; q < s.face_begin + s.n_faces; ++q) {
+      const int* fi = &h.face_idx[4 * static_cast<size_t>(q)];
+      const int tl = fi[0] >= 0 ? fi[0] / tile_ : -1, tr = fi[1] >= 0 ? fi[1] / tile_ : -1;
+      expected += (tl >= 0) + (tr >= 0 && tr != tl);
+    }
+    int prev = s.csr_begin;
+    long long found = 0;
+    for (int b = s.cell_begin / tile_; b < (s.cell_begin + s.cap.cells) / tile_; ++b) {
+      const int i0 = h.item_range[2 * b], i1 = h.item_range[2 * b + 1];
+      if (i0 != prev || i1 < i0 || i1 > s.csr_begin + s.cap.csr) return "layout: tile item range";
+      if ((i1 - i0) * 2 + 1 > 0xFFFF) return "layout: tile has too many face items";
+      prev = i1;
+      for (int i = i0; i < i1; ++i) {
+        int ids[2];
+        std::memcpy(ids, &h.items[8 * static_cast<size_t>(i)], sizeof(ids));
+        const int l = ids[0], r = (ids[1] & ((1 << kItemRightBits) - 1)) - 1, info = ids[1] >> kItemRightBits;
+        const int side = (info >> 1) - 1;
+        if (ids[1] < 0 || side < -1 || side > 3) return "layout: item info";
+        if ((l != -1 && !cell_ok(l)) || (r != -1 && !cell_ok(r)) || (l < 0 && r < 0)) return "layout: item cells";
+        if (side < 0 && (l < 0 || r < 0)) return "layout: interior item missing a cell";
+        if (!((l >= 0 && l / tile_ == b) || (r >= 0 && r / tile_ == b))) return "layout: item outside its tile";
+        ++found;
+      }
+    }
+    if (found != expected) return "layout: face item count";
     for (int q = s.face_begin; q < s.face_begin + s.n_faces; ++q) {
       const int* fi = &h.face_idx[4 * static_cast<size_t>(q)];
       const int l = fi[0], r = fi[1], side = (fi[2] >> 1) - 1;
@@ -589,14 +605,15 @@ class DeviceArena final : public Arena {
     for (auto* v : {&h.wall_first, &h.wall_cnt}) col(*v, CELLS);
     for (auto* v : {&h.cell_cx, &h.cell_cy}) col(*v, CELLS);
     for (auto* v : {&h.csr_off, &h.term_off, &h.grp_off, &h.mem_off}) col(*v, OFFSETS);
-    columns_.push_back({h.csr_ent.data(), nullptr, 4 * sizeof(int), h.csr_ent.size() / 4, CSR});
+    col(h.slot_ent, CSR);
+    columns_.push_back({h.items.data(), nullptr, 8 * sizeof(T), h.items.size() / 8, CSR});
     col(h.term_src, TERMS);
     col(h.term_wx, TERMS);
     col(h.term_wy, TERMS);
     col(h.grp_mem, GROUPS);
     col(h.grp_w, GROUPS);
     col(h.mem_grp, MEMBERS);
-    columns_.push_back({h.face_geo.data(), nullptr, 8 * sizeof(T), h.face_geo.size() / 8, FACES});
+    columns_.push_back({h.item_range.data(), nullptr, 2 * sizeof(int), h.item_range.size() / 2, TILES});
     for (auto* v : {&h.wall_owner, &h.wall_donor}) col(*v, WALLS);
     for (auto* v : {&h.wall_nx, &h.wall_ny, &h.wall_len, &h.wall_ddist}) col(*v, WALLS);
     for (auto* v : {&h.wall_mx, &h.wall_my}) col(*v, WALLS);
@@ -613,6 +630,12 @@ class DeviceArena final : public Arena {
     geometry_bytes_ = geo;
     state_bytes_ = static_cast<uint64_t>(nv) * nc * sizeof(double);
     char* base = nullptr;
+    int device = 0, lds_max = 0;
+    if (hipGetDevice(&device) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+      return "AMR device query failed";
+    if (flux_lds_bytes<T>(tile_, true) > static_cast<size_t>(lds_max))
+      return "AMR tile needs more LDS per block than the device provides";
     if (hipMalloc(reinterpret_cast<void**>(&base), slab_bytes_) != hipSuccess) return "AMR hipMalloc of the arena failed";
     slab_ = base;
     size_t offset = 0;
@@ -639,14 +662,15 @@ class DeviceArena final : public Arena {
     a.term_off = next(static_cast<const int*>(nullptr));
     a.grp_off = next(static_cast<const int*>(nullptr));
     a.mem_off = next(static_cast<const int*>(nullptr));
-    a.CSR = next(static_cast<const int4*>(nullptr));
+    a.SLOT = next(static_cast<const uint16_t*>(nullptr));
+    a.ITEM = next(static_cast<const T*>(nullptr));
     a.term_src = next(static_cast<const int*>(nullptr));
     a.term_wx = next(static_cast<const T*>(nullptr));
     a.term_wy = next(static_cast<const T*>(nullptr));
     a.grp_mem = next(static_cast<const int*>(nullptr));
     a.grp_w = next(static_cast<const T*>(nullptr));
     a.mem_grp = next(static_cast<const int*>(nullptr));
-    a.FGEO = next(static_cast<const T*>(nullptr));
+    a.item_range = next(static_cast<const int2*>(nullptr));
     a.wall_owner = next(static_cast<const int*>(nullptr));
     a.wall_donor = next(static_cast<const int*>(nullptr));
     a.wall_nx = next(static_cast<const T*>(nullptr));
@@ -742,6 +766,7 @@ class DeviceArena final : public Arena {
         case TERMS: begin = s.term_begin, count = s.cap.terms; break;
         case GROUPS: begin = s.grp_begin, count = s.cap.groups; break;
         case MEMBERS: begin = s.mem_begin, count = s.cap.members; break;
+        case TILES: begin = s.cell_begin / tile_, count = s.cap.cells / tile_; break;
         case TABLE: begin = 0, count = 0; break;
       }
       if (!count) continue;
@@ -866,6 +891,70 @@ class DeviceArena final : public Arena {
     return c;
   }
 
+  static int face_info(const amr_face& f) { return (f.axis & 1) | ((f.boundary + 1) << 1); }
+
+  // Face items of slot m: per tile, the faces with at least one cell in the tile (interior faces
+  // first, then boundary faces, each group in storage order), and every cell-face entry's index
+  // into its tile's list.  entry_face: the mesh face of each entry, in entry order.
+  void fill_items(int m, const amr_mesh_view& v, const MeshOrder& order, const std::vector<int>& entry_face) {
+    const MeshSlot& s = slots_[m];
+    HostArrays<T>& h = h_;
+    const int cb = s.cell_begin, n = static_cast<int>(v.n_cells);
+    const bool laminar = v.transport.model == 1;
+    std::vector<int> slot_of(v.n_faces, -1), list;
+    int item = s.csr_begin;
+    for (int b = 0; b < s.cap.cells / tile_; ++b) {
+      const int lo = b * tile_, hi = std::min(n, lo + tile_);
+      list.clear();
+      for (int kk = lo; kk < hi; ++kk)
+        for (int j = h.csr_off[cb + kk]; j < h.csr_off[cb + kk + 1]; ++j) {
+          const int q = entry_face[j - s.csr_begin];
+          if (slot_of[q] < 0) {
+            slot_of[q] = 0;
+            list.push_back(q);
+          }
+        }
+      std::sort(list.begin(), list.end(), [&](int x, int y) {
+        const bool bx = v.faces[x].boundary >= 0, by = v.faces[y].boundary >= 0;
+        return bx != by ? by : order.face_pos[x] < order.face_pos[y];
+      });
+      const int tb = cb / tile_ + b;
+      h.item_range[2 * tb] = item;
+      for (size_t i = 0; i < list.size(); ++i) {
+        const int q = list[i];
+        slot_of[q] = static_cast<int>(i);
+        const amr_face& face = v.faces[q];
+        T* w = &h.items[8 * static_cast<size_t>(item++)];
+        std::fill(w, w + 8, T(0));
+        const int ids[2] = {face.left >= 0 ? cb + order.cell_pos[face.left] : -1,
+                            (face.right >= 0 ? cb + order.cell_pos[face.right] + 1 : 0) |
+                                (face_info(face) << kItemRightBits)};
+        std::memcpy(w, ids, sizeof(ids));
+        w[2] = static_cast<T>(face.length);
+        if (laminar) {
+          const amr_face_geometry& geo = v.face_geometry[q];
+          w[3] = static_cast<T>(geo.dx);
+          w[4] = static_cast<T>(geo.dy);
+          w[5] = static_cast<T>(geo.ex);
+          w[6] = static_cast<T>(geo.ey);
+          w[7] = static_cast<T>(face.boundary >= 0 ? v.face_samples[q].distance : geo.distance);
+        } else {
+          w[7] = T(1);
+        }
+      }
+      h.item_range[2 * tb + 1] = item;
+      for (int kk = lo; kk < hi; ++kk) {
+        const int k = order.cell_at[kk];
+        for (int j = h.csr_off[cb + kk]; j < h.csr_off[cb + kk + 1]; ++j) {
+          const int q = entry_face[j - s.csr_begin];
+          const int right = v.faces[q].right == k ? 1 : 0;
+          h.slot_ent[j] = static_cast<uint16_t>((slot_of[q] << 1) | right);
+        }
+      }
+      for (const int q : list) slot_of[q] = -1;
+    }
+  }
+
   void pad_cell(int g, int wb) {
     HostArrays<T>& h = h_;
     h.vol[g] = h.inv_vol[g] = h.dt_area[g] = h.wall_dist[g] = h.mem_inv[g] = T(1);
@@ -876,11 +965,8 @@ class DeviceArena final : public Arena {
   void pad_face(int g, int cb) {
     HostArrays<T>& h = h_;
     int* fi = &h.face_idx[4 * static_cast<size_t>(g)];
-    T* fg = &h.face_geo[8 * static_cast<size_t>(g)];
     fi[0] = fi[1] = cb;
     fi[2] = fi[3] = 0;
-    for (int i = 0; i < 8; ++i) fg[i] = T(0);
-    fg[5] = T(1);
   }
   void pad_wall(int g, int cb) {
     HostArrays<T>& h = h_;
@@ -901,6 +987,8 @@ class DeviceArena final : public Arena {
       pad_cell(g, s.wall_begin);
     }
     for (int q = 0; q < s.cap.faces; ++q) pad_face(s.face_begin + q, s.cell_begin);
+    for (int b = s.cell_begin / tile_; b < (s.cell_begin + s.cap.cells) / tile_; ++b)
+      h.item_range[2 * b] = h.item_range[2 * b + 1] = s.csr_begin;
     for (int j = 0; j < s.cap.walls; ++j) pad_wall(s.wall_begin + j, s.cell_begin);
     MeshDesc<T> d{};
     d.cell_begin = s.cell_begin;

@@ -127,14 +127,9 @@ __device__ __forceinline__ void store_grad(const ArenaPtrs<T>& a, int k, const C
 
 // Cell vectors of the block's own tile staged in LDS: a neighbour inside the tile (most of them,
 // with cells in Hilbert order) is read from LDS instead of being gathered from L2 / HBM again.
-// The arrays live in the launch's dynamic LDS (tile_lds_bytes per block), so a block holds only
-// what its tile size needs and small tiles keep their occupancy.
+// The flux kernels also keep one chunk of face results there (FaceSlots).  Both live in the
+// launch's dynamic LDS, sized by the tile, so small tiles keep their occupancy.
 extern __shared__ __align__(16) unsigned char tile_lds[];
-
-template <class T>
-constexpr size_t tile_lds_bytes(int tile) {
-  return static_cast<size_t>(tile) * 16 * sizeof(T);
-}
 
 template <class T>
 struct TileCells {
@@ -143,6 +138,18 @@ struct TileCells {
   T* g;  // 8 per cell: the gradient record
   __device__ explicit TileCells(int tile)
       : v(reinterpret_cast<T*>(tile_lds)), p(v + 4 * tile), g(v + 8 * tile) {}
+};
+
+// Results of one chunk of face items: f = mass, x- and y-momentum, energy flux (times the
+// length, convective minus viscous), sf = SA flux, x = spectral radius x length, viscous and SA
+// dt coefficients (base pass only).
+template <class T>
+struct FaceSlots {
+  T* f;
+  T* sf;
+  T* x;
+  __device__ explicit FaceSlots(int tile)
+      : f(reinterpret_cast<T*>(tile_lds) + 16 * tile), sf(f + 4 * item_chunk(tile)), x(sf + item_chunk(tile)) {}
 };
 
 template <class T>
@@ -186,6 +193,34 @@ __device__ __forceinline__ FaceGeo<double> load_geo(const double* g) {
   const double2 a = *reinterpret_cast<const double2*>(g), b = *reinterpret_cast<const double2*>(g + 2);
   const double2 c = *reinterpret_cast<const double2*>(g + 4);
   return FaceGeo<double>{a.x, a.y, b.x, b.y, c.x, c.y};
+}
+
+// Face item of a tile: 8 T words.  Words 0-1 hold two ints (left cell or -1; right cell + 1
+// in bits 0..kItemRightBits-1 and the face info above); words 2-7 the face geometry.
+template <class T>
+struct FaceItem {
+  int left, right, info;
+  FaceGeo<T> geo;
+};
+__device__ __forceinline__ void unpack_item_ids(int w0, int w1, int& left, int& right, int& info) {
+  left = w0;
+  right = (w1 & ((1 << kItemRightBits) - 1)) - 1;
+  info = w1 >> kItemRightBits;
+}
+__device__ __forceinline__ FaceItem<float> load_item(const float* p) {
+  const int4 a = *reinterpret_cast<const int4*>(p);
+  const float4 b = *reinterpret_cast<const float4*>(p + 4);
+  FaceItem<float> it;
+  unpack_item_ids(a.x, a.y, it.left, it.right, it.info);
+  it.geo = FaceGeo<float>{__int_as_float(a.z), __int_as_float(a.w), b.x, b.y, b.z, b.w};
+  return it;
+}
+__device__ __forceinline__ FaceItem<double> load_item(const double* p) {
+  const int4 a = *reinterpret_cast<const int4*>(p);
+  FaceItem<double> it;
+  unpack_item_ids(a.x, a.y, it.left, it.right, it.info);
+  it.geo = load_geo(p + 2);
+  return it;
 }
 
 // U -> record primitives (computed in FP64 from the FP64 state, stored as T), plus molecular
@@ -527,56 +562,99 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
   const bool sa = d.sa != 0, laminar = d.laminar != 0;
   const int rb = read_buffer(c, STAGE);
   const int bit = BASE ? BAD_BASE : BAD_STAGE;
-  const int first = tile * static_cast<int>(blockDim.x);
-  const int staged = min(static_cast<int>(blockDim.x), d.cell_begin + d.n_cells - first);  // valid cells of the tile
-  const TileCells<T> lds(static_cast<int>(blockDim.x));
+  const int tn = static_cast<int>(blockDim.x), tid = static_cast<int>(threadIdx.x);
+  const int first = tile * tn;
+  const int staged = min(tn, d.cell_begin + d.n_cells - first);  // valid cells of the tile
+  const TileCells<T> lds(tn);
+  const FaceSlots<T> slots(tn);
   CellPrim<T> own{};
   CellGrad<T> owng{};
   if (valid) {
     own = load_prim(a, rb, k);
     if (laminar) owng = load_grad(a, k);
-    stage_cell(lds, static_cast<int>(threadIdx.x), own, owng, laminar);
+    stage_cell(lds, tid, own, owng, laminar);
   }
   __syncthreads();
-  double local_dt = INFINITY;
-  if (valid) {
-    T s0 = 0, s1 = 0, s2 = 0, s3 = 0, spectral = 0, diffusion = 0, sa_net = 0, sa_spec = 0;
-    bool ok = true;
-    const int end = a.csr_off[k + 1];
-    for (int j = a.csr_off[k]; j < end; ++j) {
-      const int4 e = a.CSR[j];
-      const bool self_left = e.x >= 0;
-      const int q = self_left ? e.x : ~e.x;
-      const int other = e.y;  // -1 on a boundary face
-      const int4 fi = make_int4(self_left ? k : other, self_left ? other : k, e.z, 0);
-      const FaceGeo<T> geo = load_geo(a.FGEO + 8 * q);
-      CellPrim<T> nb = own;
-      CellGrad<T> nbg = owng;
-      if (other >= 0) {
-        const int local = other - first;
-        if (local >= 0 && local < staged) {
-          nb = tile_prim(lds, local);
-          if (laminar) nbg = tile_grad(lds, local);
+  // Face items of the tile, one chunk at a time: (A) each face touching the tile is evaluated
+  // once, by one thread, into its LDS slot; (B) each cell sums its faces' slots in its cell-face
+  // order.  A face shared with another tile is evaluated by both tiles from the same inputs, so
+  // both cells see the same bits.
+  const int2 range = a.item_range[tile];
+  const int chunk = item_chunk(tn);
+  T s0 = 0, s1 = 0, s2 = 0, s3 = 0, spectral = 0, diffusion = 0, sa_net = 0, sa_spec = 0;
+  bool ok = true;
+  for (int c0 = range.x; c0 < range.y; c0 += chunk) {
+    const int c1 = min(range.y, c0 + chunk);
+    for (int i = c0 + tid; i < c1; i += tn) {
+      const FaceItem<T> it = load_item(a.ITEM + 8 * static_cast<size_t>(i));
+      CellPrim<T> L{}, R{};
+      CellGrad<T> Lg{}, Rg{};
+      const int ll = it.left - first, rl = it.right - first;
+      if (it.left >= 0) {
+        if (ll >= 0 && ll < staged) {
+          L = tile_prim(lds, ll);
+          if (laminar) Lg = tile_grad(lds, ll);
         } else {
-          nb = load_prim(a, rb, other);
-          if (laminar) nbg = load_grad(a, other);
+          L = load_prim(a, rb, it.left);
+          if (laminar) Lg = load_grad(a, it.left);
         }
       }
-      const FaceFlux<T> f = face_flux<T, BASE>(d, fi, geo, own, owng, pick(self_left, own, nb), pick(self_left, nb, own),
-                                               pick(self_left, owng, nbg), pick(self_left, nbg, owng));
+      if (it.right >= 0) {
+        if (rl >= 0 && rl < staged) {
+          R = tile_prim(lds, rl);
+          if (laminar) Rg = tile_grad(lds, rl);
+        } else {
+          R = load_prim(a, rb, it.right);
+          if (laminar) Rg = load_grad(a, it.right);
+        }
+      }
+      // A boundary face has one cell: it is the inside cell and both sides' record.
+      if (it.left < 0) {
+        L = R;
+        Lg = Rg;
+      } else if (it.right < 0) {
+        R = L;
+        Rg = Lg;
+      }
+      const FaceFlux<T> f =
+          face_flux<T, BASE>(d, make_int4(it.left, it.right, it.info, 0), it.geo, L, Lg, L, R, Lg, Rg);
       ok = ok && f.ok;
-      const T sgn = self_left ? T(1) : T(-1);
-      s0 += sgn * f.f0;
-      s1 += sgn * f.fx;
-      s2 += sgn * f.fy;
-      s3 += sgn * f.f3;
-      if (sa) sa_net += sgn * f.sflux;
-      if (BASE) {
-        spectral += f.speed * geo.len;
-        if (laminar) diffusion += f.fcoef * geo.len;
-        if (sa) sa_spec += f.sfcoef;
+      const int sl = i - c0;
+      store4(slots.f + 4 * sl, f.f0, f.fx, f.fy, f.f3);
+      if (sa) slots.sf[sl] = f.sflux;
+      if (BASE) store4(slots.x + 4 * sl, f.speed * it.geo.len, f.fcoef * it.geo.len, f.sfcoef, T(0));
+    }
+    __syncthreads();
+    if (valid) {
+      const int lo = c0 - range.x, n_slots = c1 - c0;
+      const int end = a.csr_off[k + 1];
+      for (int j = a.csr_off[k]; j < end; ++j) {
+        const int e = a.SLOT[j];
+        const int sl = (e >> 1) - lo;
+        if (sl < 0 || sl >= n_slots) continue;
+        const T sgn = (e & 1) ? T(-1) : T(1);
+        T f0, fx, fy, f3;
+        load4(slots.f + 4 * sl, f0, fx, fy, f3);
+        s0 += sgn * f0;
+        s1 += sgn * fx;
+        s2 += sgn * fy;
+        s3 += sgn * f3;
+        if (sa) sa_net += sgn * slots.sf[sl];
+        if (BASE) {
+          T xs, xd, xa, pad;
+          load4(slots.x + 4 * sl, xs, xd, xa, pad);
+          spectral += xs;
+          if (laminar) diffusion += xd;
+          if (sa) sa_spec += xa;
+        }
       }
     }
+    __syncthreads();  // the next chunk reuses the slots
+  }
+  if (!ok) flag_bad(c, bit);  // a face of the tile, whichever thread evaluated it
+  ok = true;
+  double local_dt = INFINITY;
+  if (valid) {
     const T rho = own.rho, p = own.p;
     const int w0 = a.wall_first[k], w1 = w0 + a.wall_cnt[k];
     if (w1 > w0) {
@@ -954,11 +1032,11 @@ inline int blocks_of(int n, int tile) { return (n + tile - 1) / tile; }
 template <class T>
 int launch_residual(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
   const dim3 tile(a.tile), cells(a.n_cell_blocks);
-  const size_t cell_lds = tile_lds_bytes<T>(a.tile), vel_lds = cell_lds / 4;
+  const size_t base_lds = flux_lds_bytes<T>(a.tile, true), vel_lds = grad_lds_bytes<T>(a.tile);
   int launches = 0;
   HIPAMR_LAUNCH("base_primitive", k_base_prim<T>, cells, a);
   if (a.any_laminar) HIPAMR_LAUNCH_LDS("gradient_base", (k_grad<T, 0>), cells, vel_lds, a);
-  HIPAMR_LAUNCH_LDS("flux_base", (k_flux<T, 0>), cells, cell_lds, a);
+  HIPAMR_LAUNCH_LDS("flux_base", (k_flux<T, 0>), cells, base_lds, a);
   if (a.any_visc_group) {
     HIPAMR_LAUNCH("diffusion_group", k_diffusion_group<T>, cells, a);
     HIPAMR_LAUNCH("dt_group", k_dt_group<T>, cells, a);
@@ -969,7 +1047,7 @@ int launch_residual(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
 template <class T>
 int launch_trial(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
   const dim3 tile(a.tile), cells(a.n_cell_blocks);
-  const size_t cell_lds = tile_lds_bytes<T>(a.tile), vel_lds = cell_lds / 4;
+  const size_t stage_lds = flux_lds_bytes<T>(a.tile, false), vel_lds = grad_lds_bytes<T>(a.tile);
   int launches = launch_residual(a, s, hook);
   HIPAMR_LAUNCH("stage0", k_stage0<T>, cells, a);
   if (a.any_small) {
@@ -977,13 +1055,13 @@ int launch_trial(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
     HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 0);
   }
   if (a.any_laminar) HIPAMR_LAUNCH_LDS("gradient_stage", (k_grad<T, 1>), cells, vel_lds, a);
-  HIPAMR_LAUNCH_LDS("flux_rk1", (k_flux<T, 1>), cells, cell_lds, a);
+  HIPAMR_LAUNCH_LDS("flux_rk1", (k_flux<T, 1>), cells, stage_lds, a);
   if (a.any_small) {
     HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 1);
     HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 1);
   }
   if (a.any_laminar) HIPAMR_LAUNCH_LDS("gradient_stage", (k_grad<T, 2>), cells, vel_lds, a);
-  HIPAMR_LAUNCH_LDS("flux_rk2", (k_flux<T, 2>), cells, cell_lds, a);
+  HIPAMR_LAUNCH_LDS("flux_rk2", (k_flux<T, 2>), cells, stage_lds, a);
   if (a.any_small) {
     HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 2);
     HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 2);

@@ -22,7 +22,7 @@ the steady tolerance because the update dt*r falls below one ulp of u), so its F
 wu = 8, ws = wg = 4: `design_stage_bytes` counts those bytes, `contract_stage_bytes` the
 single-width figure; the benchmark reports achieved bandwidth with the design bytes and shows
 both.  A step is three stages.  The achieved bandwidth of a stage is B_stage / (time of the
-kernels that make up the stage pass: gradient + face + collect).
+kernels that make up the stage pass: gradient + flux).
 """
 from __future__ import annotations
 
