@@ -253,11 +253,15 @@ def main():
             t0 = time.perf_counter()
             res32 = b.solve(cases, mode="steady", budget=args.budget)
             wall32 = time.perf_counter() - t0
+            solve_stats = b.stats()
         conv32 = sum(1 for r in res32 if r["converged"])
         steps32 = [r["steps"] for r in res32]
         value = 3600.0 * conv32 / wall32
         log(f"batched FP32 steady: {conv32}/{args.batch} converged in {wall32:.1f} s, steps "
-            f"{min(steps32)}/{np.mean(steps32):.0f}/{max(steps32)} -> {value:.0f} solves/hour")
+            f"min/p50/mean/p90/max {min(steps32)}/{np.median(steps32):.0f}/{np.mean(steps32):.0f}/"
+            f"{np.percentile(steps32, 90):.0f}/{max(steps32)} -> {value:.0f} solves/hour; "
+            f"{solve_stats['status_polls']} control polls, {solve_stats['advance_seconds']:.1f} s advancing; "
+            f"ideal at the full-batch step time {sum(steps32) * step_seconds / args.batch:.1f} s")
 
         # 4. FP64 re-solve of the first --fp64-check meshes (FP64 batch == FP64 sessions bitwise).
         k = min(args.fp64_check, args.batch)

@@ -144,7 +144,7 @@ typedef struct amr_session_options {
   int32_t batch_trials; /* graph replays per host poll, 1..1024 */
   int32_t graph;        /* 1: replay a captured hipGraph of one step */
   int32_t event_timing; /* 1: hipEvent time per poll batch */
-  int32_t precision;    /* 64 (FP64 state and fluxes) or 32 (FP32 state, FP64 control) */
+  int32_t precision;    /* 64 (all FP64) or 32 (FP32 fluxes and work arrays; FP64 state, control, reductions) */
 } amr_session_options;
 typedef struct amr_state_meta { double time; int64_t steps, rejected; } amr_state_meta;
 /* Mirrors AmrGpuStats (src/static_amr2d_gpu.h:9-16). */
@@ -293,10 +293,12 @@ int amr_batch_release(amr_batch* b, int32_t slot, char* err, size_t err_len);
  * failed slots ignore theirs. */
 int amr_batch_set_targets(amr_batch* b, const double* targets, const int64_t* maxsteps, char* err,
                           size_t err_len);
-/* Replay trials until every slot is finished or failed (until = 0) or until at least one slot
- * that was running at the call has finished or failed (until = 1), polling the control records
- * every batch_trials trials.  states (may be NULL) receives every slot.  0 done, 2 wall budget
- * expired (0 = no budget), 1 device error. */
+/* Replay trials until every slot is finished or failed (until = 0), until at least one slot
+ * that was running at the call has finished or failed (until = 1), or, grouped (until = 2),
+ * until an eighth of those slots have ended or eight polls have passed since the first ended,
+ * polling the control records every batch_trials trials.  Ended slots wait idle, so grouping
+ * trades a short wait for fewer checks.  states (may be NULL) receives every slot.  0 done,
+ * 2 wall budget expired (0 = no budget), 1 device error. */
 int amr_batch_advance(amr_batch* b, int32_t until, double wall_budget_seconds, amr_slot_state* states, char* err,
                       size_t err_len);
 /* Exactly `trials` trials, then one poll. */

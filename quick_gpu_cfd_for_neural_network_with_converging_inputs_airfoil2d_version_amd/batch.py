@@ -151,9 +151,11 @@ class Batch:
         self._check(self._lib.amr_batch_set_targets(self.handle, N.dptr(t), ms, err, N.ERR_LEN), err)
 
     def advance(self, until: str = "all", budget: float = 0.0) -> int:
-        """Replay trials until all slots (or any running slot) finish; 0 done, 2 budget expired."""
+        """Replay trials until all slots finish ("all"), any running slot ends ("any"), or an eighth
+        of the running slots have ended / eight polls have passed since the first ended ("group");
+        0 done, 2 budget expired."""
         err = N.err_buffer()
-        mode = {"all": 0, "any": 1}[until]
+        mode = {"all": 0, "any": 1, "group": 2}[until]
         return self._check(self._lib.amr_batch_advance(self.handle, mode, budget, self._state_buf, err, N.ERR_LEN),
                            err)
 

@@ -71,10 +71,12 @@ template <class T>
 struct ArenaPtrs {
   int n_meshes, nc, nf, nw, tile;
   int n_cell_blocks, n_face_blocks, n_wall_blocks;
+  int n_grid_blocks;  // blocks of a cell launch (>= n_cell_blocks, a multiple of 8)
   int any_laminar, any_sa, any_small, any_visc_group;
   const MeshDesc<T>* desc;
   Control* ctrl;
   const int* cell_block_mesh;  // n_cell_blocks
+  const int* block_tile;       // n_grid_blocks: tile of each block of a cell launch, -1 none
   const int* face_block_mesh;  // n_face_blocks + n_wall_blocks (faces first, then walls)
   // cells
   const T* vol;

@@ -265,7 +265,7 @@ int amr_batch_set_targets(amr_batch* b, const double* targets, const int64_t* ma
 
 int amr_batch_advance(amr_batch* b, int32_t until, double wall_budget_seconds, amr_slot_state* states, char* err,
                       size_t err_len) {
-  if (!b || (until != 0 && until != 1) || !std::isfinite(wall_budget_seconds) || wall_budget_seconds < 0)
+  if (!b || until < 0 || until > 2 || !std::isfinite(wall_budget_seconds) || wall_budget_seconds < 0)
     return fail(err, err_len, "invalid AMR batch advance");
   const auto t0 = Clock::now();
   auto last = t0;
@@ -277,6 +277,11 @@ int amr_batch_advance(amr_batch* b, int32_t until, double wall_budget_seconds, a
     n_running += running[m];
   }
   bool expired = false;
+  // until = 2: once a slot has ended, keep going until an eighth of the running slots have
+  // ended or kGroupPolls more polls have passed, so that one check serves several slots.
+  constexpr int kGroupPolls = 8;
+  const int group = std::max(1, n_running / 8);
+  int polls = 0, first_end = -1;
   while (n_running) {
     for (int trial = 0; trial < b->options.batch_trials; ++trial)
       if (launch_one(b)) return fail(err, err_len, "AMR trial launch failed");
@@ -292,7 +297,13 @@ int amr_batch_advance(amr_batch* b, int32_t until, double wall_budget_seconds, a
                    static_cast<unsigned long long>(b->stats.status_polls));
       last = now;
     }
-    if (still == 0 || (until == 1 && still < n_running)) break;
+    ++polls;
+    const int ended = n_running - still;
+    if (still == 0 || (until == 1 && ended > 0)) break;
+    if (until == 2 && ended > 0) {
+      if (first_end < 0) first_end = polls;
+      if (ended >= group || polls - first_end >= kGroupPolls) break;
+    }
     if (wall_budget_seconds > 0 && std::chrono::duration<double>(now - t0).count() >= wall_budget_seconds) {
       expired = true;
       break;
@@ -362,12 +373,17 @@ int amr_batch_monitor(amr_batch* b, const int32_t* mask, const double* frames, c
   }
   if (!any) return 0;
   std::vector<double> cells, walls;
-  if (!checked(copy_sync(b->arena->control(), eval.data(), n * sizeof(Control), hipMemcpyHostToDevice, b->stream),
+  // One synchronisation for the whole evaluation: control in, residual, monitor sums and
+  // control out are ordered on the stream; read_monitor waits for all of them.
+  std::copy(eval.begin(), eval.end(), b->host_control);
+  if (!checked(hipMemcpyAsync(b->arena->control(), b->host_control, n * sizeof(Control), hipMemcpyHostToDevice,
+                              b->stream),
                err, err_len, "AMR control H2D"))
     return 1;
   b->arena->launch_residual(b->stream);
   if (!checked(b->arena->launch_monitor(b->stream, frames), err, err_len, "AMR monitor kernels") ||
-      !checked(copy_sync(b->host_control, b->arena->control(), n * sizeof(Control), hipMemcpyDeviceToHost, b->stream),
+      !checked(hipMemcpyAsync(b->host_control, b->arena->control(), n * sizeof(Control), hipMemcpyDeviceToHost,
+                              b->stream),
                err, err_len, "AMR control D2H") ||
       !checked(b->arena->read_monitor(&cells, &walls, b->stream), err, err_len, "AMR monitor D2H"))
     return 1;
@@ -402,10 +418,13 @@ int amr_batch_monitor(amr_batch* b, const int32_t* mask, const double* frames, c
     o[10] = mz;
     status[m] = 0;
   }
-  if (!checked(copy_sync(b->arena->control(), restore.data(), n * sizeof(Control), hipMemcpyHostToDevice, b->stream),
+  // The restore is ordered before every later use of the control records on the stream; the
+  // pinned host copy is only overwritten by later device-to-host copies on the same stream.
+  std::copy(restore.begin(), restore.end(), b->host_control);
+  if (!checked(hipMemcpyAsync(b->arena->control(), b->host_control, n * sizeof(Control), hipMemcpyHostToDevice,
+                              b->stream),
                err, err_len, "AMR control restore"))
     return 1;
-  std::copy(restore.begin(), restore.end(), b->host_control);
   return 0;
 }
 

@@ -78,7 +78,34 @@ inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 // Element counts of an arena's index spaces.
 struct Totals {
   size_t meshes, cells, faces, walls, csr, terms, groups, members, cell_blocks, face_blocks, wall_blocks;
+  size_t grid_blocks;
 };
+
+// Tile of each block of a cell launch.  Blocks go to the 8 XCDs in turn (block b to XCD b % 8);
+// every slot's tiles are cut into 8 contiguous runs, run j going to XCD (j + slot) % 8, and each
+// XCD takes its runs in slot order.  So any set of running slots keeps all XCDs busy, tiles that
+// share neighbours share an L2, and the rotation evens out the runs' unequal lengths.  Blocks
+// past the end of an XCD's list get -1.
+std::vector<int> block_tiles(const std::vector<SlotCapacity>& caps, int tile) {
+  std::vector<int> list[8];
+  int t0 = 0;
+  for (size_t m = 0; m < caps.size(); ++m) {
+    const int n = caps[m].cells / tile;
+    for (int j = 0; j < 8; ++j) {
+      const int lo = t0 + static_cast<int>(static_cast<long long>(j) * n / 8);
+      const int hi = t0 + static_cast<int>(static_cast<long long>(j + 1) * n / 8);
+      std::vector<int>& l = list[(j + m) % 8];
+      for (int t = lo; t < hi; ++t) l.push_back(t);
+    }
+    t0 += n;
+  }
+  size_t longest = 0;
+  for (const auto& l : list) longest = std::max(longest, l.size());
+  std::vector<int> out(8 * std::max<size_t>(longest, 1), -1);
+  for (int x = 0; x < 8; ++x)
+    for (size_t i = 0; i < list[x].size(); ++i) out[8 * i + x] = list[x][i];
+  return out;
+}
 
 Totals totals_of(const std::vector<SlotCapacity>& caps, int tile) {
   Totals t{};
@@ -93,6 +120,7 @@ Totals totals_of(const std::vector<SlotCapacity>& caps, int tile) {
     t.members += c.members;
   }
   t.cell_blocks = t.cells / tile;
+  t.grid_blocks = block_tiles(caps, tile).size();
   t.face_blocks = t.faces / tile;
   t.wall_blocks = t.walls / tile;
   return t;
@@ -109,7 +137,8 @@ template <class T>
 SlabPlan plan_slab(const Totals& t, int nv) {
   const auto arr = [](size_t elem, size_t count) { return align256(elem * std::max<size_t>(count, 1)); };
   SlabPlan p{};
-  p.geometry = arr(sizeof(MeshDesc<T>), t.meshes) + arr(4, t.cell_blocks) + arr(4, t.face_blocks + t.wall_blocks) +
+  p.geometry = arr(sizeof(MeshDesc<T>), t.meshes) + arr(4, t.cell_blocks) + arr(4, t.grid_blocks) +
+               arr(4, t.face_blocks + t.wall_blocks) +
                5 * arr(sizeof(T), t.cells) + 2 * arr(4, t.cells) + 2 * arr(8, t.cells) + 4 * arr(4, t.cells + 1) +
                arr(2, t.csr) + arr(8 * sizeof(T), t.csr) + arr(4, t.terms) + 2 * arr(sizeof(T), t.terms) +
                arr(4, t.groups) + arr(sizeof(T), t.groups) + arr(4, t.members) + arr(8, t.cell_blocks) +
@@ -200,7 +229,7 @@ MeshOrder order_of(const amr_mesh_view& v) {
 template <class T>
 struct HostArrays {
   std::vector<MeshDesc<T>> desc;
-  std::vector<int> cell_block_mesh, face_block_mesh;
+  std::vector<int> cell_block_mesh, block_tile, face_block_mesh;
   std::vector<T> vol, inv_vol, dt_area, wall_dist, mem_inv;
   std::vector<int> wall_first, wall_cnt, csr_off, term_off, grp_off, mem_off;
   std::vector<double> cell_cx, cell_cy;
@@ -282,6 +311,7 @@ class DeviceArena final : public Arena {
       for (int b = 0; b < slots_[m].cap.walls / tile; ++b) h.face_block_mesh.push_back(static_cast<int>(m));
     n_wall_blocks_ = static_cast<int>(h.face_block_mesh.size()) - n_face_blocks_;
     n_cell_blocks_ = static_cast<int>(h.cell_block_mesh.size());
+    h.block_tile = block_tiles(caps, tile);
     if (h.face_block_mesh.empty()) h.face_block_mesh.push_back(0);
     const auto sized = [](auto& v, size_t n, auto value) { v.assign(std::max<size_t>(n, 1), value); };
     sized(h.vol, nc, T(1));
@@ -572,6 +602,13 @@ class DeviceArena final : public Arena {
   std::string check_tables() const {
     const HostArrays<T>& h = h_;
     if (static_cast<int>(h.cell_block_mesh.size()) * tile_ != nc_) return "layout: cell block table";
+    std::vector<int> seen(n_cell_blocks_, 0);
+    if (h.block_tile.size() % 8) return "layout: block tile table size";
+    for (const int t : h.block_tile) {
+      if (t < -1 || t >= n_cell_blocks_ || (t >= 0 && seen[t]++)) return "layout: block tile table";
+    }
+    for (const int c : seen)
+      if (c != 1) return "layout: block tile table misses a tile";
     for (int b = 0; b < n_cell_blocks_; ++b) {
       const MeshSlot& s = slots_[h.cell_block_mesh[b]];
       if (b * tile_ < s.cell_begin || b * tile_ >= s.cell_begin + s.cap.cells) return "layout: cell block range";
@@ -600,6 +637,7 @@ class DeviceArena final : public Arena {
     };
     col(h.desc, TABLE);
     col(h.cell_block_mesh, TABLE);
+    col(h.block_tile, TABLE);
     col(h.face_block_mesh, TABLE);
     for (auto* v : {&h.vol, &h.inv_vol, &h.dt_area, &h.wall_dist, &h.mem_inv}) col(*v, CELLS);
     for (auto* v : {&h.wall_first, &h.wall_cnt}) col(*v, CELLS);
@@ -648,6 +686,7 @@ class DeviceArena final : public Arena {
     const auto next = [&](auto* typed) { return reinterpret_cast<decltype(typed)>(columns_[i++].dev); };
     a.desc = next(static_cast<const MeshDesc<T>*>(nullptr));
     a.cell_block_mesh = next(static_cast<const int*>(nullptr));
+    a.block_tile = next(static_cast<const int*>(nullptr));
     a.face_block_mesh = next(static_cast<const int*>(nullptr));
     a.vol = next(static_cast<const T*>(nullptr));
     a.inv_vol = next(static_cast<const T*>(nullptr));
@@ -726,6 +765,7 @@ class DeviceArena final : public Arena {
     a.nw = nw;
     a.tile = tile_;
     a.n_cell_blocks = n_cell_blocks_;
+    a.n_grid_blocks = static_cast<int>(h.block_tile.size());
     a.n_face_blocks = n_face_blocks_;
     a.n_wall_blocks = n_wall_blocks_;
     a.any_laminar = features_.laminar;

@@ -52,12 +52,13 @@ __device__ __forceinline__ void flag_bad(Control* c, int bit) { atomicOr(&c->bad
 // other buffer (written by stage 0); each stage writes the buffer it does not read.
 __device__ __forceinline__ int read_buffer(const Control* c, int stage) { return stage == 1 ? c->wbuf ^ 1 : c->wbuf; }
 
-// Cell tile of this block.  The dispatcher hands consecutive blocks to the 8 XCDs in turn, each
-// with its own L2; this permutation gives every XCD one contiguous run of tiles, so the
-// neighbours a tile gathers mostly sit in the same L2.  A bijection on [0, gridDim.x).
-__device__ __forceinline__ int cell_tile() {
-  const int n = static_cast<int>(gridDim.x), b = static_cast<int>(blockIdx.x), per = n >> 3;
-  return b >= (per << 3) ? b : (b & 7) * per + (b >> 3);
+// Cell tile of this block, or -1.  The dispatcher hands consecutive blocks to the 8 XCDs in
+// turn, each with its own L2.  The host-built table splits every slot's tiles into 8 contiguous
+// runs, one per XCD: neighbouring tiles share an L2, and whichever slots are running, their
+// work is spread over all XCDs.
+template <class T>
+__device__ __forceinline__ int cell_tile(const ArenaPtrs<T>& a) {
+  return a.block_tile[blockIdx.x];
 }
 
 // ---- cell records ---------------------------------------------------------------------------
@@ -277,7 +278,8 @@ __device__ __forceinline__ void publish_dt(Control* c, double local) {
 // ------------------------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_base_prim(ArenaPtrs<T> a) {
-  const int tile = cell_tile();
+  const int tile = cell_tile(a);
+  if (tile < 0) return;
   const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
   if (!pass_on(c, PASS_BASE) || c->w_valid) return;
@@ -298,7 +300,8 @@ __global__ __launch_bounds__(kMaxTile) void k_base_prim(ArenaPtrs<T> a) {
 // half.  STAGE 0 = base pass, 1 and 2 = RK stages.
 template <class T, int STAGE>
 __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
-  const int tile = cell_tile();
+  const int tile = cell_tile(a);
+  if (tile < 0) return;
   const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
   if (!pass_on(c, STAGE == 0 ? PASS_BASE : PASS_STAGE)) return;
@@ -407,14 +410,19 @@ __device__ __forceinline__ FaceFlux<T> face_flux(const MeshDesc<T>& d, int4 fi, 
   Prim<T> wr{R.rho, R.u, R.v, R.p};
   int kind = -1;
   if (side >= 0) {
-    wl = Prim<T>{in.rho, in.u, in.v, in.p};
-    wr = wl;
+    // One call site for both orientations: the ghost state goes to the side without a cell.
+    const Prim<T> inside{in.rho, in.u, in.v, in.p};
+    Prim<T> ghost = inside;
     kind = d.side[side].kind;
-    const bool ok = l < 0 ? side_state(d.side[side], d.gamma, side, wr, wl) : side_state(d.side[side], d.gamma, side, wl, wr);
-    if (!ok) {
+    if (!side_state(d.side[side], d.gamma, side, inside, ghost)) {
       o.ok = false;
       return o;
     }
+    const bool ghost_left = l < 0;  // field-wise selects keep both states in registers
+    wl = Prim<T>{ghost_left ? ghost.r : inside.r, ghost_left ? ghost.u : inside.u, ghost_left ? ghost.v : inside.v,
+                 ghost_left ? ghost.p : inside.p};
+    wr = Prim<T>{ghost_left ? inside.r : ghost.r, ghost_left ? inside.u : ghost.u, ghost_left ? inside.v : ghost.v,
+                 ghost_left ? inside.p : ghost.p};
   }
   // Face frame: n = normal velocity, t = tangential velocity.
   const T nL = xaxis ? wl.u : wl.v, tL = xaxis ? wl.v : wl.u;
@@ -551,7 +559,8 @@ __device__ __forceinline__ void wall_shear(const CellPrim<T>& donor, T nx, T ny,
 template <class T, int STAGE>
 __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
   constexpr bool BASE = STAGE == 0;
-  const int tile = cell_tile();
+  const int tile = cell_tile(a);
+  if (tile < 0) return;
   const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
   if (!pass_on(c, BASE ? PASS_BASE : PASS_STAGE)) return;
@@ -741,7 +750,8 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
 // (reference diffusion_group_kernel and choose_dt SRD branch, src/static_amr2d_cuda.cu:164-176).
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_diffusion_group(ArenaPtrs<T> a) {
-  const int tile = cell_tile();
+  const int tile = cell_tile(a);
+  if (tile < 0) return;
   const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
   if (!pass_on(c, PASS_BASE)) return;
@@ -756,7 +766,8 @@ __global__ __launch_bounds__(kMaxTile) void k_diffusion_group(ArenaPtrs<T> a) {
 
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_dt_group(ArenaPtrs<T> a) {
-  const int tile = cell_tile();
+  const int tile = cell_tile(a);
+  if (tile < 0) return;
   const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
   if (!pass_on(c, PASS_BASE)) return;
@@ -789,7 +800,8 @@ __device__ __forceinline__ bool choose_dt(const Control* c, double fixed, double
 
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
-  const int tile = cell_tile();
+  const int tile = cell_tile(a);
+  if (tile < 0) return;
   const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
   if (!running(c)) return;
@@ -825,7 +837,8 @@ __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
 // `stage` selects the candidate: 0 and 2 write U[cur ^ 1], 1 writes U2.
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_group(ArenaPtrs<T> a, int stage) {
-  const int tile = cell_tile();
+  const int tile = cell_tile(a);
+  if (tile < 0) return;
   const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
   if (!healthy(c)) return;
@@ -852,7 +865,8 @@ __global__ __launch_bounds__(kMaxTile) void k_group(ArenaPtrs<T> a, int stage) {
 
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_redist(ArenaPtrs<T> a, int stage) {
-  const int tile = cell_tile();
+  const int tile = cell_tile(a);
+  if (tile < 0) return;
   const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
   if (!healthy(c)) return;
@@ -963,7 +977,8 @@ __device__ __forceinline__ void block_sum(double (&v)[N]) {
 // reference src/quick_airfoil.cpp:76-81).
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_monitor_cells(ArenaPtrs<T> a) {
-  const int tile = cell_tile();
+  const int tile = cell_tile(a);
+  if (tile < 0) return;
   const int m = a.cell_block_mesh[tile];
   const MeshDesc<T>& d = a.desc[m];
   const int k = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
@@ -1031,7 +1046,7 @@ inline int blocks_of(int n, int tile) { return (n + tile - 1) / tile; }
 
 template <class T>
 int launch_residual(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
-  const dim3 tile(a.tile), cells(a.n_cell_blocks);
+  const dim3 tile(a.tile), cells(a.n_grid_blocks);
   const size_t base_lds = flux_lds_bytes<T>(a.tile, true), vel_lds = grad_lds_bytes<T>(a.tile);
   int launches = 0;
   HIPAMR_LAUNCH("base_primitive", k_base_prim<T>, cells, a);
@@ -1046,7 +1061,7 @@ int launch_residual(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
 
 template <class T>
 int launch_trial(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
-  const dim3 tile(a.tile), cells(a.n_cell_blocks);
+  const dim3 tile(a.tile), cells(a.n_grid_blocks);
   const size_t stage_lds = flux_lds_bytes<T>(a.tile, false), vel_lds = grad_lds_bytes<T>(a.tile);
   int launches = launch_residual(a, s, hook);
   HIPAMR_LAUNCH("stage0", k_stage0<T>, cells, a);
@@ -1077,7 +1092,7 @@ int launch_trial(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
 
 template <class T>
 void launch_monitor(const ArenaPtrs<T>& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_monitor_cells<T>, dim3(a.n_cell_blocks), dim3(a.tile), 0, s, a);
+  hipLaunchKernelGGL(k_monitor_cells<T>, dim3(a.n_grid_blocks), dim3(a.tile), 0, s, a);
   if (a.n_wall_blocks > 0) hipLaunchKernelGGL(k_monitor_walls<T>, dim3(a.n_wall_blocks), dim3(a.tile), 0, s, a);
 }
 

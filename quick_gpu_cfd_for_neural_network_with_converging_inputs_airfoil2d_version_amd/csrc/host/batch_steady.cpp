@@ -129,7 +129,7 @@ int amr_driver_poll(amr_batch_driver* d, double wall_budget_seconds, amr_steady_
     if (wall_budget_seconds > 0) budget = std::min(budget, wall_budget_seconds - seconds_since(t0));
     if (amr_batch_set_targets(d->batch, targets.data(), maxsteps.data(), err, err_len)) return 1;
     const int status_advance =
-        amr_batch_advance(d->batch, 1, std::max(.001, budget), d->states.data(), message, sizeof(message));
+        amr_batch_advance(d->batch, 2, std::max(.001, budget), d->states.data(), message, sizeof(message));
     if (status_advance == 1) return fail(err, err_len, message);
     for (int m = 0; m < n; ++m) d->stepped[m] |= d->runs[m] != nullptr;
     // Which solves are due: failed ones end, finished or out-of-budget ones are checked.
