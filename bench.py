@@ -33,6 +33,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 from concurrent.futures import ThreadPoolExecutor
@@ -49,6 +50,19 @@ PKG = "quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+STAGE_PASS_PROFILE = Path(__file__).resolve().parent / "profiles" / "stage_pass_lf2x256_fp32.md"
+
+
+def profiled_traffic(batch: int, tile: int, seed: int):
+    """HBM bytes of one stage pass (gradient_stage + flux_rk1) of this workload as measured by
+    rocprofv3 (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a copy kernel of known size),
+    read from the committed profile summary; None when the summary covers another workload."""
+    if (batch, tile, seed) != (256, 256, 2026) or not STAGE_PASS_PROFILE.exists():
+        return None
+    m = re.search(r"^stage_pass_hbm_bytes: (\d+)$", STAGE_PASS_PROFILE.read_text(), re.M)
+    return int(m.group(1)) if m else None
 
 
 def draw_airfoils(n: int, seed: int) -> list[dict]:
@@ -350,7 +364,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": RL.HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": round(achieved / RL.HBM_PEAK_GBPS, 4), "traffic": None,
+            "frac": round(achieved / RL.HBM_PEAK_GBPS, 4), "traffic": profiled_traffic(args.batch, tile, args.seed),
             "kernel": "stage pass of the batch: gradient_stage (k_grad) + flux_rk1 (k_flux, faces + gather + RK)",
             "bytes_per_launch": int(b_design), "contract_bytes_per_launch": int(b_contract),
             "us_per_launch": round(stage_seconds * 1e6, 2), "stream_copy_gbps": round(gbps.value, 1),
