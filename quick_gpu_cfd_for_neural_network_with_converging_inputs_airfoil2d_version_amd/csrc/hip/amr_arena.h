@@ -123,6 +123,7 @@ struct ArenaPtrs {
   // down to the steady tolerance); residuals and all work arrays are T.
   double* U[2];
   double* U2;
+  T* DU;         // nv planes: RK stage increments over U[cur] (FP32 work precision without SRD)
   double* QHAT;  // SRD group averages, 4 planes
   T* R0;
   // Cell vectors gathered by faces and gradient stencils (AoS per purpose, so that a gather

@@ -52,6 +52,16 @@ __device__ __forceinline__ void flag_bad(Control* c, int bit) { atomicOr(&c->bad
 // other buffer (written by stage 0); each stage writes the buffer it does not read.
 __device__ __forceinline__ int read_buffer(const Control* c, int stage) { return stage == 1 ? c->wbuf ^ 1 : c->wbuf; }
 
+// FP32 work precision, mesh without SRD: the RK stage states U1, U2 are kept as T increments
+// over the FP64 state (U1 = U + D1, U2 = U + D2), half the bytes of FP64 copies.  The
+// increments are O(dt r), so their FP32 rounding is far below the FP32 flux error; the state and
+// the final combination stay FP64.  FP64 work precision and SRD meshes use FP64 stage states.
+// Decided per mesh, so a slot computes the same bits whatever else shares its arena.
+template <class T>
+__device__ __forceinline__ bool stage_deltas(const MeshDesc<T>& d) {
+  return sizeof(T) == 4 && !d.small;
+}
+
 // Cell tile of this block, or -1.  The dispatcher hands consecutive blocks to the 8 XCDs in
 // turn, each with its own L2.  The host-built table splits every slot's tiles into 8 contiguous
 // runs, one per XCD: neighbouring tiles share an L2, and whichever slots are running, their
@@ -227,9 +237,10 @@ __device__ __forceinline__ FaceItem<double> load_item(const double* p) {
 // U -> record primitives (computed in FP64 from the FP64 state, stored as T), plus molecular
 // and eddy viscosity for viscous meshes; false if not admissible (reference primitive rules,
 // src/static_amr2d.cpp:143-154, and a valid viscosity).
+// q: rho, rho u, rho v, energy, rho nutilde (the last only with SA).
 template <class T>
-__device__ __forceinline__ bool make_prim(const double* u, int nc, int k, const MeshDesc<T>& d, CellPrim<T>& w) {
-  const double rho = u[k], mx = u[nc + k], my = u[2 * nc + k], energy = u[3 * nc + k];
+__device__ __forceinline__ bool make_prim_q(const double (&q)[5], const MeshDesc<T>& d, CellPrim<T>& w) {
+  const double rho = q[0], mx = q[1], my = q[2], energy = q[3];
   const double ir = 1.0 / rho;
   const double vx = mx * ir, vy = my * ir;
   const double p = (static_cast<double>(d.gamma) - 1.0) * (energy - 0.5 * (mx * vx + my * vy));
@@ -240,7 +251,7 @@ __device__ __forceinline__ bool make_prim(const double* u, int nc, int k, const 
   w.p = static_cast<T>(p);
   w.phi = T(0);
   if (d.sa) {
-    const double ph = u[4 * nc + k] * ir;
+    const double ph = q[4] * ir;
     w.phi = static_cast<T>(ph);
     ok = ok && isfinite(ph);
   }
@@ -254,6 +265,11 @@ __device__ __forceinline__ bool make_prim(const double* u, int nc, int k, const 
     if (d.sa) w.eddy = w.rho * sa_eddy(w.phi, mu / w.rho);
   }
   return ok;
+}
+template <class T>
+__device__ __forceinline__ bool make_prim(const double* u, int nc, int k, const MeshDesc<T>& d, CellPrim<T>& w) {
+  const double q[5] = {u[k], u[nc + k], u[2 * nc + k], u[3 * nc + k], d.sa ? u[4 * nc + k] : 0.0};
+  return make_prim_q(q, d, w);
 }
 
 // Minimum over the block of a double (wave64 xor-shuffles, then one LDS slot per wave).
@@ -723,6 +739,29 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
         if (rate > T(0)) dt = fmin(dt, d.transport.sa_cfl / rate);
       }
       local_dt = static_cast<double>(dt);
+    } else if (stage_deltas(d)) {
+      // SSP-RK3 on increments: U2 = U + D2 with D2 = (D1 + dt r) / 4, then
+      // U' = U + 2/3 (D2 + dt r), the same combinations as below.
+      const double dt = c->trial_dt;
+      const int cur = c->cur;
+      const double* u = a.U[cur];
+      const T rv[5] = {r0, r1, r2, r3, r4};
+      double q[5] = {0, 0, 0, 0, 0};
+      for (int v = 0; v < d.nv; ++v) {
+        const int i = v * nc + k;
+        const double inc = static_cast<double>(a.DU[i]) + dt * static_cast<double>(rv[v]);
+        if (STAGE == 1) {
+          const T dn = static_cast<T>(0.25 * inc);
+          a.DU[i] = dn;
+          q[v] = u[i] + static_cast<double>(dn);
+        } else {
+          q[v] = u[i] + (2.0 / 3.0) * inc;
+          a.U[cur ^ 1][i] = q[v];
+        }
+      }
+      CellPrim<T> w;
+      if (!make_prim_q(q, d, w)) flag_bad(c, BAD_STAGE);
+      else store_prim(a, rb ^ 1, k, w);
     } else {
       // SSP-RK3: u2 = 3/4 u + 1/4 (u1 + dt r)  or  u1 = 1/3 u + 2/3 (u2 + dt r).
       const double dt = c->trial_dt;
@@ -824,6 +863,18 @@ __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
   if (k >= d.cell_begin + d.n_cells) return;
   const int nc = a.nc, cur = c->cur;
   const double* u = a.U[cur];
+  if (stage_deltas(d)) {  // U1 = U + D1, D1 = dt R0 kept in T
+    double q[5] = {0, 0, 0, 0, 0};
+    for (int v = 0; v < d.nv; ++v) {
+      const T dv = static_cast<T>(dt * static_cast<double>(a.R0[v * nc + k]));
+      a.DU[v * nc + k] = dv;
+      q[v] = u[v * nc + k] + static_cast<double>(dv);
+    }
+    CellPrim<T> w;
+    if (!make_prim_q(q, d, w)) flag_bad(c, BAD_STAGE);
+    else store_prim(a, c->wbuf ^ 1, k, w);
+    return;
+  }
   double* out = a.U[cur ^ 1];
   for (int v = 0; v < d.nv; ++v) out[v * nc + k] = u[v * nc + k] + dt * static_cast<double>(a.R0[v * nc + k]);
   if (!d.small) {
