@@ -3,7 +3,8 @@
 The counterpart of the reference's `quick_cfd.py run` (quick_cfd.py:219-231, settings from
 pipeline/configuration.py:143-167): it prepares the polygon and patch files, writes the same
 `settings.txt` keys (plus `model` and `precision`), runs the solver executable of the chosen
-backend once per mesh level and reads back `result.json` / `gpu.json`.
+backend once per mesh level, reads back `result.json` / `gpu.json` and writes the SI exports
+(export.py) of every viscous level into LEVEL/output.
 
   python -m quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd.cli run \\
       --points G0001_airfoil_points.csv --levels LF1 LF2 --backend hip --out runs/G0001
@@ -20,7 +21,10 @@ import subprocess
 import sys
 from pathlib import Path
 
+import numpy as np
+
 from . import config as CF
+from . import export as EX
 from . import geometry as G
 
 PACKAGE_DIR = Path(__file__).resolve().parent
@@ -117,6 +121,10 @@ def run(args) -> dict:
         result = json.loads(Path(f"{prefix}.result.json").read_text()) if Path(f"{prefix}.result.json").exists() else {}
         gpu = json.loads(Path(f"{prefix}.gpu.json").read_text()) if Path(f"{prefix}.gpu.json").exists() else {}
         summary["levels"][preset] = dict(returncode=done.returncode, result=result, gpu=gpu)
+        # SI exports (reference quick_cfd.py:110): viscous runs that wrote a final state.
+        if done.returncode == 0 and Path(f"{prefix}.final.viscous_loads.json").exists():
+            polygon = np.loadtxt(geometry, delimiter=",", skiprows=1)
+            summary["levels"][preset]["export"] = EX.export_case(prefix, folder / "output", c, polygon)
         print(f"[run] {preset}: rc={done.returncode} {result.get('stop_reason')} steps={result.get('steps')} "
               f"Cl={result.get('Cl')} Cd={result.get('Cd')} Cm={result.get('Cm')}", file=sys.stderr, flush=True)
     (out / "summary.json").write_text(json.dumps(summary, indent=2) + "\n")

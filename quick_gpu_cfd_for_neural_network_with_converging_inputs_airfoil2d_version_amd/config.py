@@ -28,7 +28,16 @@ DEFAULTS = {
                                "residual_region_xy_c": [-1.0, 3.0, -1.0, 1.0]}},
     "mesh": {"bounds_xy_c": [-7.0, 14.0, -10.0, 10.0], "base_shape": [42, 40], "wall_buffer": 4,
              "max_cells": 2000000, "agglomerate_small": True},
+    "output": {"fields": ["x", "y", "p", "rho", "u", "v", "mach", "temperature"], "formats": ["npz", "csv"],
+               "region_xy_c": None, "sample_points_xy_c": [], "surface": True, "surface_patches": [1, 2, 3],
+               "loads": True, "volume": True, "load_fields": None},
 }
+
+# Exported load columns (reference pipeline/configuration.py:9-10).
+DEFAULT_LOAD_FIELDS = ["drag", "lift", "fx", "fy", "pressure_force_x", "pressure_force_y", "pressure_drag",
+                       "pressure_lift", "viscous_force_x", "viscous_force_y", "viscous_drag", "viscous_lift",
+                       "moment_z", "pressure_moment_z", "viscous_moment_z", "CD", "CL", "CM_z"]
+DEFAULTS["output"]["load_fields"] = list(DEFAULT_LOAD_FIELDS)
 
 # Mesh presets.  LF1/LF2 are the reference's (config/default.json "profiles"); L7/L8 refine the
 # wall by one/two more levels with the LF2 boxes raised by the same amount.

@@ -74,6 +74,13 @@ bool write_mesh_files(const StaticAmrMesh2D& m, const Params2D& p, const std::st
       !write_npy(prefix + ".faces.npy", faces.data(), m.faces.size(), 9) ||
       !write_npy(prefix + ".walls.npy", walls.data(), m.walls.size(), 11))
     return false;
+  // Agglomeration members (old id, new id, x0, x1, y0, y1, V, cx, cy) of an agglomerated mesh
+  // (reference src/static_amr2d_binary.cpp:38-39).
+  if (!m.agglomeration.empty()) {
+    std::vector<double> members;
+    for (const auto& row : m.agglomeration) members.insert(members.end(), row.begin(), row.end());
+    if (!write_npy(prefix + ".agglomeration.npy", members.data(), m.agglomeration.size(), 9)) return false;
+  }
   const StaticAmrOptions2D& o = m.options;
   const bool sa = m.transport.sa_model != 0;
   std::ofstream out(prefix + ".mesh.json");
