@@ -28,7 +28,7 @@ struct Control {
   int cur;                   // buffer holding the accepted state: U[cur]
   int w_valid;               // W/PHI hold primitives of U[cur]
   int r0_valid;              // R0 holds L(U[cur]) (set after a rejection)
-  int wbuf;                  // cell-record buffer holding the primitives of U[cur]: CREC[wbuf]
+  int wbuf;                  // primitive-vector buffer holding the primitives of U[cur]: VREC/PREC[wbuf]
 };
 static_assert(sizeof(Control) == 104, "Control layout");
 

@@ -4,20 +4,21 @@
 //
 // One trial (reference launch_trial, src/static_amr2d_cuda.cu:241-268) is, per RHS pass:
 //   k_grad   cells: LSQ velocity gradient, SA-neg gradients and point model (reference
-//            :71-91, src/static_amr2d_sa_cuda.cuh:2-11) into the gradient half of the record
-//   k_flux   cells: for every face of the cell the boundary state, HLLC / AUSM+-up, viscous
-//            and SA face flux (reference :92-143, src/static_amr2d_sa_cuda.cuh:12-29), the
-//            no-slip cut-wall shear of its wall fragments, the gather r = -(sum F + p n l -
-//            wall shear)/V (reference :144-163); the base pass also forms the local CFL /
-//            viscous / SA step and reduces it (wave shuffles + LDS + one 64-bit atomicMin per
-//            block), stage passes apply the SSP-RK3 combination in FP64 (reference :188-194)
-//            and write the candidate's primitives, viscosity and eddy viscosity.
-// A face shared by two cells is evaluated by both with the same inputs and operations, so both
+//            :71-91, src/static_amr2d_sa_cuda.cuh:2-11) into the gradient vector GREC
+//   k_flux   tiles: (A) each face touching the tile once (boundary state, HLLC / AUSM+-up,
+//            viscous and SA face flux, reference :92-143, src/static_amr2d_sa_cuda.cuh:12-29)
+//            into an LDS slot; (B) per cell the sum of its faces' slots, the no-slip cut-wall
+//            shear of its wall fragments and r = -(sum F + p n l - wall shear)/V (reference
+//            :144-163); the base pass also forms the local CFL / viscous / SA step and reduces
+//            it (wave shuffles + LDS + one 64-bit atomicMin per block), stage passes apply the
+//            SSP-RK3 combination (reference :188-194) and write the candidate's primitives,
+//            viscosity and eddy viscosity.
+// A face shared by two tiles is evaluated by both with the same inputs and operations, so both
 // see the same flux bits; no flux array is written or re-read.  Cell data that faces and
 // gradient stencils gather lives in 4- and 8-wide vectors per cell (AoS per purpose), so one
-// neighbour costs whole 16-byte loads instead of one cache line per variable.  Primitive vectors
-// are double-buffered by stage: a stage reads one buffer and writes the next state's into the
-// other.
+// neighbour costs whole 16-byte loads; each block stages its own tile's vectors in LDS.
+// Primitive vectors are double-buffered by stage: a stage reads one buffer and writes the next
+// state's into the other.
 // Plus k_base_prim, k_stage0 (dt choice + first stage, reference :170-191), the SRD pair
 // k_group / k_redist (reference :195-207), the SRD-grouped viscous limit (reference :164-176)
 // and k_commit (accept / reject / begin, reference :54-59,212-216).
@@ -234,7 +235,7 @@ __device__ __forceinline__ FaceItem<double> load_item(const double* p) {
   return it;
 }
 
-// U -> record primitives (computed in FP64 from the FP64 state, stored as T), plus molecular
+// U -> cell primitives (computed in FP64 from the FP64 state, stored as T), plus molecular
 // and eddy viscosity for viscous meshes; false if not admissible (reference primitive rules,
 // src/static_amr2d.cpp:143-154, and a valid viscosity).
 // q: rho, rho u, rho v, energy, rho nutilde (the last only with SA).
@@ -312,7 +313,7 @@ __global__ __launch_bounds__(kMaxTile) void k_base_prim(ArenaPtrs<T> a) {
 }
 
 // ------------------------------------------------------------------------------------------
-// LSQ gradients of the stage's state (and the SA-neg point source) into the record's gradient
+// LSQ gradients of the stage's state (and the SA-neg point source) into the gradient vector GREC
 // half.  STAGE 0 = base pass, 1 and 2 = RK stages.
 template <class T, int STAGE>
 __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
