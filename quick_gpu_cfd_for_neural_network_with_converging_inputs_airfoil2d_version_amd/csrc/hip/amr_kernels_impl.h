@@ -313,8 +313,8 @@ __global__ __launch_bounds__(kMaxTile) void k_base_prim(ArenaPtrs<T> a) {
 }
 
 // ------------------------------------------------------------------------------------------
-// LSQ gradients of the stage's state (and the SA-neg point source) into the gradient vector GREC
-// half.  STAGE 0 = base pass, 1 and 2 = RK stages.
+// LSQ gradients of the stage's state (and the SA-neg point source) into the gradient vector
+// GREC.  STAGE 0 = base pass, 1 and 2 = RK stages.
 template <class T, int STAGE>
 __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
   const int tile = cell_tile(a);
