@@ -129,12 +129,11 @@ struct ArenaPtrs {
   // Cell vectors gathered by faces and gradient stencils (AoS per purpose, so that a gather
   // costs whole 16-byte loads): VREC = rho, u, v, nutilde and PREC = p, mu, eddy viscosity, -
   // (double-buffered by stage: a stage reads one buffer and writes the next state's into the
-  // other), GREC = ux, uy, vx, vy, nutilde_x, nutilde_y, SA diffusion, - (per stage).
+  // other), GREC = ux, uy, vx, vy, nutilde_x, nutilde_y, SA diffusion, SA source (per stage).
   T* VREC[2];
   T* PREC[2];
   T* GREC;
-  T* SSRC;    // SA-neg source (own cell only)
-  T* SRATE;   // SA-neg source rate
+  T* SRATE;   // SA-neg source rate (base pass; the source itself rides in GREC's 8th slot)
   T* CELL_DT; // deferred local dt (SRD-grouped viscous limit only)
   T* DRATE;
   T* DGROUP;

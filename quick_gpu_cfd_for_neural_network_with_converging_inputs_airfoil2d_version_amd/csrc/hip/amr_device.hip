@@ -152,9 +152,9 @@ SlabPlan plan_slab(const Totals& t, int nv) {
   p.mon_cell = arr(6 * sizeof(double), t.cell_blocks);
   p.mon_wall = arr(3 * sizeof(double), t.wall_blocks);
   // FP64 state U[2], U2 (nv planes each) and QHAT (4); cell vectors VREC[2], PREC[2] (4 T)
-  // and GREC (8 T); T planes R0 (nv), SSRC, SRATE, CELL_DT, DRATE, DGROUP, and with FP32 work
+  // and GREC (8 T); T planes R0 (nv), SRATE, CELL_DT, DRATE, DGROUP, and with FP32 work
   // precision the RK stage increments DU (nv); monitor partials.
-  const size_t state_planes = 3 * nv + 4, cell_planes = nv + 5 + (sizeof(T) == 4 ? nv : 0);
+  const size_t state_planes = 3 * nv + 4, cell_planes = nv + 4 + (sizeof(T) == 4 ? nv : 0);
   p.work = state_planes * p.dplane + 4 * p.vec4 + p.vec8 + cell_planes * p.cplane + p.mon_cell + p.mon_wall;
   p.total = p.geometry + p.control + p.work;
   return p;
@@ -750,7 +750,6 @@ class DeviceArena final : public Arena {
     a.PREC[1] = plane(vec4, 1);
     a.GREC = plane(vec8, 1);
     a.R0 = plane(cplane, nv);
-    a.SSRC = plane(cplane, 1);
     a.SRATE = plane(cplane, 1);
     a.CELL_DT = plane(cplane, 1);
     a.DRATE = plane(cplane, 1);

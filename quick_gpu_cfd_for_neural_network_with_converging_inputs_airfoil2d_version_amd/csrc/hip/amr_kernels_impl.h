@@ -132,9 +132,10 @@ __device__ __forceinline__ void store_prim(const ArenaPtrs<T>& a, int buf, int k
   store4(a.PREC[buf] + 4 * k, w.p, w.mu, w.eddy, T(0));
 }
 template <class T>
-__device__ __forceinline__ void store_grad(const ArenaPtrs<T>& a, int k, const CellGrad<T>& g) {
+// The gradient vector's 8th slot carries the cell's SA-neg source (read by the cell itself only).
+__device__ __forceinline__ void store_grad(const ArenaPtrs<T>& a, int k, const CellGrad<T>& g, T sa_source) {
   store4(a.GREC + 8 * k, g.ux, g.uy, g.vx, g.vy);
-  store4(a.GREC + 8 * k + 4, g.sx, g.sy, g.sdiff, T(0));
+  store4(a.GREC + 8 * k + 4, g.sx, g.sy, g.sdiff, sa_source);
 }
 
 // Cell vectors of the block's own tile staged in LDS: a neighbour inside the tile (most of them,
@@ -375,17 +376,18 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
     }
   }
   CellGrad<T> g{g0, g1, g2, g3, sx, sy, T(0)};
+  T sa_source = T(0);
   if (sa) {
     const SaPoint<T> pt = sa_point(phi, own.mu / rho, a.wall_dist[k], fabs(g2 - g1));
     const T source = rho * (pt.source + T(SaConst<T>::cb2 / SaConst<T>::sigma) * (sx * sx + sy * sy)) -
                      pt.diffusion * (rx * sx + ry * sy);
     g.sdiff = pt.diffusion;
-    a.SSRC[k] = source;
-    a.SRATE[k] = pt.source_rate;
+    sa_source = source;
+    if (STAGE == 0) a.SRATE[k] = pt.source_rate;  // the local dt of the base pass only
     if (!(pt.diffusion > T(0)) || !isfinite(source) || !isfinite(pt.source_rate))
       flag_bad(c, STAGE == 0 ? BAD_BASE : BAD_STAGE);
   }
-  store_grad(a, k, g);
+  store_grad(a, k, g, sa_source);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -714,7 +716,7 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
     const T r0 = -s0 * iv, r1 = -s1 * iv, r2 = -s2 * iv, r3 = -s3 * iv;
     T r4 = T(0);
     if (sa) {
-      r4 = -sa_net * iv + a.SSRC[k];
+      r4 = -sa_net * iv + a.GREC[8 * k + 7];
       if (!isfinite(r4)) flag_bad(c, bit);
     }
     if (BASE) {
