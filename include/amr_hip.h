@@ -295,8 +295,8 @@ int amr_batch_set_targets(amr_batch* b, const double* targets, const int64_t* ma
                           size_t err_len);
 /* Replay trials until every slot is finished or failed (until = 0), until at least one slot
  * that was running at the call has finished or failed (until = 1), or, grouped (until = 2),
- * until an eighth of those slots have ended or eight polls have passed since the first ended,
- * polling the control records every batch_trials trials.  Ended slots wait idle, so grouping
+ * until an eighth of those slots have ended or min(8, running / 32) polls have passed since
+ * the first ended, polling the control records every batch_trials trials.  Ended slots wait idle, so grouping
  * trades a short wait for fewer checks.  states (may be NULL) receives every slot.  0 done,
  * 2 wall budget expired (0 = no budget), 1 device error. */
 int amr_batch_advance(amr_batch* b, int32_t until, double wall_budget_seconds, amr_slot_state* states, char* err,

@@ -152,8 +152,8 @@ class Batch:
 
     def advance(self, until: str = "all", budget: float = 0.0) -> int:
         """Replay trials until all slots finish ("all"), any running slot ends ("any"), or an eighth
-        of the running slots have ended / eight polls have passed since the first ended ("group");
-        0 done, 2 budget expired."""
+        of the running slots have ended / min(8, running // 32) polls have passed since the first
+        ended ("group"); 0 done, 2 budget expired."""
         err = N.err_buffer()
         mode = {"all": 0, "any": 1, "group": 2}[until]
         return self._check(self._lib.amr_batch_advance(self.handle, mode, budget, self._state_buf, err, N.ERR_LEN),

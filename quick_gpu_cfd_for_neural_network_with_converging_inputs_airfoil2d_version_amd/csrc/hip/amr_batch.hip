@@ -278,9 +278,11 @@ int amr_batch_advance(amr_batch* b, int32_t until, double wall_budget_seconds, a
   }
   bool expired = false;
   // until = 2: once a slot has ended, keep going until an eighth of the running slots have
-  // ended or kGroupPolls more polls have passed, so that one check serves several slots.
-  constexpr int kGroupPolls = 8;
+  // ended or a few more polls have passed, so that one check serves several slots.  The wait
+  // shrinks with the number of running slots (8 polls at 256 and more, none at 32 and fewer):
+  // a waiting slot is a larger share of the work when few are running.
   const int group = std::max(1, n_running / 8);
+  const int wait_polls = std::min(8, n_running / 32);
   int polls = 0, first_end = -1;
   while (n_running) {
     for (int trial = 0; trial < b->options.batch_trials; ++trial)
@@ -302,7 +304,7 @@ int amr_batch_advance(amr_batch* b, int32_t until, double wall_budget_seconds, a
     if (still == 0 || (until == 1 && ended > 0)) break;
     if (until == 2 && ended > 0) {
       if (first_end < 0) first_end = polls;
-      if (ended >= group || polls - first_end >= kGroupPolls) break;
+      if (ended >= group || polls - first_end >= wait_polls) break;
     }
     if (wall_budget_seconds > 0 && std::chrono::duration<double>(now - t0).count() >= wall_budget_seconds) {
       expired = true;
