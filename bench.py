@@ -175,7 +175,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2026)
-    ap.add_argument("--fp64-check", type=int, default=64, help="meshes re-solved in FP64 for dCL/dCD/dCM")
+    ap.add_argument("--fp64-check", type=int, default=32, help="meshes re-solved in FP64 for dCL/dCD/dCM")
     ap.add_argument("--budget", type=float, default=240.0, help="wall budget of each steady solve (s)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--workers", type=int, default=16)

@@ -170,3 +170,22 @@ def test_driver_matches_session_solve_bitwise():
         print(r["steps"], r["cl"], ref["steps"], ref["cl"], flush=True)
         for key in ("steps", "rejected", "time", "cl", "cd", "cm", "residual", "checks", "reason"):
             assert r[key] == ref[key], key
+
+
+@pytest.mark.parametrize("tile", [128, 256])
+def test_fp64_batch_at_large_tiles_matches_oracle(tile):
+    """The larger launch tiles (whose face-item chunks and XCD runs differ from tile 64's)
+    against the FP64 oracle over t* = 0.01 on an SA-neg and a laminar AUSM mesh."""
+    items = sa_meshes()[:1] + four_variable_meshes()[2:]
+    # One arena per conserved-variable count: SA-neg (5) and laminar (4).
+    for case, mesh, params, u0 in items:
+        with B.Batch.of_meshes([mesh], [params], [u0], precision=64, tile=tile) as b:
+            b.set_targets(0.01)
+            assert b.advance("all") == 0
+            u, meta = b.download(0)
+        ref, ometa = O.run(mesh, params, u0, 0.01)
+        nv = 5 if case.sa_model else 4
+        err = max(per_variable_l2(u, ref, nv))
+        print("tile", tile, "cells", mesh.n_cells, "steps", meta["steps"], ometa["steps"], "rel L2", err, flush=True)
+        assert meta["steps"] == ometa["steps"]
+        assert err <= 1e-10
