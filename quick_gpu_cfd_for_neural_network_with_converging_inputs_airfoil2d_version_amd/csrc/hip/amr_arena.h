@@ -28,7 +28,7 @@ struct Control {
   int cur;                   // buffer holding the accepted state: U[cur]
   int w_valid;               // W/PHI hold primitives of U[cur]
   int r0_valid;              // R0 holds L(U[cur]) (set after a rejection)
-  int wbuf;                  // primitive-vector buffer holding the primitives of U[cur]: VREC/PREC[wbuf]
+  int wbuf;                  // primitive-vector buffer (0..2) holding the primitives of U[cur]
 };
 static_assert(sizeof(Control) == 104, "Control layout");
 
@@ -128,10 +128,10 @@ struct ArenaPtrs {
   T* R0;
   // Cell vectors gathered by faces and gradient stencils (AoS per purpose, so that a gather
   // costs whole 16-byte loads): VREC = rho, u, v, nutilde and PREC = p, mu, eddy viscosity, -
-  // (double-buffered by stage: a stage reads one buffer and writes the next state's into the
-  // other), GREC = ux, uy, vx, vy, nutilde_x, nutilde_y, SA diffusion, SA source (per stage).
-  T* VREC[2];
-  T* PREC[2];
+  // (three buffers rotated by stage, Control::wbuf holding U[cur]'s), GREC = ux, uy, vx, vy,
+  // nutilde_x, nutilde_y, SA diffusion, SA source (per stage).
+  T* VREC[3];
+  T* PREC[3];
   T* GREC;
   T* SRATE;   // SA-neg source rate (base pass; the source itself rides in GREC's 8th slot)
   T* CELL_DT; // deferred local dt (SRD-grouped viscous limit only)
@@ -154,6 +154,10 @@ int launch_trial(const ArenaPtrs<T>& a, hipStream_t stream, LaunchHook* hook = n
 // Base pass only: primitives of U[cur] and R0 = L(U[cur]) (the residual evaluation).
 template <class T>
 int launch_residual(const ArenaPtrs<T>& a, hipStream_t stream, LaunchHook* hook = nullptr);
+// Primitives of U[cur] for every running mesh whose base primitives are not valid (after the
+// host wrote its state): must follow every launch_target before trials are launched.
+template <class T>
+int launch_prims(const ArenaPtrs<T>& a, hipStream_t stream, LaunchHook* hook = nullptr);
 // Steady-monitor partial sums of the residual R0 (cells whose centroid lies in the mesh's
 // region) and of the wall pressure/shear force and moment about the mesh's moment point, both
 // taken from mon_frame.

@@ -259,6 +259,7 @@ int amr_batch_set_targets(amr_batch* b, const double* targets, const int64_t* ma
     return 1;
   hipamr::launch_target(b->arena->control(), b->n, 0.0, 0, b->arena->target_array(), b->arena->maxsteps_array(),
                         b->stream);
+  b->arena->launch_prims(b->stream);  // base primitives of slots whose state the host wrote
   if (!checked(hipGetLastError(), err, err_len, "AMR target kernel")) return 1;
   return poll(b, err, err_len);
 }

@@ -63,6 +63,9 @@ class Arena {
   virtual uint64_t state_bytes() const = 0;
   virtual int launch_trial(hipStream_t s, LaunchHook* hook = nullptr) = 0;
   virtual int launch_residual(hipStream_t s, LaunchHook* hook = nullptr) = 0;
+  // Base primitives of every running slot whose host-written state has none yet; follows every
+  // launch_target before trials are launched.
+  virtual int launch_prims(hipStream_t s) = 0;
   virtual Control* control() = 0;
   // Device arrays of per-slot targets / step caps used by launch_target.
   virtual double* target_array() = 0;

@@ -151,11 +151,11 @@ SlabPlan plan_slab(const Totals& t, int nv) {
   p.vec8 = arr(8 * sizeof(T), t.cells);
   p.mon_cell = arr(6 * sizeof(double), t.cell_blocks);
   p.mon_wall = arr(3 * sizeof(double), t.wall_blocks);
-  // FP64 state U[2], U2 (nv planes each) and QHAT (4); cell vectors VREC[2], PREC[2] (4 T)
+  // FP64 state U[2], U2 (nv planes each) and QHAT (4); cell vectors VREC[3], PREC[3] (4 T)
   // and GREC (8 T); T planes R0 (nv), SRATE, CELL_DT, DRATE, DGROUP, and with FP32 work
   // precision the RK stage increments DU (nv); monitor partials.
   const size_t state_planes = 3 * nv + 4, cell_planes = nv + 4 + (sizeof(T) == 4 ? nv : 0);
-  p.work = state_planes * p.dplane + 4 * p.vec4 + p.vec8 + cell_planes * p.cplane + p.mon_cell + p.mon_wall;
+  p.work = state_planes * p.dplane + 6 * p.vec4 + p.vec8 + cell_planes * p.cplane + p.mon_cell + p.mon_wall;
   p.total = p.geometry + p.control + p.work;
   return p;
 }
@@ -744,10 +744,8 @@ class DeviceArena final : public Arena {
     a.U[1] = dplanes(nv);
     a.U2 = dplanes(nv);
     a.QHAT = dplanes(4);
-    a.VREC[0] = plane(vec4, 1);
-    a.VREC[1] = plane(vec4, 1);
-    a.PREC[0] = plane(vec4, 1);
-    a.PREC[1] = plane(vec4, 1);
+    for (int b = 0; b < 3; ++b) a.VREC[b] = plane(vec4, 1);
+    for (int b = 0; b < 3; ++b) a.PREC[b] = plane(vec4, 1);
     a.GREC = plane(vec8, 1);
     a.R0 = plane(cplane, nv);
     a.SRATE = plane(cplane, 1);
@@ -852,6 +850,7 @@ class DeviceArena final : public Arena {
   uint64_t state_bytes() const override { return state_bytes_; }
   int launch_trial(hipStream_t s, LaunchHook* hook) override { return hipamr::launch_trial<T>(ptrs_, s, hook); }
   int launch_residual(hipStream_t s, LaunchHook* hook) override { return hipamr::launch_residual<T>(ptrs_, s, hook); }
+  int launch_prims(hipStream_t s) override { return hipamr::launch_prims<T>(ptrs_, s, nullptr); }
   Control* control() override { return ptrs_.ctrl; }
   double* target_array() override { return targets_; }
   long long* maxsteps_array() override { return maxsteps_; }

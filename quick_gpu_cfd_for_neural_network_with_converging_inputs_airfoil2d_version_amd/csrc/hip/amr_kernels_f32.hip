@@ -8,6 +8,7 @@ namespace hipamr {
 
 template int launch_trial<float>(const ArenaPtrs<float>&, hipStream_t, LaunchHook*);
 template int launch_residual<float>(const ArenaPtrs<float>&, hipStream_t, LaunchHook*);
+template int launch_prims<float>(const ArenaPtrs<float>&, hipStream_t, LaunchHook*);
 template void launch_monitor<float>(const ArenaPtrs<float>&, hipStream_t);
 
 }  // namespace hipamr

@@ -12,6 +12,7 @@ void launch_target(Control* ctrl, int n, double target, long long maxsteps, cons
 
 template int launch_trial<double>(const ArenaPtrs<double>&, hipStream_t, LaunchHook*);
 template int launch_residual<double>(const ArenaPtrs<double>&, hipStream_t, LaunchHook*);
+template int launch_prims<double>(const ArenaPtrs<double>&, hipStream_t, LaunchHook*);
 template void launch_monitor<double>(const ArenaPtrs<double>&, hipStream_t);
 
 }  // namespace hipamr

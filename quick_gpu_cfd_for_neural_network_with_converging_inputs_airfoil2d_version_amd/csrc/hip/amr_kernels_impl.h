@@ -49,9 +49,12 @@ __device__ __forceinline__ bool pass_on(const Control* c, int pass) {
 enum : int { BAD_BASE = 1, BAD_STAGE = 2 };
 __device__ __forceinline__ void flag_bad(Control* c, int bit) { atomicOr(&c->bad, bit); }
 
-// Primitive buffer read by a pass: the base pass and stage 2 read buffer wbuf, stage 1 reads the
-// other buffer (written by stage 0); each stage writes the buffer it does not read.
-__device__ __forceinline__ int read_buffer(const Control* c, int stage) { return stage == 1 ? c->wbuf ^ 1 : c->wbuf; }
+// Three primitive buffers per cell.  A = wbuf holds the primitives of U[cur]; stage 0 writes
+// B = A + 1 (mod 3), read by stage 1, which writes C = A + 2, read by stage 2, which writes the
+// candidate state's primitives into B.  An accepted trial makes B the new A; a rejected one
+// leaves A intact, so the base primitives are recomputed only after the host changes a state.
+__device__ __forceinline__ int read_buffer(const Control* c, int stage) { return (c->wbuf + stage) % 3; }
+__device__ __forceinline__ int write_buffer(const Control* c, int stage) { return (c->wbuf + (stage == 1 ? 2 : 1)) % 3; }
 
 // FP32 work precision, mesh without SRD: the RK stage states U1, U2 are kept as T increments
 // over the FP64 state (U1 = U + D1, U2 = U + D2), half the bytes of FP64 copies.  The
@@ -764,7 +767,7 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
       }
       CellPrim<T> w;
       if (!make_prim_q(q, d, w)) flag_bad(c, BAD_STAGE);
-      else store_prim(a, rb ^ 1, k, w);
+      else store_prim(a, write_buffer(c, STAGE), k, w);
     } else {
       // SSP-RK3: u2 = 3/4 u + 1/4 (u1 + dt r)  or  u1 = 1/3 u + 2/3 (u2 + dt r).
       const double dt = c->trial_dt;
@@ -781,7 +784,7 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
       if (!d.small) {
         CellPrim<T> w;
         if (!make_prim(out, nc, k, d, w)) flag_bad(c, BAD_STAGE);
-        else store_prim(a, rb ^ 1, k, w);
+        else store_prim(a, write_buffer(c, STAGE), k, w);
       }
     }
   }
@@ -875,7 +878,7 @@ __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
     }
     CellPrim<T> w;
     if (!make_prim_q(q, d, w)) flag_bad(c, BAD_STAGE);
-    else store_prim(a, c->wbuf ^ 1, k, w);
+    else store_prim(a, write_buffer(c, 0), k, w);
     return;
   }
   double* out = a.U[cur ^ 1];
@@ -883,7 +886,7 @@ __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
   if (!d.small) {
     CellPrim<T> w;
     if (!make_prim(out, nc, k, d, w)) flag_bad(c, BAD_STAGE);
-    else store_prim(a, c->wbuf ^ 1, k, w);
+    else store_prim(a, write_buffer(c, 0), k, w);
   }
 }
 
@@ -945,7 +948,7 @@ __global__ __launch_bounds__(kMaxTile) void k_redist(ArenaPtrs<T> a, int stage) 
   u[3 * nc + k] = s3 * inv;
   CellPrim<T> w;
   if (!make_prim(u, nc, k, d, w)) flag_bad(c, BAD_STAGE);
-  else store_prim(a, read_buffer(c, stage) ^ 1, k, w);
+  else store_prim(a, write_buffer(c, stage), k, w);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -979,8 +982,7 @@ __global__ void k_commit(Control* ctrl, int n) {
       c->retrying = 1;
       c->dt = c->trial_dt * 0.5;
       if (c->attempt >= 21) c->fatal = 5;
-      c->w_valid = 0;
-      c->r0_valid = 1;
+      c->r0_valid = 1;  // R0 and the base primitives (buffer wbuf) still describe U[cur]
     } else {  // accept: the candidate buffers become the state
       const double remaining = c->target - c->time;
       c->time = c->trial_dt == remaining ? c->target : c->time + c->trial_dt;
@@ -989,7 +991,7 @@ __global__ void k_commit(Control* ctrl, int n) {
       c->retrying = 0;
       c->attempt = 0;
       c->cur ^= 1;
-      c->wbuf ^= 1;
+      c->wbuf = (c->wbuf + 1) % 3;
       c->w_valid = 1;
       c->r0_valid = 0;
       c->finished = c->time >= c->target;
@@ -1098,12 +1100,12 @@ inline int blocks_of(int n, int tile) { return (n + tile - 1) / tile; }
   } while (0)
 #define HIPAMR_LAUNCH(name, kernel, grid, ...) HIPAMR_LAUNCH_LDS(name, kernel, grid, 0, __VA_ARGS__)
 
+// The base pass proper (gradients, faces, residual, local dt); the base primitives are valid.
 template <class T>
-int launch_residual(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
+int launch_base_pass(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
   const dim3 tile(a.tile), cells(a.n_grid_blocks);
   const size_t base_lds = flux_lds_bytes<T>(a.tile, true), vel_lds = grad_lds_bytes<T>(a.tile);
   int launches = 0;
-  HIPAMR_LAUNCH("base_primitive", k_base_prim<T>, cells, a);
   if (a.any_laminar) HIPAMR_LAUNCH_LDS("gradient_base", (k_grad<T, 0>), cells, vel_lds, a);
   HIPAMR_LAUNCH_LDS("flux_base", (k_flux<T, 0>), cells, base_lds, a);
   if (a.any_visc_group) {
@@ -1114,10 +1116,25 @@ int launch_residual(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
 }
 
 template <class T>
+int launch_prims(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
+  const dim3 tile(a.tile), cells(a.n_grid_blocks);
+  int launches = 0;
+  HIPAMR_LAUNCH("base_primitive", k_base_prim<T>, cells, a);
+  return launches;
+}
+
+template <class T>
+int launch_residual(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
+  return launch_prims(a, s, hook) + launch_base_pass(a, s, hook);
+}
+
+// A trial never needs the base primitives recomputed: they are valid after launch_prims (run
+// whenever the host starts trials) and survive rejected trials.
+template <class T>
 int launch_trial(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
   const dim3 tile(a.tile), cells(a.n_grid_blocks);
   const size_t stage_lds = flux_lds_bytes<T>(a.tile, false), vel_lds = grad_lds_bytes<T>(a.tile);
-  int launches = launch_residual(a, s, hook);
+  int launches = launch_base_pass(a, s, hook);
   HIPAMR_LAUNCH("stage0", k_stage0<T>, cells, a);
   if (a.any_small) {
     HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 0);

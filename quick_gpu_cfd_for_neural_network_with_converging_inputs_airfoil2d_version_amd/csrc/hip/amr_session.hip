@@ -147,6 +147,7 @@ int amr_hip_advance(amr_session* s, double tend, int64_t maxsteps, double wall_b
   auto last = t0;
   bool expired = false;
   hipamr::launch_target(s->arena->control(), 1, tend, maxsteps, nullptr, nullptr, s->stream);
+  s->arena->launch_prims(s->stream);  // base primitives when the state was written by the host
   for (;;) {
     if (s->options.event_timing && !checked(hipEventRecord(s->start, s->stream), err, err_len, "AMR timing record start"))
       return 1;
@@ -314,6 +315,7 @@ int amr_hip_run(amr_session* s, double tend, int32_t trials, amr_state_meta* sta
   if (!s || !state || trials < 1 || !std::isfinite(tend) || tend < s->host_control->time)
     return fail(err, err_len, "invalid AMR GPU run");
   hipamr::launch_target(s->arena->control(), 1, tend, INT64_MAX, nullptr, nullptr, s->stream);
+  s->arena->launch_prims(s->stream);  // base primitives when the state was written by the host
   for (int i = 0; i < trials; ++i)
     if (launch_one(s)) return fail(err, err_len, "AMR trial launch failed");
   if (!checked(hipGetLastError(), err, err_len, "AMR trial kernels") ||
@@ -332,6 +334,7 @@ int amr_hip_profile(amr_session* s, double tend, int32_t trials, amr_kernel_time
   if (!s || !out || !count || trials < 1 || capacity < 1 || !std::isfinite(tend) || tend < s->host_control->time)
     return fail(err, err_len, "invalid AMR GPU profile request");
   hipamr::launch_target(s->arena->control(), 1, tend, INT64_MAX, nullptr, nullptr, s->stream);
+  s->arena->launch_prims(s->stream);  // base primitives when the state was written by the host
   hipamr::EventHook hook(s->stream);
   for (int i = 0; i < trials; ++i) s->arena->launch_trial(s->stream, &hook);
   if (!checked(hipGetLastError(), err, err_len, "AMR trial kernels") ||
