@@ -89,6 +89,61 @@ def build_one(spec: dict, preset: str = "LF2"):
 SERVE_MIX = (("LF1", 0.50, 48_000), ("LF2", 0.35, 90_000), ("L7", 0.15, 180_000))  # preset, share, ~steps
 
 
+def config1_workload(args) -> dict:
+    """Config 1: FP64 Euler, slip wall, HLLC, first order, SSP-RK3, CFL 0.15 on LF1 around a
+    symmetric 12% NACA-4-style section (200 points), M 1.65, AoA 1.5 deg, farfield, to t* = 0.01,
+    with SRD small cells and agglomerated.  The FP64 CPU oracle (one core) and the FP64 device
+    session run the same case; steps must agree and the states within 1e-10 relative L2."""
+    from oracle import pyoracle as O
+    from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import config as CF
+    from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import geometry as G
+    from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import mesh as M
+    from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd.session import Session
+
+    poly, patches = G.reorder_points(G.naca4_points(0.12, 0.0, 0.4, 100))
+    rows = {}
+    for label, agglomerate in (("srd", False), ("agglomerated", True)):
+        case = CF.make_case("LF1", {"physics": {"model": "euler", "turbulence": "none"},
+                                    "mesh": {"agglomerate_small": agglomerate}})
+        mesh = M.build_case_mesh(case, poly, patches)
+        params = M.params_of(case)
+        u0 = mesh.initialize(params, case.freestream)
+        t0 = time.perf_counter()
+        ref, meta = O.run(mesh, params, u0, 0.01)
+        cpu = time.perf_counter() - t0
+        with Session(mesh, params, u0) as s:
+            s.advance(0.01)  # warm-up solve (graph capture, first launches)
+        with Session(mesh, params, u0) as s:
+            t0 = time.perf_counter()
+            s.advance(0.01)
+            gpu = time.perf_counter() - t0
+            got, gmeta = s.download(), dict(s.meta)
+        nv = 4
+        err = max(float(np.linalg.norm(got.reshape(nv, -1)[v] - ref.reshape(nv, -1)[v]) /
+                        np.linalg.norm(ref.reshape(nv, -1)[v])) for v in range(nv))
+        n = mesh.n_cells
+        rows[label] = dict(cells=n, small_cells=int(mesh.info.small_cells), steps=meta["steps"],
+                           gpu_steps=gmeta["steps"], cpu_seconds=round(cpu, 4), gpu_seconds=round(gpu, 4),
+                           cpu_cell_steps_per_s=round(n * meta["steps"] / cpu), gpu_cell_steps_per_s=round(
+                               n * gmeta["steps"] / gpu), state_relative_l2=err)
+        log(f"config 1 {label}: {n} cells, {meta['steps']} steps (device {gmeta['steps']}), CPU {cpu:.3f} s, "
+            f"device {gpu:.4f} s, relative L2 {err:.2e}")
+        if gmeta["steps"] != meta["steps"] or err > 1e-10:
+            raise SystemExit(f"config 1 {label}: device result differs from the oracle")
+    cpu_rate = rows["agglomerated"]["cpu_cell_steps_per_s"]
+    return {
+        "metric": "config 1 FP64 Euler LF1 NACA 0012 to t*=0.01: cell-steps per second", "unit": "cell-steps/s",
+        "value": rows["agglomerated"]["gpu_cell_steps_per_s"], "n_gpus": 1, "steps": rows["agglomerated"]["steps"],
+        "warmup": 1, "ms_per_step": round(rows["agglomerated"]["gpu_seconds"] / rows["agglomerated"]["steps"] * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: symmetric 12% NACA-4-style section, uniform freestream start",
+        "config": {"workload": "config 1: FP64 Euler LF1, t*=0.01, SRD and agglomerated", "parallelism": "single GPU"},
+        "cpu_baseline": {"value": cpu_rate, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+                         "sample": f"the full config 1 agglomerated run on the FP64 oracle ({rows['agglomerated']['steps']} steps)"},
+        "detail": rows,
+    }
+
+
 def serve_workload(args) -> dict:
     """Continuous batching: requests (config-3 airfoils on LF1 / LF2 / level-7 meshes, 50/35/15 %)
     arrive as a Poisson stream and are admitted into an arena of slot classes sized by the HBM
@@ -184,6 +239,8 @@ def main():
     ap.add_argument("--graph", action=argparse.BooleanOptionalAction, default=True,
                     help="replay captured hipGraphs in the timed steps (--no-graph: direct launches)")
     ap.add_argument("--tile", type=int, default=0, help="launch tile (0: autotune over 64/128/256)")
+    ap.add_argument("--config1", action="store_true",
+                    help="instead of the batch workload: config 1 (FP64 Euler LF1, t*=0.01) on the oracle and the device")
     ap.add_argument("--serve", type=int, default=0,
                     help="instead of the batch workload: serve this many Poisson-arriving requests (LF1/LF2/L7 mix)")
     ap.add_argument("--serve-slots", type=int, default=192, help="slots of the serving arena")
@@ -207,6 +264,9 @@ def main():
     if device_count() < 1:
         raise SystemExit("bench.py needs a HIP device")
     lib = N.load_library()
+    if args.config1:
+        print(json.dumps(config1_workload(args)), flush=True)
+        return
     if args.serve:
         print(json.dumps(serve_workload(args)), flush=True)
         return
