@@ -72,6 +72,7 @@ struct ArenaPtrs {
   int n_meshes, nc, nf, nw, tile;
   int n_cell_blocks, n_face_blocks, n_wall_blocks;
   int n_grid_blocks;  // blocks of a cell launch (>= n_cell_blocks, a multiple of 8)
+  int commit_first, commit_stride;  // slots a trial's commit covers: first, first + stride, ...
   int any_laminar, any_sa, any_small, any_visc_group;
   const MeshDesc<T>* desc;
   Control* ctrl;

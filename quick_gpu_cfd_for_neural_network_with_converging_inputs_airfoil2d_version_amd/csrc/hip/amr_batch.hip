@@ -1,7 +1,9 @@
 // amr_batch.hip -- many meshes in one device arena behind the C ABI (amr_batch_* in amr_hip.h).
 //
-// One stream, one arena of slots, one captured trial graph for all slots.  The host polls the
-// array of control records (104 bytes per slot) every batch_trials trials.  Slot independence:
+// One arena of slots.  With two or more slots the trial runs as two independent branches (even
+// and odd slots), each captured as its own graph and replayed on its own stream, so that one
+// branch's kernels fill the GPU while the other's drain; the streams join at every poll.  The
+// host polls the array of control records (104 bytes per slot) every batch_trials trials.  Slot independence:
 // every kernel reads only its own slot's ranges and control record, the only cross-block
 // reductions (minimum dt, monitor partials) are per slot and exact (atomicMin on the bit
 // pattern) or done in a fixed block order, so a slot's result does not depend on its
@@ -29,9 +31,12 @@ struct amr_batch {
   std::unique_ptr<Arena> arena;
   amr_batch_options options{};
   amr_gpu_stats stats{};
-  hipStream_t stream = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipStream_t stream = nullptr;        // branch 0; every host-side operation is ordered on it
+  hipStream_t side = nullptr;          // branch 1 (branched arenas only)
+  hipEvent_t host_done = nullptr;      // recorded on `stream` after host-side work, awaited by `side`
+  bool side_must_wait = false;         // host-side work was queued on `stream` since the last join
+  std::vector<hipGraph_t> graphs;      // one per branch
+  std::vector<hipGraphExec_t> execs;
   Control* host_control = nullptr;  // pinned, n slots
   double* host_targets = nullptr;   // pinned, n slots
   long long* host_maxsteps = nullptr;
@@ -39,12 +44,15 @@ struct amr_batch {
 
   ~amr_batch() {
     if (stream) (void)hipStreamSynchronize(stream);
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    if (side) (void)hipStreamSynchronize(side);
+    for (hipGraphExec_t e : execs) (void)hipGraphExecDestroy(e);
+    for (hipGraph_t g : graphs) (void)hipGraphDestroy(g);
     arena.reset();
     if (host_control) (void)hipHostFree(host_control);
     if (host_targets) (void)hipHostFree(host_targets);
     if (host_maxsteps) (void)hipHostFree(host_maxsteps);
+    if (host_done) (void)hipEventDestroy(host_done);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -62,11 +70,14 @@ hipamr::SlotCapacity to_cap(const amr_slot_capacity& c) {
   return hipamr::SlotCapacity{c.cells, c.faces, c.walls, c.csr, c.terms, c.groups, c.members};
 }
 
+// Join both branches, then read every control record.
 int poll(amr_batch* b, char* err, size_t err_len) {
+  if (b->side && !checked(hipStreamSynchronize(b->side), err, err_len, "AMR branch join")) return 1;
   if (!checked(copy_sync(b->host_control, b->arena->control(), b->n * sizeof(Control), hipMemcpyDeviceToHost,
                          b->stream),
                err, err_len, "AMR control D2H"))
     return 1;
+  b->side_must_wait = false;
   b->stats.control_d2h_bytes += b->n * sizeof(Control);
   ++b->stats.status_polls;
   return 0;
@@ -89,9 +100,23 @@ void fill_states(const amr_batch* b, amr_slot_state* out) {
   }
 }
 
+// One trial of every slot: each branch on its own stream.  The side stream first waits for
+// host-side work still queued on the main stream (a monitor's control restore).
 int launch_one(amr_batch* b) {
-  if (b->exec) return hipGraphLaunch(b->exec, b->stream) == hipSuccess ? 0 : 1;
-  b->arena->launch_trial(b->stream);
+  if (b->side && b->side_must_wait) {
+    if (hipEventRecord(b->host_done, b->stream) != hipSuccess || hipStreamWaitEvent(b->side, b->host_done, 0) != hipSuccess)
+      return 1;
+    b->side_must_wait = false;
+  }
+  const int branches = b->arena->n_branches();
+  for (int j = 0; j < branches; ++j) {
+    hipStream_t s = j == 0 ? b->stream : b->side;
+    if (!b->execs.empty()) {
+      if (hipGraphLaunch(b->execs[j], s) != hipSuccess) return 1;
+    } else {
+      b->arena->launch_trial_branch(j, s);
+    }
+  }
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -109,16 +134,28 @@ int finish_setup(amr_batch* b, Clock::time_point t0, char* err, size_t err_len) 
   b->stats.allocated_bytes = b->arena->device_bytes();
   b->stats.geometry_h2d_bytes = b->arena->geometry_bytes();
   b->stats.pinned_bytes = n * (sizeof(Control) + sizeof(double) + sizeof(long long));
+  const int branches = b->arena->n_branches();
+  if (branches > 1) {
+    if (!checked(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), err, err_len, "AMR branch stream") ||
+        !checked(hipEventCreateWithFlags(&b->host_done, hipEventDisableTiming), err, err_len, "AMR branch event"))
+      return 1;
+  }
   if (b->options.graph) {
-    if (!checked(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal), err, err_len, "AMR graph begin"))
-      return 1;
-    b->arena->launch_trial(b->stream);
-    if (!checked(hipStreamEndCapture(b->stream, &b->graph), err, err_len, "AMR graph end") ||
-        !checked(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), err, err_len, "AMR graph instantiate"))
-      return 1;
-    size_t nodes = 0;
-    if (!checked(hipGraphGetNodes(b->graph, nullptr, &nodes), err, err_len, "AMR graph node count")) return 1;
-    b->stats.graph_nodes_per_trial = nodes;
+    for (int j = 0; j < branches; ++j) {
+      hipStream_t s = j == 0 ? b->stream : b->side;
+      hipGraph_t g = nullptr;
+      hipGraphExec_t e = nullptr;
+      if (!checked(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), err, err_len, "AMR graph begin"))
+        return 1;
+      b->arena->launch_trial_branch(j, s);
+      if (!checked(hipStreamEndCapture(s, &g), err, err_len, "AMR graph end")) return 1;
+      b->graphs.push_back(g);
+      if (!checked(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), err, err_len, "AMR graph instantiate")) return 1;
+      b->execs.push_back(e);
+      size_t nodes = 0;
+      if (!checked(hipGraphGetNodes(g, nullptr, &nodes), err, err_len, "AMR graph node count")) return 1;
+      b->stats.graph_nodes_per_trial += nodes;
+    }
   }
   b->stats.setup_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
   return 0;
@@ -428,6 +465,7 @@ int amr_batch_monitor(amr_batch* b, const int32_t* mask, const double* frames, c
                               b->stream),
                err, err_len, "AMR control restore"))
     return 1;
+  b->side_must_wait = true;
   return 0;
 }
 

@@ -66,6 +66,10 @@ class Arena {
   // Base primitives of every running slot whose host-written state has none yet; follows every
   // launch_target before trials are launched.
   virtual int launch_prims(hipStream_t s) = 0;
+  // Independent branches of the trial (slots m with m % n_branches() == j) and the trial of one
+  // branch: its slots' kernels and commit only, for its own stream.
+  virtual int n_branches() const = 0;
+  virtual int launch_trial_branch(int j, hipStream_t s) = 0;
   virtual Control* control() = 0;
   // Device arrays of per-slot targets / step caps used by launch_target.
   virtual double* target_array() = 0;

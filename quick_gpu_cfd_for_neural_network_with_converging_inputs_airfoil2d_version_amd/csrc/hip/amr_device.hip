@@ -86,11 +86,17 @@ struct Totals {
 // XCD takes its runs in slot order.  So any set of running slots keeps all XCDs busy, tiles that
 // share neighbours share an L2, and the rotation evens out the runs' unequal lengths.  Blocks
 // past the end of an XCD's list get -1.
-std::vector<int> block_tiles(const std::vector<SlotCapacity>& caps, int tile) {
+// With branches > 1 only the slots m with m % branches == branch are listed (the table of one
+// branch of the trial, see branch_count).
+std::vector<int> block_tiles(const std::vector<SlotCapacity>& caps, int tile, int branches = 1, int branch = 0) {
   std::vector<int> list[8];
   int t0 = 0;
   for (size_t m = 0; m < caps.size(); ++m) {
     const int n = caps[m].cells / tile;
+    if (static_cast<int>(m % branches) != branch) {
+      t0 += n;
+      continue;
+    }
     for (int j = 0; j < 8; ++j) {
       const int lo = t0 + static_cast<int>(static_cast<long long>(j) * n / 8);
       const int hi = t0 + static_cast<int>(static_cast<long long>(j + 1) * n / 8);
@@ -107,6 +113,25 @@ std::vector<int> block_tiles(const std::vector<SlotCapacity>& caps, int tile) {
   return out;
 }
 
+// An arena of several slots runs its trial as two independent branches (even and odd slots) on
+// two streams, so that one branch's kernels fill the GPU while the other's drain.
+int branch_count(size_t slots) { return slots >= 2 ? 2 : 1; }
+
+// The block tables of an arena: the whole arena first, then one per branch (when branched).
+std::vector<int> all_block_tiles(const std::vector<SlotCapacity>& caps, int tile, std::vector<int>* offsets) {
+  std::vector<int> out = block_tiles(caps, tile);
+  offsets->assign(1, 0);
+  const int k = branch_count(caps.size());
+  if (k > 1)
+    for (int j = 0; j < k; ++j) {
+      offsets->push_back(static_cast<int>(out.size()));
+      const std::vector<int> part = block_tiles(caps, tile, k, j);
+      out.insert(out.end(), part.begin(), part.end());
+    }
+  offsets->push_back(static_cast<int>(out.size()));
+  return out;
+}
+
 Totals totals_of(const std::vector<SlotCapacity>& caps, int tile) {
   Totals t{};
   t.meshes = caps.size();
@@ -120,7 +145,8 @@ Totals totals_of(const std::vector<SlotCapacity>& caps, int tile) {
     t.members += c.members;
   }
   t.cell_blocks = t.cells / tile;
-  t.grid_blocks = block_tiles(caps, tile).size();
+  std::vector<int> offsets;
+  t.grid_blocks = all_block_tiles(caps, tile, &offsets).size();
   t.face_blocks = t.faces / tile;
   t.wall_blocks = t.walls / tile;
   return t;
@@ -312,7 +338,7 @@ class DeviceArena final : public Arena {
       for (int b = 0; b < slots_[m].cap.walls / tile; ++b) h.face_block_mesh.push_back(static_cast<int>(m));
     n_wall_blocks_ = static_cast<int>(h.face_block_mesh.size()) - n_face_blocks_;
     n_cell_blocks_ = static_cast<int>(h.cell_block_mesh.size());
-    h.block_tile = block_tiles(caps, tile);
+    h.block_tile = all_block_tiles(caps, tile, &table_offsets_);
     if (h.face_block_mesh.empty()) h.face_block_mesh.push_back(0);
     const auto sized = [](auto& v, size_t n, auto value) { v.assign(std::max<size_t>(n, 1), value); };
     sized(h.vol, nc, T(1));
@@ -603,13 +629,27 @@ class DeviceArena final : public Arena {
   std::string check_tables() const {
     const HostArrays<T>& h = h_;
     if (static_cast<int>(h.cell_block_mesh.size()) * tile_ != nc_) return "layout: cell block table";
-    std::vector<int> seen(n_cell_blocks_, 0);
-    if (h.block_tile.size() % 8) return "layout: block tile table size";
-    for (const int t : h.block_tile) {
-      if (t < -1 || t >= n_cell_blocks_ || (t >= 0 && seen[t]++)) return "layout: block tile table";
+    // The whole-arena table lists every tile once; the branch tables together list every tile
+    // once, each only tiles of its own slots.
+    const int tables = static_cast<int>(table_offsets_.size()) - 1;
+    const int branches = tables == 1 ? 1 : tables - 1;
+    std::vector<int> whole(n_cell_blocks_, 0), parts(n_cell_blocks_, 0);
+    for (int t = 0; t < tables; ++t) {
+      const int lo = table_offsets_[t], hi = table_offsets_[t + 1];
+      if ((hi - lo) % 8) return "layout: block tile table size";
+      for (int i = lo; i < hi; ++i) {
+        const int tl = h.block_tile[i];
+        if (tl < -1 || tl >= n_cell_blocks_) return "layout: block tile table";
+        if (tl < 0) continue;
+        if (t == 0) {
+          if (whole[tl]++) return "layout: block tile table";
+        } else if (parts[tl]++ || h.cell_block_mesh[tl] % branches != t - 1) {
+          return "layout: branch block tile table";
+        }
+      }
     }
-    for (const int c : seen)
-      if (c != 1) return "layout: block tile table misses a tile";
+    for (int b = 0; b < n_cell_blocks_; ++b)
+      if (whole[b] != 1 || (tables > 1 && parts[b] != 1)) return "layout: block tile table misses a tile";
     for (int b = 0; b < n_cell_blocks_; ++b) {
       const MeshSlot& s = slots_[h.cell_block_mesh[b]];
       if (b * tile_ < s.cell_begin || b * tile_ >= s.cell_begin + s.cap.cells) return "layout: cell block range";
@@ -764,7 +804,9 @@ class DeviceArena final : public Arena {
     a.nw = nw;
     a.tile = tile_;
     a.n_cell_blocks = n_cell_blocks_;
-    a.n_grid_blocks = static_cast<int>(h.block_tile.size());
+    a.n_grid_blocks = table_offsets_[1];  // the whole-arena table
+    a.commit_first = 0;
+    a.commit_stride = 1;
     a.n_face_blocks = n_face_blocks_;
     a.n_wall_blocks = n_wall_blocks_;
     a.any_laminar = features_.laminar;
@@ -850,6 +892,16 @@ class DeviceArena final : public Arena {
   uint64_t state_bytes() const override { return state_bytes_; }
   int launch_trial(hipStream_t s, LaunchHook* hook) override { return hipamr::launch_trial<T>(ptrs_, s, hook); }
   int launch_residual(hipStream_t s, LaunchHook* hook) override { return hipamr::launch_residual<T>(ptrs_, s, hook); }
+  int n_branches() const override { return static_cast<int>(table_offsets_.size()) == 2 ? 1 : static_cast<int>(table_offsets_.size()) - 2; }
+  int launch_trial_branch(int j, hipStream_t s) override {
+    if (n_branches() == 1) return hipamr::launch_trial<T>(ptrs_, s, nullptr);
+    ArenaPtrs<T> a = ptrs_;  // the branch's block table; its slots commit in its own launch
+    a.block_tile = ptrs_.block_tile + table_offsets_[1 + j];
+    a.n_grid_blocks = table_offsets_[2 + j] - table_offsets_[1 + j];
+    a.commit_first = j;
+    a.commit_stride = n_branches();
+    return hipamr::launch_trial<T>(a, s, nullptr);
+  }
   int launch_prims(hipStream_t s) override { return hipamr::launch_prims<T>(ptrs_, s, nullptr); }
   Control* control() override { return ptrs_.ctrl; }
   double* target_array() override { return targets_; }
@@ -1060,6 +1112,7 @@ class DeviceArena final : public Arena {
   uint64_t slab_bytes_ = 0, geometry_bytes_ = 0, state_bytes_ = 0;
   int nc_ = 0, nf_ = 0, nw_ = 0, tile_ = 256;
   int n_cell_blocks_ = 0, n_face_blocks_ = 0, n_wall_blocks_ = 0;
+  std::vector<int> table_offsets_;  // block tables in block_tile: whole arena, then each branch
 };
 
 template <class T>

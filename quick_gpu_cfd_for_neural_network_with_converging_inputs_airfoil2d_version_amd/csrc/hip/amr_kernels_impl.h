@@ -969,8 +969,9 @@ __device__ __forceinline__ void begin_trial(Control* c) {
   if (!c->r0_valid) c->dt_min = kInfBits;
 }
 
-__global__ void k_commit(Control* ctrl, int n) {
-  const int m = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+// Commits slots first, first + stride, ... (< n): a branch of the trial commits its own slots.
+__global__ void k_commit(Control* ctrl, int n, int first, int stride) {
+  const int m = first + stride * static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   if (m >= n) return;
   Control* c = ctrl + m;
   if (c->active) {
@@ -1154,7 +1155,9 @@ int launch_trial(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
   }
   {
     const dim3 tile(256);
-    HIPAMR_LAUNCH("commit", k_commit, dim3(blocks_of(a.n_meshes, 256)), a.ctrl, a.n_meshes);
+    const int count = (a.n_meshes - a.commit_first + a.commit_stride - 1) / a.commit_stride;
+    HIPAMR_LAUNCH("commit", k_commit, dim3(blocks_of(count, 256)), a.ctrl, a.n_meshes, a.commit_first,
+                  a.commit_stride);
   }
   return launches;
 }
