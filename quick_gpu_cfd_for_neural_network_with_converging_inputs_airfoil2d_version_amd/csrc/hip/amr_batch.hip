@@ -1,8 +1,8 @@
 // amr_batch.hip -- many meshes in one device arena behind the C ABI (amr_batch_* in amr_hip.h).
 //
-// One arena of slots.  With two or more slots the trial runs as two independent branches (even
-// and odd slots), each captured as its own graph and replayed on its own stream, so that one
-// branch's kernels fill the GPU while the other's drain; the streams join at every poll.  The
+// One arena of slots.  With two or more slots the trial runs as independent branches (slot m in
+// branch m % branches), each captured as its own graph and replayed on its own stream, so that
+// one branch's kernels fill the GPU while another's drain; the streams join at every poll.  The
 // host polls the array of control records (104 bytes per slot) every batch_trials trials.  Slot independence:
 // every kernel reads only its own slot's ranges and control record, the only cross-block
 // reductions (minimum dt, monitor partials) are per slot and exact (atomicMin on the bit
@@ -32,8 +32,8 @@ struct amr_batch {
   amr_batch_options options{};
   amr_gpu_stats stats{};
   hipStream_t stream = nullptr;        // branch 0; every host-side operation is ordered on it
-  hipStream_t side = nullptr;          // branch 1 (branched arenas only)
-  hipEvent_t host_done = nullptr;      // recorded on `stream` after host-side work, awaited by `side`
+  std::vector<hipStream_t> sides;      // branches 1, 2, ... (branched arenas only)
+  hipEvent_t host_done = nullptr;      // recorded on `stream` after host-side work, awaited by `sides`
   bool side_must_wait = false;         // host-side work was queued on `stream` since the last join
   std::vector<hipGraph_t> graphs;      // one per branch
   std::vector<hipGraphExec_t> execs;
@@ -44,7 +44,7 @@ struct amr_batch {
 
   ~amr_batch() {
     if (stream) (void)hipStreamSynchronize(stream);
-    if (side) (void)hipStreamSynchronize(side);
+    for (hipStream_t s : sides) (void)hipStreamSynchronize(s);
     for (hipGraphExec_t e : execs) (void)hipGraphExecDestroy(e);
     for (hipGraph_t g : graphs) (void)hipGraphDestroy(g);
     arena.reset();
@@ -52,7 +52,7 @@ struct amr_batch {
     if (host_targets) (void)hipHostFree(host_targets);
     if (host_maxsteps) (void)hipHostFree(host_maxsteps);
     if (host_done) (void)hipEventDestroy(host_done);
-    if (side) (void)hipStreamDestroy(side);
+    for (hipStream_t s : sides) (void)hipStreamDestroy(s);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -70,9 +70,10 @@ hipamr::SlotCapacity to_cap(const amr_slot_capacity& c) {
   return hipamr::SlotCapacity{c.cells, c.faces, c.walls, c.csr, c.terms, c.groups, c.members};
 }
 
-// Join both branches, then read every control record.
+// Join the branches, then read every control record.
 int poll(amr_batch* b, char* err, size_t err_len) {
-  if (b->side && !checked(hipStreamSynchronize(b->side), err, err_len, "AMR branch join")) return 1;
+  for (hipStream_t s : b->sides)
+    if (!checked(hipStreamSynchronize(s), err, err_len, "AMR branch join")) return 1;
   if (!checked(copy_sync(b->host_control, b->arena->control(), b->n * sizeof(Control), hipMemcpyDeviceToHost,
                          b->stream),
                err, err_len, "AMR control D2H"))
@@ -100,17 +101,20 @@ void fill_states(const amr_batch* b, amr_slot_state* out) {
   }
 }
 
-// One trial of every slot: each branch on its own stream.  The side stream first waits for
+hipStream_t branch_stream(const amr_batch* b, int j) { return j == 0 ? b->stream : b->sides[j - 1]; }
+
+// One trial of every slot: each branch on its own stream.  The side streams first wait for
 // host-side work still queued on the main stream (a monitor's control restore).
 int launch_one(amr_batch* b) {
-  if (b->side && b->side_must_wait) {
-    if (hipEventRecord(b->host_done, b->stream) != hipSuccess || hipStreamWaitEvent(b->side, b->host_done, 0) != hipSuccess)
-      return 1;
+  if (!b->sides.empty() && b->side_must_wait) {
+    if (hipEventRecord(b->host_done, b->stream) != hipSuccess) return 1;
+    for (hipStream_t s : b->sides)
+      if (hipStreamWaitEvent(s, b->host_done, 0) != hipSuccess) return 1;
     b->side_must_wait = false;
   }
   const int branches = b->arena->n_branches();
   for (int j = 0; j < branches; ++j) {
-    hipStream_t s = j == 0 ? b->stream : b->side;
+    hipStream_t s = branch_stream(b, j);
     if (!b->execs.empty()) {
       if (hipGraphLaunch(b->execs[j], s) != hipSuccess) return 1;
     } else {
@@ -135,14 +139,17 @@ int finish_setup(amr_batch* b, Clock::time_point t0, char* err, size_t err_len) 
   b->stats.geometry_h2d_bytes = b->arena->geometry_bytes();
   b->stats.pinned_bytes = n * (sizeof(Control) + sizeof(double) + sizeof(long long));
   const int branches = b->arena->n_branches();
-  if (branches > 1) {
-    if (!checked(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), err, err_len, "AMR branch stream") ||
-        !checked(hipEventCreateWithFlags(&b->host_done, hipEventDisableTiming), err, err_len, "AMR branch event"))
-      return 1;
+  for (int j = 1; j < branches; ++j) {
+    hipStream_t s = nullptr;
+    if (!checked(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), err, err_len, "AMR branch stream")) return 1;
+    b->sides.push_back(s);
   }
+  if (branches > 1 &&
+      !checked(hipEventCreateWithFlags(&b->host_done, hipEventDisableTiming), err, err_len, "AMR branch event"))
+    return 1;
   if (b->options.graph) {
     for (int j = 0; j < branches; ++j) {
-      hipStream_t s = j == 0 ? b->stream : b->side;
+      hipStream_t s = branch_stream(b, j);
       hipGraph_t g = nullptr;
       hipGraphExec_t e = nullptr;
       if (!checked(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), err, err_len, "AMR graph begin"))
