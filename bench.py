@@ -59,10 +59,14 @@ def profiled_traffic(batch: int, tile: int, seed: int):
     """HBM bytes of one stage pass (gradient_stage + flux_rk1) of this workload as measured by
     rocprofv3 (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a copy kernel of known size),
     read from the committed profile summary; None when the summary covers another workload."""
-    if (batch, tile, seed) != (256, 256, 2026) or not STAGE_PASS_PROFILE.exists():
+    if not STAGE_PASS_PROFILE.exists():
         return None
-    m = re.search(r"^stage_pass_hbm_bytes: (\d+)$", STAGE_PASS_PROFILE.read_text(), re.M)
-    return int(m.group(1)) if m else None
+    text = STAGE_PASS_PROFILE.read_text()
+    w = re.search(r"^stage_pass_profile: batch=(\d+) tile=(\d+) seed=(\d+)$", text, re.M)
+    m = re.search(r"^stage_pass_hbm_bytes: (\d+)$", text, re.M)
+    if not w or not m or tuple(int(x) for x in w.groups()) != (batch, tile, seed):
+        return None
+    return int(m.group(1))
 
 
 def draw_airfoils(n: int, seed: int) -> list[dict]:

@@ -37,17 +37,19 @@ enum : int { VISC_VOLUME = 0, VISC_SRD_RATE = 1, VISC_SRD_GROUP = 2 };
 // Face items pack (right cell + 1) into the low bits of one int and the face info above them.
 constexpr int kItemRightBits = 27;
 
-// Face items held in LDS at once per block, per cell of the tile (a tile of an LF1 / LF2 mesh
-// touches 2.2 - 2.7 faces per cell; a tile with more items is processed in several chunks).
-constexpr int kItemChunkNum = 5, kItemChunkDen = 2;
-__host__ __device__ constexpr int item_chunk(int tile) { return tile * kItemChunkNum / kItemChunkDen; }
+// Face items held in LDS at once per block (a tile of an LF1 / LF2 mesh touches 2.2 - 2.7 faces
+// per cell; a tile with more items is processed in several chunks): 5/2 per cell of the tile in
+// a stage pass, 3/2 in the base pass, whose items take 9 T instead of 5.  Measured on 256 LF2
+// meshes at tile 128: the base pass gains from the smaller chunk (202 against 210 us, more
+// blocks per CU), a stage pass loses from it (236 against 216 us, a second chunk round).
+__host__ __device__ constexpr int item_chunk(int tile, bool base) { return base ? tile * 3 / 2 : tile * 5 / 2; }
 
 // Dynamic LDS of a flux launch: 16 T per cell, then per item of a chunk 4 + 1 T (convective and
 // viscous flux, SA flux) and, in the base pass, 4 T more (dt terms).  Gradient launches: 4 T
 // per cell.
 template <class T>
 constexpr size_t flux_lds_bytes(int tile, bool base) {
-  return (static_cast<size_t>(tile) * 16 + static_cast<size_t>(item_chunk(tile)) * (base ? 9 : 5)) * sizeof(T);
+  return (static_cast<size_t>(tile) * 16 + static_cast<size_t>(item_chunk(tile, base)) * (base ? 9 : 5)) * sizeof(T);
 }
 template <class T>
 constexpr size_t grad_lds_bytes(int tile) {

@@ -164,8 +164,8 @@ struct FaceSlots {
   T* f;
   T* sf;
   T* x;
-  __device__ explicit FaceSlots(int tile)
-      : f(reinterpret_cast<T*>(tile_lds) + 16 * tile), sf(f + 4 * item_chunk(tile)), x(sf + item_chunk(tile)) {}
+  __device__ FaceSlots(int tile, bool base)
+      : f(reinterpret_cast<T*>(tile_lds) + 16 * tile), sf(f + 4 * item_chunk(tile, base)), x(sf + item_chunk(tile, base)) {}
 };
 
 template <class T>
@@ -597,7 +597,7 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
   const int first = tile * tn;
   const int staged = min(tn, d.cell_begin + d.n_cells - first);  // valid cells of the tile
   const TileCells<T> lds(tn);
-  const FaceSlots<T> slots(tn);
+  const FaceSlots<T> slots(tn, BASE);
   CellPrim<T> own{};
   CellGrad<T> owng{};
   if (valid) {
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
   // order.  A face shared with another tile is evaluated by both tiles from the same inputs, so
   // both cells see the same bits.
   const int2 range = a.item_range[tile];
-  const int chunk = item_chunk(tn);
+  const int chunk = item_chunk(tn, BASE);
   T s0 = 0, s1 = 0, s2 = 0, s3 = 0, spectral = 0, diffusion = 0, sa_net = 0, sa_spec = 0;
   bool ok = true;
   for (int c0 = range.x; c0 < range.y; c0 += chunk) {
