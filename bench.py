@@ -52,21 +52,20 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-STAGE_PASS_PROFILE = Path(__file__).resolve().parent / "profiles" / "stage_pass_lf2x256_fp32.md"
+PROFILES = Path(__file__).resolve().parent / "profiles"
 
 
 def profiled_traffic(batch: int, tile: int, seed: int):
     """HBM bytes of one stage pass (gradient_stage + flux_rk1) of this workload as measured by
     rocprofv3 (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a copy kernel of known size),
     read from the committed profile summary; None when the summary covers another workload."""
-    if not STAGE_PASS_PROFILE.exists():
-        return None
-    text = STAGE_PASS_PROFILE.read_text()
-    w = re.search(r"^stage_pass_profile: batch=(\d+) tile=(\d+) seed=(\d+)$", text, re.M)
-    m = re.search(r"^stage_pass_hbm_bytes: (\d+)$", text, re.M)
-    if not w or not m or tuple(int(x) for x in w.groups()) != (batch, tile, seed):
-        return None
-    return int(m.group(1))
+    for path in sorted(PROFILES.glob("stage_pass_*.md")):
+        text = path.read_text()
+        w = re.search(r"^stage_pass_profile: batch=(\d+) tile=(\d+) seed=(\d+)$", text, re.M)
+        m = re.search(r"^stage_pass_hbm_bytes: (\d+)$", text, re.M)
+        if w and m and tuple(int(x) for x in w.groups()) == (batch, tile, seed):
+            return int(m.group(1))
+    return None
 
 
 def draw_airfoils(n: int, seed: int) -> list[dict]:
