@@ -26,8 +26,11 @@ HOST_SOURCES = ["host/cutcell.cpp", "host/mesher.cpp", "host/viscous_prep.cpp", 
 HIP_SOURCES = ["hip/amr_kernels_f64.hip", "hip/amr_kernels_f32.hip", "hip/amr_device.hip", "hip/amr_session.hip",
                "hip/amr_batch.hip"]
 # Per-source extra flags: FP32 quotients and square roots of the FP32 kernels are not correctly
-# rounded (about 1 ulp: reciprocal and multiply) instead of the scaled IEEE sequences.
-HIP_EXTRA = {"hip/amr_kernels_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt"]}
+# rounded (about 1 ulp: reciprocal and multiply) instead of the scaled IEEE sequences, and FP32
+# denormals flush to zero, so that a quotient is v_rcp + v_mul without the frexp / ldexp range
+# scaling (8% fewer VALU instructions in the flux kernels).  Flow quantities never come near
+# 1e-38; the FP64 state, control and reductions keep full IEEE behaviour.
+HIP_EXTRA = {"hip/amr_kernels_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"]}
 INCLUDES = ["-I", str(ROOT / "include"), "-I", str(CSRC / "host"), "-I", str(CSRC / "hip")]
 ARCH = os.environ.get("HIPAMR_ARCH", "gfx950")
 

@@ -44,12 +44,12 @@ constexpr int kItemRightBits = 27;
 // blocks per CU), a stage pass loses from it (236 against 216 us, a second chunk round).
 __host__ __device__ constexpr int item_chunk(int tile, bool base) { return base ? tile * 3 / 2 : tile * 5 / 2; }
 
-// Dynamic LDS of a flux launch: 16 T per cell, then per item of a chunk 4 + 1 T (convective and
-// viscous flux, SA flux) and, in the base pass, 4 T more (dt terms).  Gradient launches: 4 T
-// per cell.
+// Dynamic LDS of a flux launch: 8 T per cell (primitive vectors), then per item of a chunk 4 + 1
+// T (convective and viscous flux, SA flux) and, in the base pass, 4 T more (dt terms).
+// Gradient launches: 4 T per cell.
 template <class T>
 constexpr size_t flux_lds_bytes(int tile, bool base) {
-  return (static_cast<size_t>(tile) * 16 + static_cast<size_t>(item_chunk(tile, base)) * (base ? 9 : 5)) * sizeof(T);
+  return (static_cast<size_t>(tile) * 8 + static_cast<size_t>(item_chunk(tile, base)) * (base ? 9 : 5)) * sizeof(T);
 }
 template <class T>
 constexpr size_t grad_lds_bytes(int tile) {

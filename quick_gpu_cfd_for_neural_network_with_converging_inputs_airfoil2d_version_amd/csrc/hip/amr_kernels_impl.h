@@ -141,19 +141,19 @@ __device__ __forceinline__ void store_grad(const ArenaPtrs<T>& a, int k, const C
   store4(a.GREC + 8 * k + 4, g.sx, g.sy, g.sdiff, sa_source);
 }
 
-// Cell vectors of the block's own tile staged in LDS: a neighbour inside the tile (most of them,
-// with cells in Hilbert order) is read from LDS instead of being gathered from L2 / HBM again.
-// The flux kernels also keep one chunk of face results there (FaceSlots).  Both live in the
-// launch's dynamic LDS, sized by the tile, so small tiles keep their occupancy.
+// Primitive vectors of the block's own tile staged in LDS: a neighbour inside the tile (most of
+// them, with cells in Hilbert order) is read from LDS instead of being gathered from L2 / HBM
+// again.  The flux kernels also keep one chunk of face results there (FaceSlots).  Both live in
+// the launch's dynamic LDS, sized by the tile, so small tiles keep their occupancy.  Gradient
+// records are not staged: the LDS they took (32 B per cell) is worth more as occupancy (the
+// flux kernels run at 6 waves per SIMD, bounded by registers and LDS together).
 extern __shared__ __align__(16) unsigned char tile_lds[];
 
 template <class T>
 struct TileCells {
   T* v;  // 4 per cell: rho, u, v, phi
   T* p;  // 4 per cell: p, mu, eddy, 0
-  T* g;  // 8 per cell: the gradient record
-  __device__ explicit TileCells(int tile)
-      : v(reinterpret_cast<T*>(tile_lds)), p(v + 4 * tile), g(v + 8 * tile) {}
+  __device__ explicit TileCells(int tile) : v(reinterpret_cast<T*>(tile_lds)), p(v + 4 * tile) {}
 };
 
 // Results of one chunk of face items: f = mass, x- and y-momentum, energy flux (times the
@@ -165,18 +165,13 @@ struct FaceSlots {
   T* sf;
   T* x;
   __device__ FaceSlots(int tile, bool base)
-      : f(reinterpret_cast<T*>(tile_lds) + 16 * tile), sf(f + 4 * item_chunk(tile, base)), x(sf + item_chunk(tile, base)) {}
+      : f(reinterpret_cast<T*>(tile_lds) + 8 * tile), sf(f + 4 * item_chunk(tile, base)), x(sf + item_chunk(tile, base)) {}
 };
 
 template <class T>
-__device__ __forceinline__ void stage_cell(const TileCells<T>& t, int i, const CellPrim<T>& w, const CellGrad<T>& g,
-                                           bool grads) {
+__device__ __forceinline__ void stage_cell(const TileCells<T>& t, int i, const CellPrim<T>& w) {
   store4(t.v + 4 * i, w.rho, w.u, w.v, w.phi);
   store4(t.p + 4 * i, w.p, w.mu, w.eddy, T(0));
-  if (grads) {
-    store4(t.g + 8 * i, g.ux, g.uy, g.vx, g.vy);
-    store4(t.g + 8 * i + 4, g.sx, g.sy, g.sdiff, T(0));
-  }
 }
 template <class T>
 __device__ __forceinline__ CellPrim<T> tile_prim(const TileCells<T>& t, int i) {
@@ -185,14 +180,6 @@ __device__ __forceinline__ CellPrim<T> tile_prim(const TileCells<T>& t, int i) {
   load4(t.v + 4 * i, w.rho, w.u, w.v, w.phi);
   load4(t.p + 4 * i, w.p, w.mu, w.eddy, pad);
   return w;
-}
-template <class T>
-__device__ __forceinline__ CellGrad<T> tile_grad(const TileCells<T>& t, int i) {
-  CellGrad<T> g;
-  T pad;
-  load4(t.g + 8 * i, g.ux, g.uy, g.vx, g.vy);
-  load4(t.g + 8 * i + 4, g.sx, g.sy, g.sdiff, pad);
-  return g;
 }
 
 // Face geometry record: length, dx, dy, ex, ey, distance.
@@ -578,8 +565,12 @@ __device__ __forceinline__ void wall_shear(const CellPrim<T>& donor, T nx, T ny,
 // ------------------------------------------------------------------------------------------
 // Residual of every cell from its faces and wall fragments.  STAGE 0: base residual R0 and the
 // local dt.  1, 2: second and third SSP-RK3 combinations, then the candidate's primitives.
+// FP32: at most 80 VGPRs (6 waves per SIMD; the kernel otherwise takes 84-88, 5 waves): with the
+// LDS of tile 128 that is 6 waves resident, and the stage passes take 200 instead of 215 us on
+// 256 LF2 meshes (8 waves needs 64 VGPRs and spills: slower).  FP64 would spill at 80: no bound.
 template <class T, int STAGE>
-__global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
+__global__ __launch_bounds__(kMaxTile) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) void k_flux(
+    ArenaPtrs<T> a) {
   constexpr bool BASE = STAGE == 0;
   const int tile = cell_tile(a);
   if (tile < 0) return;
@@ -599,11 +590,9 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
   const TileCells<T> lds(tn);
   const FaceSlots<T> slots(tn, BASE);
   CellPrim<T> own{};
-  CellGrad<T> owng{};
   if (valid) {
     own = load_prim(a, rb, k);
-    if (laminar) owng = load_grad(a, k);
-    stage_cell(lds, tid, own, owng, laminar);
+    stage_cell(lds, tid, own);
   }
   __syncthreads();
   // Face items of the tile, one chunk at a time: (A) each face touching the tile is evaluated
@@ -624,20 +613,18 @@ __global__ __launch_bounds__(kMaxTile) void k_flux(ArenaPtrs<T> a) {
       if (it.left >= 0) {
         if (ll >= 0 && ll < staged) {
           L = tile_prim(lds, ll);
-          if (laminar) Lg = tile_grad(lds, ll);
         } else {
           L = load_prim(a, rb, it.left);
-          if (laminar) Lg = load_grad(a, it.left);
         }
+        if (laminar) Lg = load_grad(a, it.left);
       }
       if (it.right >= 0) {
         if (rl >= 0 && rl < staged) {
           R = tile_prim(lds, rl);
-          if (laminar) Rg = tile_grad(lds, rl);
         } else {
           R = load_prim(a, rb, it.right);
-          if (laminar) Rg = load_grad(a, it.right);
         }
+        if (laminar) Rg = load_grad(a, it.right);
       }
       // A boundary face has one cell: it is the inside cell and both sides' record.
       if (it.left < 0) {
