@@ -565,12 +565,13 @@ __device__ __forceinline__ void wall_shear(const CellPrim<T>& donor, T nx, T ny,
 // ------------------------------------------------------------------------------------------
 // Residual of every cell from its faces and wall fragments.  STAGE 0: base residual R0 and the
 // local dt.  1, 2: second and third SSP-RK3 combinations, then the candidate's primitives.
-// FP32: at most 80 VGPRs (6 waves per SIMD; the kernel otherwise takes 84-88, 5 waves): with the
-// LDS of tile 128 that is 6 waves resident, and the stage passes take 200 instead of 215 us on
-// 256 LF2 meshes (8 waves needs 64 VGPRs and spills: slower).  FP64 would spill at 80: no bound.
+// FP32 register budget: 6 waves per SIMD in the base pass (80 VGPRs; it needs 83) and 7 in the
+// stage passes (72 VGPRs, 2 spilled; they need 78), which the LDS of a 128-cell tile also admits.
+// On 256 LF2 meshes: base flux 187 -> 177 us, stage flux 188 -> 181 us against the unbounded
+// kernels; 8 waves (64 VGPRs) spills 11-14 and is slower.  FP64 (190-200 VGPRs) has no bound.
 template <class T, int STAGE>
-__global__ __launch_bounds__(kMaxTile) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1))) void k_flux(
-    ArenaPtrs<T> a) {
+__global__ __launch_bounds__(kMaxTile) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (STAGE == 0 ? 6 : 7) : 1)))
+void k_flux(ArenaPtrs<T> a) {
   constexpr bool BASE = STAGE == 0;
   const int tile = cell_tile(a);
   if (tile < 0) return;
@@ -589,11 +590,7 @@ __global__ __launch_bounds__(kMaxTile) __attribute__((amdgpu_waves_per_eu(sizeof
   const int staged = min(tn, d.cell_begin + d.n_cells - first);  // valid cells of the tile
   const TileCells<T> lds(tn);
   const FaceSlots<T> slots(tn, BASE);
-  CellPrim<T> own{};
-  if (valid) {
-    own = load_prim(a, rb, k);
-    stage_cell(lds, tid, own);
-  }
+  if (valid) stage_cell(lds, tid, load_prim(a, rb, k));
   __syncthreads();
   // Face items of the tile, one chunk at a time: (A) each face touching the tile is evaluated
   // once, by one thread, into its LDS slot; (B) each cell sums its faces' slots in its cell-face
@@ -673,6 +670,9 @@ __global__ __launch_bounds__(kMaxTile) __attribute__((amdgpu_waves_per_eu(sizeof
   ok = true;
   double local_dt = INFINITY;
   if (valid) {
+    // The cell's own record is read back from LDS rather than kept in registers across the
+    // face loop (fewer live registers there).
+    const CellPrim<T> own = tile_prim(lds, tid);
     const T rho = own.rho, p = own.p;
     const int w0 = a.wall_first[k], w1 = w0 + a.wall_cnt[k];
     if (w1 > w0) {
