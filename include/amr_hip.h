@@ -242,8 +242,10 @@ int amr_hip_copy_bandwidth(uint64_t bytes, int32_t repeats, double* gbps, double
  * tile-aligned ranges of the cell, face and wall index spaces and of the ragged tables, sized
  * by its capacity.  Every slot has its own time, target, step size, counters and fatal code,
  * so a slot's trajectory does not depend on which other meshes share the arena (same
- * precision and tile give bitwise the same result as running it alone).  One hipGraph holds
- * one trial of every slot; finished, failed and empty slots are no-ops inside it. */
+ * precision and tile give bitwise the same result as running it alone).  A trial of every
+ * slot runs as up to three independent branches (slot m in branch m % 3), each a hipGraph
+ * replayed on its own stream; the streams join at every poll.  Finished, failed and empty
+ * slots are no-ops inside them. */
 typedef struct amr_batch amr_batch;
 typedef struct amr_slot_capacity {
   int32_t cells;   /* tile multiple, at least n_cells + 1 */
@@ -255,7 +257,7 @@ typedef struct amr_batch_features { int32_t laminar, sa, small, visc_group; } am
 typedef struct amr_batch_options {
   int32_t precision;    /* 64 or 32 */
   int32_t tile;         /* launch tile: 64, 128 or 256 threads per block */
-  int32_t graph;        /* 1: replay a captured hipGraph of one trial of every slot */
+  int32_t graph;        /* 1: replay captured hipGraphs (one per branch) of one trial of every slot */
   int32_t batch_trials; /* trials per host poll, 1..1024 */
 } amr_batch_options;
 typedef struct amr_slot_state {
@@ -315,7 +317,8 @@ int amr_batch_download(amr_batch* b, int32_t slot, double* u, uint64_t u_len, am
  * 3 not evaluated (mask 0, empty or failed slot). */
 int amr_batch_monitor(amr_batch* b, const int32_t* mask, const double* frames, const double* mach,
                       const double* aoa_degrees, double* out, int32_t* status, char* err, size_t err_len);
-/* Per-kernel device time over `trials` directly launched trials (as amr_hip_profile). */
+/* Per-kernel device time over `trials` directly launched trials of the whole arena on one
+ * stream, unbranched (as amr_hip_profile). */
 int amr_batch_profile(amr_batch* b, int32_t trials, amr_kernel_time* out, int32_t capacity, int32_t* count,
                       char* err, size_t err_len);
 void amr_batch_stats(const amr_batch* b, amr_gpu_stats* out);
