@@ -34,8 +34,10 @@ static_assert(sizeof(Control) == 104, "Control layout");
 
 enum : int { VISC_VOLUME = 0, VISC_SRD_RATE = 1, VISC_SRD_GROUP = 2 };
 
-// Face items pack (right cell + 1) into the low bits of one int and the face info above them.
+// Face items pack (right cell + 1) into the low bits of one int, the face info (4 bits) above
+// them and the irregular-geometry flag in bit 31.
 constexpr int kItemRightBits = 27;
+constexpr int kItemInfoMask = 15;
 
 // Face items held in LDS at once per block (a tile of an LF1 / LF2 mesh touches 2.2 - 2.7 faces
 // per cell; a tile with more items is processed in several chunks): 5/2 per cell of the tile in
@@ -103,10 +105,15 @@ struct ArenaPtrs {
   const int* mem_grp;
   const T* mem_inv;      // 1 / N_k
   // Face items: per cell block (tile) the faces with at least one cell in the tile, [x, y) in
-  // ITEM; 8 T words per item: ints {left or -1, (right + 1) | info << kItemRightBits} in words 0-1, then
-  // {length, dx, dy, ex, ey, distance}.  Face info: bit 0 axis, bits 1..3 side + 1 (0 interior).
+  // the item arrays, 4 T words per item in each.  ITEM_A: ints {left or -1, (right + 1) |
+  // info << kItemRightBits | irregular << 31} (words 0-1 in FP32, word 0 in FP64), length, d_n;
+  // ITEM_B: d_t, ex, ey, distance, where d_n / d_t are the centroid offsets along / across the
+  // face's axis.  Face info: bit 0 axis, bits 1..3 side + 1 (0 interior).  A regular item
+  // (d_t = ex = ey = 0 and distance = |d_n|, bit for bit) has its ITEM_B words implied and never
+  // read; a tile lists its irregular items first, so the ITEM_B words it reads lie together.
   const int2* item_range;
-  const T* ITEM;
+  const T* ITEM_A;
+  const T* ITEM_B;
   // walls
   const int* wall_owner;
   const int* wall_donor;

@@ -168,7 +168,7 @@ SlabPlan plan_slab(const Totals& t, int nv) {
   p.geometry = arr(sizeof(MeshDesc<T>), t.meshes) + arr(4, t.cell_blocks) + arr(4, t.grid_blocks) +
                arr(4, t.face_blocks + t.wall_blocks) +
                5 * arr(sizeof(T), t.cells) + 2 * arr(4, t.cells) + 2 * arr(8, t.cells) + 4 * arr(4, t.cells + 1) +
-               arr(2, t.csr) + arr(8 * sizeof(T), t.csr) + arr(4, t.terms) + 2 * arr(sizeof(T), t.terms) +
+               arr(2, t.csr) + 2 * arr(4 * sizeof(T), t.csr) + arr(4, t.terms) + 2 * arr(sizeof(T), t.terms) +
                arr(4, t.groups) + arr(sizeof(T), t.groups) + arr(4, t.members) + arr(8, t.cell_blocks) +
                2 * arr(4, t.walls) + 4 * arr(sizeof(T), t.walls) + 2 * arr(8, t.walls);
   p.control = arr(sizeof(Control), t.meshes) + arr(sizeof(double), t.meshes) + arr(sizeof(long long), t.meshes) +
@@ -264,7 +264,7 @@ struct HostArrays {
   std::vector<double> cell_cx, cell_cy;
   std::vector<int> term_src, grp_mem, mem_grp;
   std::vector<uint16_t> slot_ent;  // per cell-face entry: item index in the cell's tile << 1 | right
-  std::vector<T> items;            // 8 per item (in the cell-face index space)
+  std::vector<T> items_a, items_b;  // 4 + 4 per item (in the cell-face index space)
   std::vector<int> item_range;     // 2 per cell block
   std::vector<T> term_wx, term_wy, grp_w;
   std::vector<int> face_idx;  // host only, for the layout check: 4 per face: left, right, info, 0
@@ -357,7 +357,8 @@ class DeviceArena final : public Arena {
     h.grp_off.assign(nc + 1, 0);
     h.mem_off.assign(nc + 1, 0);
     sized(h.slot_ent, static_cast<size_t>(ncsr), uint16_t(0));
-    sized(h.items, 8 * static_cast<size_t>(std::max(ncsr, 1)), T(0));
+    sized(h.items_a, 4 * static_cast<size_t>(std::max(ncsr, 1)), T(0));
+    sized(h.items_b, 4 * static_cast<size_t>(std::max(ncsr, 1)), T(0));
     h.item_range.assign(2 * static_cast<size_t>(std::max(n_cell_blocks_, 1)), 0);
     sized(h.term_src, nt, -1);
     sized(h.term_wx, nt, T(0));
@@ -562,7 +563,7 @@ class DeviceArena final : public Arena {
         const int e = h.slot_ent[j], i = i0 + (e >> 1);
         if (i >= i1) return "layout: cell-face slot";
         int ids[2], l, r;
-        std::memcpy(ids, &h.items[8 * static_cast<size_t>(i)], sizeof(ids));
+        std::memcpy(ids, &h.items_a[4 * static_cast<size_t>(i)], sizeof(ids));
         l = ids[0];
         r = (ids[1] & ((1 << kItemRightBits) - 1)) - 1;
         if ((e & 1 ? r : l) != k) return "layout: cell-face orientation";
@@ -603,10 +604,17 @@ class DeviceArena final : public Arena {
       prev = i1;
       for (int i = i0; i < i1; ++i) {
         int ids[2];
-        std::memcpy(ids, &h.items[8 * static_cast<size_t>(i)], sizeof(ids));
-        const int l = ids[0], r = (ids[1] & ((1 << kItemRightBits) - 1)) - 1, info = ids[1] >> kItemRightBits;
-        const int side = (info >> 1) - 1;
-        if (ids[1] < 0 || side < -1 || side > 3) return "layout: item info";
+        std::memcpy(ids, &h.items_a[4 * static_cast<size_t>(i)], sizeof(ids));
+        const int l = ids[0], r = (ids[1] & ((1 << kItemRightBits) - 1)) - 1;
+        const int info = (ids[1] >> kItemRightBits) & kItemInfoMask, side = (info >> 1) - 1;
+        if (side < -1 || side > 3) return "layout: item info";
+        // Irregular items first in the tile.
+        const bool irregular = ids[1] < 0;
+        if (irregular && i > i0) {
+          int prev_ids[2];
+          std::memcpy(prev_ids, &h.items_a[4 * static_cast<size_t>(i - 1)], sizeof(prev_ids));
+          if (prev_ids[1] >= 0) return "layout: irregular item after a regular one";
+        }
         if ((l != -1 && !cell_ok(l)) || (r != -1 && !cell_ok(r)) || (l < 0 && r < 0)) return "layout: item cells";
         if (side < 0 && (l < 0 || r < 0)) return "layout: interior item missing a cell";
         if (!((l >= 0 && l / tile_ == b) || (r >= 0 && r / tile_ == b))) return "layout: item outside its tile";
@@ -687,7 +695,8 @@ class DeviceArena final : public Arena {
     for (auto* v : {&h.cell_cx, &h.cell_cy}) col(*v, CELLS);
     for (auto* v : {&h.csr_off, &h.term_off, &h.grp_off, &h.mem_off}) col(*v, OFFSETS);
     col(h.slot_ent, CSR);
-    columns_.push_back({h.items.data(), nullptr, 8 * sizeof(T), h.items.size() / 8, CSR});
+    columns_.push_back({h.items_a.data(), nullptr, 4 * sizeof(T), h.items_a.size() / 4, CSR});
+    columns_.push_back({h.items_b.data(), nullptr, 4 * sizeof(T), h.items_b.size() / 4, CSR});
     col(h.term_src, TERMS);
     col(h.term_wx, TERMS);
     col(h.term_wy, TERMS);
@@ -745,7 +754,8 @@ class DeviceArena final : public Arena {
     a.grp_off = next(static_cast<const int*>(nullptr));
     a.mem_off = next(static_cast<const int*>(nullptr));
     a.SLOT = next(static_cast<const uint16_t*>(nullptr));
-    a.ITEM = next(static_cast<const T*>(nullptr));
+    a.ITEM_A = next(static_cast<const T*>(nullptr));
+    a.ITEM_B = next(static_cast<const T*>(nullptr));
     a.term_src = next(static_cast<const int*>(nullptr));
     a.term_wx = next(static_cast<const T*>(nullptr));
     a.term_wy = next(static_cast<const T*>(nullptr));
@@ -987,9 +997,30 @@ class DeviceArena final : public Arena {
 
   static int face_info(const amr_face& f) { return (f.axis & 1) | ((f.boundary + 1) << 1); }
 
-  // Face items of slot m: per tile, the faces with at least one cell in the tile (interior faces
-  // first, then boundary faces, each group in storage order), and every cell-face entry's index
-  // into its tile's list.  entry_face: the mesh face of each entry, in entry order.
+  // Geometry words of face q as stored (rounded to T): b = {d_t, ex, ey, distance}; returns d_n.
+  // Irregular unless d_t, ex and ey are +0 and distance is |d_n|, bit for bit.
+  static T face_words(const amr_mesh_view& v, int q, bool laminar, T (&b)[4], bool* irregular) {
+    const amr_face& face = v.faces[q];
+    T dn = T(0);
+    b[0] = b[1] = b[2] = b[3] = T(0);
+    if (laminar) {
+      const amr_face_geometry& geo = v.face_geometry[q];
+      const bool xaxis = (face.axis & 1) == 0;
+      dn = static_cast<T>(xaxis ? geo.dx : geo.dy);
+      b[0] = static_cast<T>(xaxis ? geo.dy : geo.dx);
+      b[1] = static_cast<T>(geo.ex);
+      b[2] = static_cast<T>(geo.ey);
+      b[3] = static_cast<T>(face.boundary >= 0 ? v.face_samples[q].distance : geo.distance);
+    }
+    const T implied[4] = {T(0), T(0), T(0), std::fabs(dn)};
+    *irregular = laminar && std::memcmp(b, implied, sizeof(implied)) != 0;
+    return dn;
+  }
+
+  // Face items of slot m: per tile, the faces with at least one cell in the tile (irregular
+  // items first; then interior faces before boundary faces, each group in storage order), and
+  // every cell-face entry's index into its tile's list.  entry_face: the mesh face of each
+  // entry, in entry order.  Euler meshes read no face geometry but the length: all regular.
   void fill_items(int m, const amr_mesh_view& v, const MeshOrder& order, const std::vector<int>& entry_face) {
     const MeshSlot& s = slots_[m];
     HostArrays<T>& h = h_;
@@ -1008,7 +1039,14 @@ class DeviceArena final : public Arena {
             list.push_back(q);
           }
         }
+      for (const int q : list) {
+        T words[4];
+        bool irregular;
+        face_words(v, q, laminar, words, &irregular);
+        slot_of[q] = irregular ? 1 : 0;  // sort key only; replaced by the index below
+      }
       std::sort(list.begin(), list.end(), [&](int x, int y) {
+        if (slot_of[x] != slot_of[y]) return slot_of[x] > slot_of[y];
         const bool bx = v.faces[x].boundary >= 0, by = v.faces[y].boundary >= 0;
         return bx != by ? by : order.face_pos[x] < order.face_pos[y];
       });
@@ -1018,23 +1056,19 @@ class DeviceArena final : public Arena {
         const int q = list[i];
         slot_of[q] = static_cast<int>(i);
         const amr_face& face = v.faces[q];
-        T* w = &h.items[8 * static_cast<size_t>(item++)];
-        std::fill(w, w + 8, T(0));
-        const int ids[2] = {face.left >= 0 ? cb + order.cell_pos[face.left] : -1,
-                            (face.right >= 0 ? cb + order.cell_pos[face.right] + 1 : 0) |
-                                (face_info(face) << kItemRightBits)};
-        std::memcpy(w, ids, sizeof(ids));
-        w[2] = static_cast<T>(face.length);
-        if (laminar) {
-          const amr_face_geometry& geo = v.face_geometry[q];
-          w[3] = static_cast<T>(geo.dx);
-          w[4] = static_cast<T>(geo.dy);
-          w[5] = static_cast<T>(geo.ex);
-          w[6] = static_cast<T>(geo.ey);
-          w[7] = static_cast<T>(face.boundary >= 0 ? v.face_samples[q].distance : geo.distance);
-        } else {
-          w[7] = T(1);
-        }
+        T* wa = &h.items_a[4 * static_cast<size_t>(item)];
+        T* wb = &h.items_b[4 * static_cast<size_t>(item++)];
+        std::fill(wa, wa + 4, T(0));
+        bool irregular;
+        const T dn = face_words(v, q, laminar, *reinterpret_cast<T(*)[4]>(wb), &irregular);
+        const unsigned right_info = static_cast<unsigned>(face.right >= 0 ? cb + order.cell_pos[face.right] + 1 : 0) |
+                                    (static_cast<unsigned>(face_info(face)) << kItemRightBits) |
+                                    (irregular ? 1u << 31 : 0u);
+        const int ids[2] = {face.left >= 0 ? cb + order.cell_pos[face.left] : -1, static_cast<int>(right_info)};
+        std::memcpy(wa, ids, sizeof(ids));
+        wa[2] = static_cast<T>(face.length);
+        wa[3] = dn;
+        if (!irregular) std::fill(wb, wb + 4, T(0));
       }
       h.item_range[2 * tb + 1] = item;
       for (int kk = lo; kk < hi; ++kk) {

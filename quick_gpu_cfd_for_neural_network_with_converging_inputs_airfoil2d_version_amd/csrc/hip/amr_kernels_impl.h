@@ -187,43 +187,36 @@ template <class T>
 struct FaceGeo {
   T len, dx, dy, ex, ey, dist;
 };
-__device__ __forceinline__ FaceGeo<float> load_geo(const float* g) {
-  const float4 a = *reinterpret_cast<const float4*>(g);
-  const float2 b = *reinterpret_cast<const float2*>(g + 4);
-  return FaceGeo<float>{a.x, a.y, a.z, a.w, b.x, b.y};
-}
-__device__ __forceinline__ FaceGeo<double> load_geo(const double* g) {
-  const double2 a = *reinterpret_cast<const double2*>(g), b = *reinterpret_cast<const double2*>(g + 2);
-  const double2 c = *reinterpret_cast<const double2*>(g + 4);
-  return FaceGeo<double>{a.x, a.y, b.x, b.y, c.x, c.y};
-}
 
-// Face item of a tile: 8 T words.  Words 0-1 hold two ints (left cell or -1; right cell + 1
-// in bits 0..kItemRightBits-1 and the face info above); words 2-7 the face geometry.
+// Face item i of a tile: 4 T words of ITEM_A (two ints: left cell or -1; right cell + 1 in bits
+// 0..kItemRightBits-1, the face info above, bit 31 set for irregular geometry; then length and
+// d_n), and for an irregular item 4 T words of ITEM_B (d_t, ex, ey, distance).  A regular item's
+// geometry is d_t = ex = ey = 0, distance = |d_n|: the same bits the host would have stored.
 template <class T>
 struct FaceItem {
   int left, right, info;
   FaceGeo<T> geo;
 };
-__device__ __forceinline__ void unpack_item_ids(int w0, int w1, int& left, int& right, int& info) {
-  left = w0;
-  right = (w1 & ((1 << kItemRightBits) - 1)) - 1;
-  info = w1 >> kItemRightBits;
-}
-__device__ __forceinline__ FaceItem<float> load_item(const float* p) {
-  const int4 a = *reinterpret_cast<const int4*>(p);
-  const float4 b = *reinterpret_cast<const float4*>(p + 4);
-  FaceItem<float> it;
-  unpack_item_ids(a.x, a.y, it.left, it.right, it.info);
-  it.geo = FaceGeo<float>{__int_as_float(a.z), __int_as_float(a.w), b.x, b.y, b.z, b.w};
+template <class T>
+__device__ __forceinline__ FaceItem<T> finish_item(int w0, int w1, T len, T dn, const T* b) {
+  FaceItem<T> it;
+  it.left = w0;
+  it.right = (w1 & ((1 << kItemRightBits) - 1)) - 1;
+  it.info = (w1 >> kItemRightBits) & kItemInfoMask;
+  T dt = T(0), ex = T(0), ey = T(0), dist = fabs(dn);
+  if (w1 < 0) load4(b, dt, ex, ey, dist);
+  const bool xaxis = (it.info & 1) == 0;
+  it.geo = FaceGeo<T>{len, xaxis ? dn : dt, xaxis ? dt : dn, ex, ey, dist};
   return it;
 }
-__device__ __forceinline__ FaceItem<double> load_item(const double* p) {
-  const int4 a = *reinterpret_cast<const int4*>(p);
-  FaceItem<double> it;
-  unpack_item_ids(a.x, a.y, it.left, it.right, it.info);
-  it.geo = load_geo(p + 2);
-  return it;
+__device__ __forceinline__ FaceItem<float> load_item(const float* A, const float* B, size_t i) {
+  const int4 a = *reinterpret_cast<const int4*>(A + 4 * i);
+  return finish_item<float>(a.x, a.y, __int_as_float(a.z), __int_as_float(a.w), B + 4 * i);
+}
+__device__ __forceinline__ FaceItem<double> load_item(const double* A, const double* B, size_t i) {
+  const int4 a = *reinterpret_cast<const int4*>(A + 4 * i);
+  const double2 g = *reinterpret_cast<const double2*>(A + 4 * i + 2);
+  return finish_item<double>(a.x, a.y, g.x, g.y, B + 4 * i);
 }
 
 // U -> cell primitives (computed in FP64 from the FP64 state, stored as T), plus molecular
@@ -603,7 +596,7 @@ void k_flux(ArenaPtrs<T> a) {
   for (int c0 = range.x; c0 < range.y; c0 += chunk) {
     const int c1 = min(range.y, c0 + chunk);
     for (int i = c0 + tid; i < c1; i += tn) {
-      const FaceItem<T> it = load_item(a.ITEM + 8 * static_cast<size_t>(i));
+      const FaceItem<T> it = load_item(a.ITEM_A, a.ITEM_B, static_cast<size_t>(i));
       CellPrim<T> L{}, R{};
       CellGrad<T> Lg{}, Rg{};
       const int ll = it.left - first, rl = it.right - first;
