@@ -727,7 +727,9 @@ void k_flux(ArenaPtrs<T> a) {
       local_dt = static_cast<double>(dt);
     } else if (stage_deltas(d)) {
       // SSP-RK3 on increments: U2 = U + D2 with D2 = (D1 + dt r) / 4, then
-      // U' = U + 2/3 (D2 + dt r), the same combinations as below.
+      // U' = U + 2/3 (D2 + dt r), the same combinations as below.  D1 = T(dt R0) is recomputed
+      // here exactly as the first stage formed it (the base residual is intact), so the first
+      // stage stores no increment.
       const double dt = c->trial_dt;
       const int cur = c->cur;
       const double* u = a.U[cur];
@@ -735,7 +737,8 @@ void k_flux(ArenaPtrs<T> a) {
       double q[5] = {0, 0, 0, 0, 0};
       for (int v = 0; v < d.nv; ++v) {
         const int i = v * nc + k;
-        const double inc = static_cast<double>(a.DU[i]) + dt * static_cast<double>(rv[v]);
+        const T prev = STAGE == 1 ? static_cast<T>(dt * static_cast<double>(a.R0[i])) : a.DU[i];
+        const double inc = static_cast<double>(prev) + dt * static_cast<double>(rv[v]);
         if (STAGE == 1) {
           const T dn = static_cast<T>(0.25 * inc);
           a.DU[i] = dn;
@@ -849,11 +852,10 @@ __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
   if (k >= d.cell_begin + d.n_cells) return;
   const int nc = a.nc, cur = c->cur;
   const double* u = a.U[cur];
-  if (stage_deltas(d)) {  // U1 = U + D1, D1 = dt R0 kept in T
+  if (stage_deltas(d)) {  // U1 = U + D1, D1 = dt R0 in T (not stored: the next stage re-forms it)
     double q[5] = {0, 0, 0, 0, 0};
     for (int v = 0; v < d.nv; ++v) {
       const T dv = static_cast<T>(dt * static_cast<double>(a.R0[v * nc + k]));
-      a.DU[v * nc + k] = dv;
       q[v] = u[v * nc + k] + static_cast<double>(dv);
     }
     CellPrim<T> w;
