@@ -35,6 +35,9 @@ struct amr_batch {
   std::vector<hipStream_t> sides;      // branches 1, 2, ... (branched arenas only)
   hipEvent_t host_done = nullptr;      // recorded on `stream` after host-side work, awaited by `sides`
   bool side_must_wait = false;         // host-side work was queued on `stream` since the last join
+  std::vector<hipEvent_t> joined;      // per side stream: its part of a batch is done
+  Control* snap[2] = {nullptr, nullptr};  // pinned control snapshots of the pipelined advance
+  hipEvent_t snap_done[2] = {nullptr, nullptr};
   std::vector<hipGraph_t> graphs;      // one per branch
   std::vector<hipGraphExec_t> execs;
   Control* host_control = nullptr;  // pinned, n slots
@@ -52,6 +55,11 @@ struct amr_batch {
     if (host_targets) (void)hipHostFree(host_targets);
     if (host_maxsteps) (void)hipHostFree(host_maxsteps);
     if (host_done) (void)hipEventDestroy(host_done);
+    for (hipEvent_t e : joined) (void)hipEventDestroy(e);
+    for (int i = 0; i < 2; ++i) {
+      if (snap_done[i]) (void)hipEventDestroy(snap_done[i]);
+      if (snap[i]) (void)hipHostFree(snap[i]);
+    }
     for (hipStream_t s : sides) (void)hipStreamDestroy(s);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -103,6 +111,48 @@ void fill_states(const amr_batch* b, amr_slot_state* out) {
 
 hipStream_t branch_stream(const amr_batch* b, int j) { return j == 0 ? b->stream : b->sides[j - 1]; }
 
+// One batch of the pipelined advance: batch_trials trials of every branch, then, on the main
+// stream once every branch is done, a copy of all control records into snapshot `k`
+// (snap_done[k] completes with it).  The side streams start the batch only after the work
+// already queued on the main stream (the previous snapshot's copy, host-side writes), so a
+// slot's record is never copied while one of its trials runs.
+int queue_batch(amr_batch* b, int k) {
+  if (!b->sides.empty()) {
+    if (hipEventRecord(b->host_done, b->stream) != hipSuccess) return 1;
+    for (hipStream_t s : b->sides)
+      if (hipStreamWaitEvent(s, b->host_done, 0) != hipSuccess) return 1;
+    b->side_must_wait = false;
+  }
+  const int branches = b->arena->n_branches();
+  for (int trial = 0; trial < b->options.batch_trials; ++trial)
+    for (int j = 0; j < branches; ++j) {
+      hipStream_t s = branch_stream(b, j);
+      if (!b->execs.empty()) {
+        if (hipGraphLaunch(b->execs[j], s) != hipSuccess) return 1;
+      } else {
+        b->arena->launch_trial_branch(j, s);
+      }
+    }
+  for (size_t j = 0; j < b->sides.size(); ++j)
+    if (hipEventRecord(b->joined[j], b->sides[j]) != hipSuccess ||
+        hipStreamWaitEvent(b->stream, b->joined[j], 0) != hipSuccess)
+      return 1;
+  if (hipMemcpyAsync(b->snap[k], b->arena->control(), b->n * sizeof(Control), hipMemcpyDeviceToHost, b->stream) !=
+          hipSuccess ||
+      hipEventRecord(b->snap_done[k], b->stream) != hipSuccess)
+    return 1;
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+// Waits for snapshot k and makes it the host's view of the control records.
+int take_snapshot(amr_batch* b, int k, char* err, size_t err_len) {
+  if (!checked(hipEventSynchronize(b->snap_done[k]), err, err_len, "AMR control snapshot")) return 1;
+  std::copy(b->snap[k], b->snap[k] + b->n, b->host_control);
+  b->stats.control_d2h_bytes += b->n * sizeof(Control);
+  ++b->stats.status_polls;
+  return 0;
+}
+
 // One trial of every slot: each branch on its own stream.  The side streams first wait for
 // host-side work still queued on the main stream (a monitor's control restore).
 int launch_one(amr_batch* b) {
@@ -137,13 +187,21 @@ int finish_setup(amr_batch* b, Clock::time_point t0, char* err, size_t err_len) 
   if (poll(b, err, err_len)) return 1;
   b->stats.allocated_bytes = b->arena->device_bytes();
   b->stats.geometry_h2d_bytes = b->arena->geometry_bytes();
-  b->stats.pinned_bytes = n * (sizeof(Control) + sizeof(double) + sizeof(long long));
+  b->stats.pinned_bytes = n * (3 * sizeof(Control) + sizeof(double) + sizeof(long long));
   const int branches = b->arena->n_branches();
   for (int j = 1; j < branches; ++j) {
     hipStream_t s = nullptr;
+    hipEvent_t e = nullptr;
     if (!checked(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), err, err_len, "AMR branch stream")) return 1;
     b->sides.push_back(s);
+    if (!checked(hipEventCreateWithFlags(&e, hipEventDisableTiming), err, err_len, "AMR branch event")) return 1;
+    b->joined.push_back(e);
   }
+  for (int i = 0; i < 2; ++i)
+    if (!checked(hipHostMalloc(reinterpret_cast<void**>(&b->snap[i]), n * sizeof(Control), hipHostMallocDefault), err,
+                 err_len, "AMR pinned control snapshot") ||
+        !checked(hipEventCreateWithFlags(&b->snap_done[i], hipEventDisableTiming), err, err_len, "AMR snapshot event"))
+      return 1;
   if (branches > 1 &&
       !checked(hipEventCreateWithFlags(&b->host_done, hipEventDisableTiming), err, err_len, "AMR branch event"))
     return 1;
@@ -329,10 +387,15 @@ int amr_batch_advance(amr_batch* b, int32_t until, double wall_budget_seconds, a
   const int group = std::max(1, n_running / 8);
   const int wait_polls = std::min(8, n_running / 32);
   int polls = 0, first_end = -1;
+  // Pipelined: batch k + 1 is queued before the host waits for batch k's snapshot, so the GPU
+  // never waits for the host; on exit the batch in flight completes and its snapshot is the
+  // state returned.  A slot stops at its own target, so the extra batch changes no trajectory.
+  int k = 0;
+  if (n_running && queue_batch(b, k)) return fail(err, err_len, "AMR trial launch failed");
   while (n_running) {
-    for (int trial = 0; trial < b->options.batch_trials; ++trial)
-      if (launch_one(b)) return fail(err, err_len, "AMR trial launch failed");
-    if (!checked(hipGetLastError(), err, err_len, "AMR trial kernels") || poll(b, err, err_len)) return 1;
+    if (queue_batch(b, k ^ 1)) return fail(err, err_len, "AMR trial launch failed");
+    if (take_snapshot(b, k, err, err_len)) return 1;
+    k ^= 1;
     int still = 0;
     for (int m = 0; m < b->n; ++m) {
       const Control& c = b->host_control[m];
@@ -356,6 +419,7 @@ int amr_batch_advance(amr_batch* b, int32_t until, double wall_budget_seconds, a
       break;
     }
   }
+  if (polls && take_snapshot(b, k, err, err_len)) return 1;  // the batch still in flight
   b->stats.advance_seconds += std::chrono::duration<double>(Clock::now() - t0).count();
   fill_states(b, states);
   return expired ? 2 : 0;
