@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import _native as N
+from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import autotune as AT
 from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import batch as B
 from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import serve as S
 
@@ -61,6 +62,17 @@ def test_batch_without_device_raises():
     cap, feat = B.capacity(mesh)
     with pytest.raises(RuntimeError, match="no HIP device"):
         B.Batch.with_slots([cap], feat)
+
+
+def test_autotuner_keeps_the_larger_tile_within_the_tie_margin():
+    # A clear winner is kept whatever its size.
+    assert AT.choose({64: 1.00e-3, 128: 1.10e-3, 256: 1.20e-3}) == 64
+    assert AT.choose({64: 1.20e-3, 128: 1.00e-3, 256: 1.05e-3}) == 128
+    # Within 1% of the fastest the larger tile wins (the measured 256-mesh case: 749.9 / 752.2 us).
+    assert AT.choose({64: 749.9e-6, 128: 752.2e-6, 256: 768.2e-6}) == 128
+    assert AT.choose({64: 1.000e-3, 128: 1.009e-3, 256: 1.0099e-3}) == 256
+    assert AT.choose({64: 1.000e-3, 128: 1.011e-3, 256: 1.02e-3}) == 64
+    assert AT.choose({128: 2e-3}) == 128
 
 
 def test_percentile_and_poisson():
