@@ -111,11 +111,13 @@ void fill_states(const amr_batch* b, amr_slot_state* out) {
 
 hipStream_t branch_stream(const amr_batch* b, int j) { return j == 0 ? b->stream : b->sides[j - 1]; }
 
-// One batch of the pipelined advance: batch_trials trials of every branch, then, on the main
-// stream once every branch is done, a copy of all control records into snapshot `k`
-// (snap_done[k] completes with it).  The side streams start the batch only after the work
-// already queued on the main stream (the previous snapshot's copy, host-side writes), so a
-// slot's record is never copied while one of its trials runs.
+// One batch of the pipelined advance: batch_trials trials of every branch with a running slot
+// in the host's latest view (a slot starts running only through the host, between advances,
+// so a branch without one stays idle), then, on the main stream once every branch is done, a
+// copy of all control records into snapshot `k` (snap_done[k] completes with it).  The side
+// streams start the batch only after the work already queued on the main stream (the previous
+// snapshot's copy, host-side writes), so a slot's record is never copied while one of its
+// trials runs.
 int queue_batch(amr_batch* b, int k) {
   if (!b->sides.empty()) {
     if (hipEventRecord(b->host_done, b->stream) != hipSuccess) return 1;
@@ -124,8 +126,14 @@ int queue_batch(amr_batch* b, int k) {
     b->side_must_wait = false;
   }
   const int branches = b->arena->n_branches();
+  std::vector<char> busy(branches, branches == 1);
+  for (int m = 0; m < b->n && branches > 1; ++m) {
+    const Control& c = b->host_control[m];
+    busy[m % branches] |= !c.finished && !c.fatal;
+  }
   for (int trial = 0; trial < b->options.batch_trials; ++trial)
     for (int j = 0; j < branches; ++j) {
+      if (!busy[j]) continue;
       hipStream_t s = branch_stream(b, j);
       if (!b->execs.empty()) {
         if (hipGraphLaunch(b->execs[j], s) != hipSuccess) return 1;
