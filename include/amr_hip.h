@@ -298,9 +298,12 @@ int amr_batch_set_targets(amr_batch* b, const double* targets, const int64_t* ma
 /* Replay trials until every slot is finished or failed (until = 0), until at least one slot
  * that was running at the call has finished or failed (until = 1), or, grouped (until = 2),
  * until an eighth of those slots have ended or min(8, running / 32) polls have passed since
- * the first ended, polling the control records every batch_trials trials.  Ended slots wait idle, so grouping
- * trades a short wait for fewer checks.  states (may be NULL) receives every slot.  0 done,
- * 2 wall budget expired (0 = no budget), 1 device error. */
+ * the first ended, polling the control records every batch_trials trials.  Ended slots wait
+ * idle, so grouping trades a short wait for fewer checks.  Polls are pipelined: the next
+ * batch_trials trials are already queued when a poll is read, so slots still running when the
+ * condition is met advance up to batch_trials more trials (never past their targets or step
+ * caps) before the call returns.  states (may be NULL) receives every slot as of the return.
+ * 0 done, 2 wall budget expired (0 = no budget), 1 device error. */
 int amr_batch_advance(amr_batch* b, int32_t until, double wall_budget_seconds, amr_slot_state* states, char* err,
                       size_t err_len);
 /* Exactly `trials` trials, then one poll. */

@@ -153,7 +153,8 @@ class Batch:
     def advance(self, until: str = "all", budget: float = 0.0) -> int:
         """Replay trials until all slots finish ("all"), any running slot ends ("any"), or an eighth
         of the running slots have ended / min(8, running // 32) polls have passed since the first
-        ended ("group"); 0 done, 2 budget expired."""
+        ended ("group"); 0 done, 2 budget expired.  Polls are pipelined: slots still running when
+        the condition is seen take up to batch_trials more trials (never past their targets)."""
         err = N.err_buffer()
         mode = {"all": 0, "any": 1, "group": 2}[until]
         return self._check(self._lib.amr_batch_advance(self.handle, mode, budget, self._state_buf, err, N.ERR_LEN),
