@@ -34,6 +34,11 @@ static_assert(sizeof(Control) == 104, "Control layout");
 
 enum : int { VISC_VOLUME = 0, VISC_SRD_RATE = 1, VISC_SRD_GROUP = 2 };
 
+// Words per cell of the PREC vectors (p, mu, eddy viscosity): unpadded in FP32 (12 bytes, one
+// dwordx3 access), padded to two 16-byte halves in FP64.
+template <class T>
+__host__ __device__ constexpr int prec_width() { return sizeof(T) == 4 ? 3 : 4; }
+
 // Face items pack (right cell + 1) into the low bits of one int, the face info (4 bits) above
 // them and the irregular-geometry flag in bit 31.
 constexpr int kItemRightBits = 27;
@@ -137,8 +142,9 @@ struct ArenaPtrs {
   double* QHAT;  // SRD group averages, 4 planes
   T* R0;
   // Cell vectors gathered by faces and gradient stencils (AoS per purpose, so that a gather
-  // costs whole 16-byte loads): VREC = rho, u, v, nutilde and PREC = p, mu, eddy viscosity, -
-  // (three buffers rotated by stage, Control::wbuf holding U[cur]'s), GREC = ux, uy, vx, vy,
+  // costs whole vector loads): VREC = rho, u, v, nutilde and PREC = p, mu, eddy viscosity
+  // (prec_width T per cell: 3 in FP32, a 12-byte load; 4 in FP64, two 16-byte loads) (three
+  // buffers rotated by stage, Control::wbuf holding U[cur]'s), GREC = ux, uy, vx, vy,
   // nutilde_x, nutilde_y, SA diffusion, SA source (per stage).
   T* VREC[3];
   T* PREC[3];

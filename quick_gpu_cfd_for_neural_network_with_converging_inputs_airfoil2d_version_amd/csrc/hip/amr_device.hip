@@ -158,7 +158,7 @@ Totals totals_of(const std::vector<SlotCapacity>& caps, int tile) {
 // least one element and starts on a 256-byte boundary.
 struct SlabPlan {
   size_t geometry, control, work, total;
-  size_t cplane, dplane, vec4, vec8, mon_cell, mon_wall;
+  size_t cplane, dplane, vec4, prec, vec8, mon_cell, mon_wall;
 };
 
 template <class T>
@@ -176,14 +176,15 @@ SlabPlan plan_slab(const Totals& t, int nv) {
   p.cplane = arr(sizeof(T), t.cells);
   p.dplane = arr(sizeof(double), t.cells);
   p.vec4 = arr(4 * sizeof(T), t.cells);
+  p.prec = arr(prec_width<T>() * sizeof(T), t.cells);
   p.vec8 = arr(8 * sizeof(T), t.cells);
   p.mon_cell = arr(6 * sizeof(double), t.cell_blocks);
   p.mon_wall = arr(3 * sizeof(double), t.wall_blocks);
-  // FP64 state U[2], U2 (nv planes each) and QHAT (4); cell vectors VREC[3], PREC[3] (4 T)
-  // and GREC (8 T); T planes R0 (nv), SRATE, CELL_DT, DRATE, DGROUP, and with FP32 work
+  // FP64 state U[2], U2 (nv planes each) and QHAT (4); cell vectors VREC[3] (4 T), PREC[3]
+  // (3 T in FP32, 4 in FP64) and GREC (8 T); T planes R0 (nv), SRATE, CELL_DT, DRATE, DGROUP, and with FP32 work
   // precision the RK stage increments DU (nv); monitor partials.
   const size_t state_planes = 3 * nv + 4, cell_planes = nv + 4 + (sizeof(T) == 4 ? nv : 0);
-  p.work = state_planes * p.dplane + 6 * p.vec4 + p.vec8 + cell_planes * p.cplane + p.mon_cell + p.mon_wall;
+  p.work = state_planes * p.dplane + 3 * p.vec4 + 3 * p.prec + p.vec8 + cell_planes * p.cplane + p.mon_cell + p.mon_wall;
   p.total = p.geometry + p.control + p.work;
   return p;
 }
@@ -714,7 +715,7 @@ class DeviceArena final : public Arena {
     const SlabPlan plan = plan_slab<T>(totals_, nv);
     if (geo != plan.geometry) return "AMR arena size bookkeeping";
     const int nc = std::max(nc_, 1), nf = std::max(nf_, 1), nw = std::max(nw_, 1);
-    const size_t cplane = plan.cplane, dplane = plan.dplane, vec4 = plan.vec4, vec8 = plan.vec8;
+    const size_t cplane = plan.cplane, dplane = plan.dplane, vec4 = plan.vec4, prec = plan.prec, vec8 = plan.vec8;
     const size_t mon_cell = plan.mon_cell, mon_wall = plan.mon_wall;
     slab_bytes_ = plan.total;
     geometry_bytes_ = geo;
@@ -797,7 +798,7 @@ class DeviceArena final : public Arena {
     a.U2 = dplanes(nv);
     a.QHAT = dplanes(4);
     for (int b = 0; b < 3; ++b) a.VREC[b] = plane(vec4, 1);
-    for (int b = 0; b < 3; ++b) a.PREC[b] = plane(vec4, 1);
+    for (int b = 0; b < 3; ++b) a.PREC[b] = plane(prec, 1);
     a.GREC = plane(vec8, 1);
     a.R0 = plane(cplane, nv);
     a.SRATE = plane(cplane, 1);

@@ -99,6 +99,23 @@ __device__ __forceinline__ void load4(const double* p, double& a, double& b, dou
   const double2 v = *reinterpret_cast<const double2*>(p), w = *reinterpret_cast<const double2*>(p + 2);
   a = v.x, b = v.y, c = w.x, d = w.y;
 }
+// PREC records: 12-byte loads and stores in FP32, the first three of 4 padded words in FP64.
+__device__ __forceinline__ void load3(const float* p, float& a, float& b, float& c) {
+  const float3 v = *reinterpret_cast<const float3*>(p);
+  a = v.x, b = v.y, c = v.z;
+}
+__device__ __forceinline__ void load3(const double* p, double& a, double& b, double& c) {
+  const double2 v = *reinterpret_cast<const double2*>(p);
+  a = v.x, b = v.y, c = p[2];
+}
+__device__ __forceinline__ void store3(float* p, float a, float b, float c) {
+  *reinterpret_cast<float3*>(p) = make_float3(a, b, c);
+}
+__device__ __forceinline__ void store3(double* p, double a, double b, double c) {
+  *reinterpret_cast<double2*>(p) = make_double2(a, b);
+  p[2] = c;
+  p[3] = 0.0;
+}
 __device__ __forceinline__ void store4(float* p, float a, float b, float c, float d) {
   *reinterpret_cast<float4*>(p) = make_float4(a, b, c, d);
 }
@@ -116,9 +133,8 @@ __device__ __forceinline__ CellVel<T> load_vel(const ArenaPtrs<T>& a, int buf, i
 template <class T>
 __device__ __forceinline__ CellPrim<T> load_prim(const ArenaPtrs<T>& a, int buf, int k) {
   CellPrim<T> w;
-  T pad;
   load4(a.VREC[buf] + 4 * k, w.rho, w.u, w.v, w.phi);
-  load4(a.PREC[buf] + 4 * k, w.p, w.mu, w.eddy, pad);
+  load3(a.PREC[buf] + prec_width<T>() * k, w.p, w.mu, w.eddy);
   return w;
 }
 template <class T>
@@ -132,7 +148,7 @@ __device__ __forceinline__ CellGrad<T> load_grad(const ArenaPtrs<T>& a, int k) {
 template <class T>
 __device__ __forceinline__ void store_prim(const ArenaPtrs<T>& a, int buf, int k, const CellPrim<T>& w) {
   store4(a.VREC[buf] + 4 * k, w.rho, w.u, w.v, w.phi);
-  store4(a.PREC[buf] + 4 * k, w.p, w.mu, w.eddy, T(0));
+  store3(a.PREC[buf] + prec_width<T>() * k, w.p, w.mu, w.eddy);
 }
 template <class T>
 // The gradient vector's 8th slot carries the cell's SA-neg source (read by the cell itself only).
