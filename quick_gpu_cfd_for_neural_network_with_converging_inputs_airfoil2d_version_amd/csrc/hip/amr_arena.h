@@ -2,8 +2,9 @@
 //
 // Every mesh of an arena owns block-aligned ranges of a global cell, face and wall index
 // space (each range padded to the launch tile, so a thread block never spans two meshes).
-// All index arrays hold global ids.  Per-element data are structure-of-arrays planes of one
-// width per array: cell planes have stride nc, face planes nf, wall planes nw.
+// All index arrays hold global ids.  Per-element data are structure-of-arrays planes (cell
+// planes have stride nc, face planes nf, wall planes nw), except what kernels gather from
+// neighbours, which is kept as short per-element vectors (VREC, PREC, GREC, face items).
 //
 // Per mesh, an on-device Control drives the SSP-RK3 trial state machine of the reference
 // (src/static_amr2d_cuda.cu:17-21,51-59,170-187,212-216): time, target, dt, step and

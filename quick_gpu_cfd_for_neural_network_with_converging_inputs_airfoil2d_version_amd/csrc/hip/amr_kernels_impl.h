@@ -15,10 +15,11 @@
 //            viscosity and eddy viscosity.
 // A face shared by two tiles is evaluated by both with the same inputs and operations, so both
 // see the same flux bits; no flux array is written or re-read.  Cell data that faces and
-// gradient stencils gather lives in 4- and 8-wide vectors per cell (AoS per purpose), so one
-// neighbour costs whole 16-byte loads; each block stages its own tile's vectors in LDS.
-// Primitive vectors are double-buffered by stage: a stage reads one buffer and writes the next
-// state's into the other.
+// gradient stencils gather lives in short vectors per cell (AoS per purpose: 4 T, 3 T in FP32
+// for p / mu / eddy viscosity, 8 T of gradients), so one neighbour costs whole vector loads;
+// each flux block stages its own tile's primitive vectors in LDS.  Primitive vectors rotate
+// through three buffers: a stage reads one and writes the next state's into another, and the
+// base state's survive a rejected trial.
 // Plus k_base_prim, k_stage0 (dt choice + first stage, reference :170-191), the SRD pair
 // k_group / k_redist (reference :195-207), the SRD-grouped viscous limit (reference :164-176)
 // and k_commit (accept / reject / begin, reference :54-59,212-216).
