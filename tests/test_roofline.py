@@ -29,3 +29,30 @@ def test_design_bytes_exceed_contract_by_the_fp64_state():
     d, c = RL.design_stage_bytes(mesh, 32), RL.contract_stage_bytes(mesh, 32)
     assert d - c == 4 * 3 * 5 * n
     assert RL.design_stage_bytes(mesh, 64) == RL.contract_stage_bytes(mesh, 64)
+
+
+def test_bench_reads_the_profile_summary_of_its_own_workload_only():
+    # bench.py reports measured HBM traffic only from the committed rocprofv3 summary whose
+    # workload line matches (batch, tile, seed); the summaries carry the figure they state.
+    import importlib.util
+    import re
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parents[1]
+    spec = importlib.util.spec_from_file_location("bench_module", root / "bench.py")
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    for path in sorted((root / "profiles").glob("stage_pass_*.md")):
+        text = path.read_text()
+        w = re.search(r"^stage_pass_profile: batch=(\d+) tile=(\d+) seed=(\d+)$", text, re.M)
+        if not w:
+            continue
+        batch, tile, seed = (int(x) for x in w.groups())
+        stated = int(re.search(r"^stage_pass_hbm_bytes: (\d+)$", text, re.M).group(1))
+        assert bench.profiled_traffic(batch, tile, seed) == stated
+        # The summary's MB figure is the same measurement.
+        mb = float(re.search(r"Measured HBM traffic: ([\d.]+) MB per stage pass", text).group(1))
+        assert abs(mb - stated / 1e6) < 0.1
+    assert bench.profiled_traffic(256, 128, 1) is None
+    assert bench.profiled_traffic(255, 128, 2026) is None
+    assert bench.profiled_traffic(256, 256, 2026) is None
