@@ -69,6 +69,39 @@ def test_batch_matches_alone_bitwise(precision):
                 assert max(per_variable_l2(u, ref, 4)) <= 1e-10
 
 
+def test_poll_cadence_changes_no_trajectory():
+    """Polls are pipelined (the next batch of trials is queued before the host reads the last),
+    so a call returns after some slots have taken extra trials; slots stop at their own
+    targets, so many short calls ("any", one trial per poll) and one long call ("all", 32 per
+    poll) give the same bits, times and counters, and no slot passes its target."""
+    items = four_variable_meshes()
+    meshes, params, states = [m for _, m, _, _ in items], [p for _, _, p, _ in items], [u for _, _, _, u in items]
+
+    def run(until, trials):
+        with B.Batch.of_meshes(meshes, params, states, precision=32, batch_trials=trials) as b:
+            b.set_targets(0.01)
+            calls = 0
+            while True:
+                running = [not (s["finished"] or s["fatal"]) for s in b.states()]
+                if not any(running):
+                    break
+                assert b.advance(until) == 0
+                calls += 1
+                now = b.states()
+                assert all(s["time"] <= s["target"] for s in now)
+                if until == "any":  # at least one slot that was running has ended
+                    assert any(r and (s["finished"] or s["fatal"]) for r, s in zip(running, now))
+            return [b.download(k) for k in range(len(meshes))], calls
+
+    ref, ref_calls = run("all", 32)
+    got, calls = run("any", 1)
+    print("calls", ref_calls, calls, [meta for _, meta in ref], flush=True)
+    assert ref_calls == 1 and calls >= 2
+    for (u, meta), (ru, rmeta) in zip(got, ref):
+        assert meta == rmeta
+        assert np.array_equal(u, ru)
+
+
 def test_sa_batch_matches_alone_bitwise():
     items = sa_meshes()
     refs = [alone(m, p, u, [0.01], 64) for _, m, p, u in items]
