@@ -10,7 +10,7 @@ if str(ROOT) not in sys.path:
 
 
 def pytest_configure(config):
-    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device); run on the GPU box")
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
 
 
 def pytest_collection_modifyitems(config, items):
