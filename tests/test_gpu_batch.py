@@ -115,6 +115,34 @@ def test_sa_batch_matches_alone_bitwise():
             assert np.array_equal(u, ref_u)
 
 
+@pytest.mark.parametrize("tile", [128, 256])
+def test_fp32_sa_branches_at_production_tiles_match_single_slot_bitwise(tile):
+    """The benchmark's configuration: FP32 SA-neg LF1 / LF2 meshes at tile 128 (and 256), four
+    slots in three trial branches (branch 0 holds two).  Each mesh must give the bits it gets in
+    a one-slot arena at the same tile (sessions use their own tile, so they are not the
+    reference here)."""
+    items = []
+    for src, preset in (("G0002", "LF1"), ("G0100", "LF2"), ("G0001", "LF2"), ("G0010", "LF1")):
+        case, mesh, params = case_mesh(src, preset)
+        items.append((mesh, params, mesh.initialize(params, case.freestream)))
+    refs = []
+    for mesh, params, u0 in items:
+        with B.Batch.of_meshes([mesh], [params], [u0], precision=32, tile=tile) as b:
+            b.set_targets(0.01)
+            assert b.advance("all") == 0
+            refs.append(b.download(0))
+    with B.Batch.of_meshes([m for m, _, _ in items], [p for _, p, _ in items], [u for _, _, u in items],
+                           precision=32, tile=tile) as b:
+        assert b.stats()["graph_nodes_per_trial"] > 0
+        b.set_targets(0.01)
+        assert b.advance("all") == 0
+        for k, (ref_u, ref_meta) in enumerate(refs):
+            u, meta = b.download(k)
+            print(tile, k, meta, flush=True)
+            assert meta == ref_meta
+            assert np.array_equal(u, ref_u)
+
+
 def test_refill_and_release_keep_neighbours_bitwise():
     items = four_variable_meshes()
     (ca, ma, pa, ua), (cb, mb, pb, ub), (cc, mc, pc, uc) = items
