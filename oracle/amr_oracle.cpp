@@ -872,6 +872,7 @@ void oracle_flux(int kind, const double* left, const double* right, double gamma
 int oracle_boundary_state(const amr_boundary* b, int legacy, double gamma, int side, const double* wi, double* wb) {
   return boundary_state(*b, legacy, gamma, side, wi, wb) ? 1 : 0;
 }
+int oracle_boundary_valid(const amr_boundary* b, int legacy, int side) { return boundary_valid(*b, legacy, side) ? 1 : 0; }
 // out = {fv1, modified_vorticity, fw, source, source_rate, diffusion, eddy_kinematic}
 void oracle_sa_point(double working, double molecular, double distance, double omega, double* out) {
   const SaPoint a = sa_point(working, molecular, distance, omega);

@@ -53,6 +53,8 @@ def oracle_lib():
         lib.oracle_boundary_state.restype = C.c_int
         lib.oracle_boundary_state.argtypes = [C.POINTER(N.amr_boundary), C.c_int, C.c_double, C.c_int,
                                               N.c_double_p, N.c_double_p]
+        lib.oracle_boundary_valid.restype = C.c_int
+        lib.oracle_boundary_valid.argtypes = [C.POINTER(N.amr_boundary), C.c_int, C.c_int]
         lib.oracle_sa_point.restype = None
         lib.oracle_sa_point.argtypes = [C.c_double] * 4 + [N.c_double_p]
         lib.oracle_viscosity.restype = C.c_double
@@ -85,6 +87,11 @@ def ref_lib():
         lib.ref_redistribute.argtypes = [H, N.c_double_p, C.c_uint64]
         lib.ref_wall_flux.restype = C.c_int
         lib.ref_wall_flux.argtypes = [H, P, N.c_double_p, C.c_uint64, N.c_double_p, C.c_uint64]
+        lib.ref_boundary_state.restype = C.c_int
+        lib.ref_boundary_state.argtypes = [C.POINTER(N.amr_boundary), C.c_int, C.c_double, C.c_int, N.c_double_p,
+                                           N.c_double_p]
+        lib.ref_boundary_valid.restype = C.c_int
+        lib.ref_boundary_valid.argtypes = [C.POINTER(N.amr_boundary), C.c_int, C.c_int]
         _ref = lib
     return _ref
 
@@ -150,6 +157,10 @@ def boundary_state(boundary, legacy: int, gamma: float, side: int, interior):
     return bool(ok), wb
 
 
+def boundary_valid(boundary, legacy: int, side: int) -> bool:
+    return bool(oracle_lib().oracle_boundary_valid(C.byref(boundary), legacy, side))
+
+
 def sa_point(working, molecular, distance, omega) -> dict:
     out = np.zeros(7)
     oracle_lib().oracle_sa_point(working, molecular, distance, omega, N.dptr(out))
@@ -171,9 +182,10 @@ def ref_rhs(ref_mesh, params, u: np.ndarray) -> np.ndarray:
     return r
 
 
-def ref_run(ref_mesh, params, u: np.ndarray, tend: float, maxsteps: int = 50_000_000, time: float = 0.0):
+def ref_run(ref_mesh, params, u: np.ndarray, tend: float, maxsteps: int = 50_000_000, time: float = 0.0,
+            steps: int = 0, rejected: int = 0):
     u = np.array(u, dtype=np.float64, copy=True)
-    meta = N.amr_state_meta(time, 0, 0)
+    meta = N.amr_state_meta(time, steps, rejected)
     err = N.err_buffer()
     status = ref_lib().ref_run(ref_mesh.handle, C.byref(params), tend, maxsteps, N.dptr(u), u.size, C.byref(meta),
                                err, N.ERR_LEN)
@@ -193,3 +205,14 @@ def ref_wall_flux(ref_mesh, params, u: np.ndarray, n_walls: int) -> np.ndarray:
     if ref_lib().ref_wall_flux(ref_mesh.handle, C.byref(params), N.dptr(u), u.size, N.dptr(out), out.size):
         raise OracleError("reference viscous diagnostics failed")
     return out
+
+
+def ref_boundary_state(boundary, legacy: int, gamma: float, side: int, interior):
+    wi = np.ascontiguousarray(interior, dtype=np.float64)
+    wb = np.zeros(4)
+    ok = ref_lib().ref_boundary_state(C.byref(boundary), legacy, gamma, side, N.dptr(wi), N.dptr(wb))
+    return bool(ok), wb
+
+
+def ref_boundary_valid(boundary, legacy: int, side: int) -> bool:
+    return bool(ref_lib().ref_boundary_valid(C.byref(boundary), legacy, side))

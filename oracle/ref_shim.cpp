@@ -8,11 +8,14 @@
 //   ref_run          static_amr2d_run        (src/static_amr2d.cpp:384-432)
 //   ref_redistribute static_amr2d_redistribute (src/static_amr2d.cpp:379-382)
 //   ref_wall_flux    static_amr2d_viscous_diagnostics (src/static_amr2d_viscous.cpp:137-142)
+//   ref_boundary_state / ref_boundary_valid
+//                    amr_bc_state / amr_bc_valid (src/static_amr2d_boundary.h:32-102)
 #include <cstring>
 #include <string>
 
 #include "amr_hip.h"
 #include "static_amr2d.h"
+#include "static_amr2d_boundary.h"
 #include "static_amr2d_sa.h"
 
 using namespace cfd;
@@ -223,6 +226,20 @@ int ref_wall_flux(const ref_mesh* m, const amr_params* params, const double* u, 
   if (fields.wall_flux.size() != len) return 2;
   std::memcpy(wall_flux, fields.wall_flux.data(), len * sizeof(double));
   return 0;
+}
+
+// Exterior Riemann state of one side (1 = valid state, 0 = physical/configuration failure).
+int ref_boundary_state(const amr_boundary* b, int legacy, double gamma, int side, const double* wi, double* wb) {
+  static_assert(sizeof(amr_boundary) == sizeof(AmrBoundary2D), "boundary record layout");
+  AmrBoundary2D rb;
+  std::memcpy(&rb, b, sizeof(rb));
+  return amr_bc_state(rb, legacy, gamma, side, wi, wb) ? 1 : 0;
+}
+
+int ref_boundary_valid(const amr_boundary* b, int legacy, int side) {
+  AmrBoundary2D rb;
+  std::memcpy(&rb, b, sizeof(rb));
+  return amr_bc_valid(rb, legacy, side) ? 1 : 0;
 }
 
 }  // extern "C"

@@ -19,7 +19,7 @@ DEFAULTS = {
                    "outlet_pressure_Pa": 18754.0},
     "physics": {"model": "laminar", "turbulence": "sa-neg", "viscosity": "sutherland", "constant_mu_Pa_s": None,
                 "sutherland": {"mu0_Pa_s": 1.716e-05, "T0_K": 273.15, "S_K": 110.4},
-                "sa_nu_inf_over_nu": 3.0, "wall_probe_min_h": 0.25},
+                "sa_nu_inf_over_nu": 3.0, "wall_probe_min_h": 0.25, "viscous_stabilization": "volume"},
     "solver": {"backend": "hip", "flux": "hllc", "CFL": 0.15, "viscous_CFL": 0.1, "SA_CFL": 0.1,
                "time_step": {"mode": "cfl", "dt_seconds": None}, "mode": "steady", "stop_time_star": 100.0,
                "budget_seconds": 118.0, "moment_reference_xy_c": [0.25, 0.0], "precision": 64,
@@ -50,6 +50,11 @@ PRESETS = {
     "L8": {"levels": 8, "boxes": [[-0.5, 2.0, -0.75, 0.75, 6], [1.0, 4.0, -0.35, 0.65, 5], [-2.0, 6.0, -3.0, 3.0, 3],
                                   [2.0, 3.25, -0.35, 0.65, 6]]},
 }
+
+# Viscous time-step limit (reference AmrViscousStabilization, src/static_amr2d_viscous.h:14, and
+# the static-AMR command line's --viscous-stabilization, src/static_amr2d_cli.cpp:113): "srd" maps
+# the diffusion rate through the SRD neighbourhoods of a non-agglomerated mesh.
+STABILIZATION = {"volume": 0, "srd": 1}
 
 BC_KINDS = {"transmissive": 1, "reflect": 2, "farfield": 3, "total-inlet": 4, "pressure-outlet": 5, "no-slip": 6}
 
@@ -110,6 +115,7 @@ class Case:
     convergence: dict = field(default_factory=dict)
     moment_reference: tuple = (0.25, 0.0)
     stop_time: float = 100.0
+    stabilization: int = 0  # viscous dt limit: 0 true volume, 1 SRD-mapped diffusion rate
 
     @property
     def freestream(self) -> tuple:
@@ -146,4 +152,4 @@ def make_case(preset: str = "LF2", overrides: dict | None = None) -> Case:
                           max_cells=m["max_cells"], agglomerate_small=bool(m["agglomerate_small"]),
                           boxes=[list(x) for x in profile["boxes"]]),
         convergence=dict(s["convergence"]), moment_reference=tuple(s["moment_reference_xy_c"]),
-        stop_time=s["stop_time_star"])
+        stop_time=s["stop_time_star"], stabilization=STABILIZATION[p["viscous_stabilization"]])

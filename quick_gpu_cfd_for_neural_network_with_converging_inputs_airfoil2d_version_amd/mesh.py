@@ -29,9 +29,16 @@ def params_of(case: Case) -> N.amr_params:
     return p
 
 
-def boundaries_of(case: Case) -> "C.Array":
+SIDE_FIELDS = ("kind", "reference", "total_pressure", "total_temperature", "gas_constant", "direction",
+               "outlet_pressure", "wall_velocity")
+
+
+def boundaries_of(case: Case, sides: dict | None = None) -> "C.Array":
     """Side boundaries: freestream reference state and outlet pressure on all four sides
-    (src/quick_airfoil.cpp:45-46)."""
+    (src/quick_airfoil.cpp:45-46).  `sides` maps a side (0 xmin, 1 xmax, 2 ymin, 3 ymax) to
+    nondimensional field overrides, the settings the reference's static-AMR command line takes
+    per side (src/static_amr2d_cli.cpp:149-170: kind, reference state, total pressure /
+    temperature / gas constant / direction of a total inlet, outlet pressure, wall velocity)."""
     arr = (N.amr_boundary * 4)()
     ref = case.freestream
     for side in range(4):
@@ -43,6 +50,17 @@ def boundaries_of(case: Case) -> "C.Array":
         b.direction[0], b.direction[1] = 1.0, 0.0
         b.outlet_pressure = case.outlet_p_star
         b.wall_velocity[0], b.wall_velocity[1] = 0.0, 0.0
+    for side, fields in (sides or {}).items():
+        b = arr[int(side)]
+        for name, value in fields.items():
+            if name not in SIDE_FIELDS:
+                raise KeyError(f"unknown boundary field {name}")
+            if name in ("reference", "direction", "wall_velocity"):
+                target = getattr(b, name)
+                for i, x in enumerate(value):
+                    target[i] = float(x)
+            else:
+                setattr(b, name, int(value) if name == "kind" else float(value))
     return arr
 
 
@@ -51,7 +69,7 @@ def transport_of(case: Case) -> N.amr_transport:
     t = N.amr_transport()
     t.model = case.model
     t.law = case.law if case.model else 0
-    t.stabilization = 0
+    t.stabilization = case.stabilization if case.model else 0
     t.mu_ref = case.mu_star if case.model else 0.0
     t.gas_constant = 1.0
     t.temperature_ref = 1.0 / case.gamma if case.law == 1 else 1.0
@@ -109,8 +127,8 @@ class Mesh:
             raise MeshError(err.value.decode())
         return cls(lib, prefix, handle)
 
-    def set_physics(self, case: Case):
-        b, t = boundaries_of(case), transport_of(case)
+    def set_physics(self, case: Case, sides: dict | None = None):
+        b, t = boundaries_of(case, sides), transport_of(case)
         if self._fn("set_physics")(self.handle, b, C.byref(t)):
             raise MeshError("set_physics failed")
         self.case = case
@@ -196,9 +214,10 @@ class Mesh:
 
 
 def build_case_mesh(case: Case, polygon: np.ndarray, patches: np.ndarray | None, lib=None,
-                    prefix: str = "amr_mesh") -> Mesh:
-    """Mesh + physics + viscous tables for one case, as the solver executable sets them up."""
+                    prefix: str = "amr_mesh", sides: dict | None = None) -> Mesh:
+    """Mesh + physics + viscous tables for one case, as the solver executable sets them up
+    (`sides`: per-side boundary overrides, see boundaries_of)."""
     mesh = Mesh.build(polygon, patches, case.mesh_options, lib=lib, prefix=prefix)
-    mesh.set_physics(case)
+    mesh.set_physics(case, sides)
     mesh.prepare_viscous(params_of(case))
     return mesh

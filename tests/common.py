@@ -43,13 +43,13 @@ def merge(*parts):
     return out
 
 
-def case_mesh(name_or_polygon, preset="LF1", overrides=None, lib=None, prefix="amr_mesh"):
+def case_mesh(name_or_polygon, preset="LF1", overrides=None, lib=None, prefix="amr_mesh", sides=None):
     case = CF.make_case(preset, overrides)
     if isinstance(name_or_polygon, str):
         poly, patches = example_polygon(name_or_polygon)
     else:
         poly, patches = name_or_polygon
-    mesh = M.build_case_mesh(case, poly, patches, lib=lib, prefix=prefix)
+    mesh = M.build_case_mesh(case, poly, patches, lib=lib, prefix=prefix, sides=sides)
     return case, mesh, M.params_of(case)
 
 
