@@ -19,6 +19,8 @@ CSRC = PACKAGE / "csrc"
 OUT = PACKAGE / "libamr_hip.so"
 DRIVER = PACKAGE / "quick_airfoil_hip"
 DRIVER_SOURCE = "host/quick_airfoil_hip.cpp"
+CALLER_SOURCE = ROOT / "tests" / "capi" / "api_caller.cpp"
+CALLER = ROOT / "tests" / "capi" / "api_caller"
 OBJ = ROOT / "_build" / "obj"
 
 HOST_SOURCES = ["host/cutcell.cpp", "host/mesher.cpp", "host/viscous_prep.cpp", "host/mesh_capi.cpp",
@@ -80,6 +82,14 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     _run(["g++", str(driver_obj), "-L", str(PACKAGE), "-l:libamr_hip.so", "-L", "/opt/rocm/lib",
           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", str(tmp)], verbose)
     tmp.replace(DRIVER)
+    # The test suite's caller of the C++ backend API (a reference-style caller of
+    # static_amr2d_gpu.h linked against the library), when the tests are present.
+    if CALLER_SOURCE.exists():
+        tmp = CALLER.with_suffix(".tmp")
+        _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", str(ROOT / "include"), str(CALLER_SOURCE), "-L", str(PACKAGE),
+              "-l:libamr_hip.so", "-L", "/opt/rocm/lib", f"-Wl,-rpath,$ORIGIN/../../{PACKAGE.name}",
+              "-Wl,-rpath,/opt/rocm/lib", "-o", str(tmp)], verbose)
+        tmp.replace(CALLER)
     return OUT
 
 

@@ -42,9 +42,15 @@ enum : int { PASS_BASE = 0, PASS_STAGE = 1 };
 
 __device__ __forceinline__ bool running(const Control* c) { return c->active && !c->fatal; }
 __device__ __forceinline__ bool healthy(const Control* c) { return running(c) && !c->bad; }
-__device__ __forceinline__ bool pass_on(const Control* c, int pass) {
-  return healthy(c) && (pass == PASS_STAGE || !c->r0_valid);
+// Whether a pass runs for a slot.  `active`, `fatal` and `r0_valid` change between the launches
+// of a pass's kernels only (every kernel that uses barriers), while `bad` is raised by other
+// blocks during a launch: pass_live leaves it out, and a kernel whose waves meet at barriers
+// takes `bad` from one snapshot per block (taken by thread 0 before its first barrier), so its
+// waves return or go on together and never read LDS that other waves left unstaged.
+__device__ __forceinline__ bool pass_live(const Control* c, int pass) {
+  return running(c) && (pass == PASS_STAGE || !c->r0_valid);
 }
+__device__ __forceinline__ bool pass_on(const Control* c, int pass) { return pass_live(c, pass) && !c->bad; }
 // Admissibility bits: 1 = the base pass failed (fatal 2 at the dt choice), 2 = a candidate
 // stage failed (the trial is rejected).  Separate bits keep the dt choice race-free.
 enum : int { BAD_BASE = 1, BAD_STAGE = 2 };
@@ -322,9 +328,11 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
   if (tile < 0) return;
   const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
-  if (!pass_on(c, STAGE == 0 ? PASS_BASE : PASS_STAGE)) return;
+  if (!pass_live(c, STAGE == 0 ? PASS_BASE : PASS_STAGE)) return;
   const MeshDesc<T>& d = a.desc[m];
   if (!d.laminar) return;
+  __shared__ int failed;  // block snapshot of the slot's admissibility bits
+  if (threadIdx.x == 0) failed = c->bad;
   const int first = tile * static_cast<int>(blockDim.x);
   const int k = first + static_cast<int>(threadIdx.x);
   const int staged = min(static_cast<int>(blockDim.x), d.cell_begin + d.n_cells - first);  // valid cells of the tile
@@ -337,7 +345,7 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
     store4(vel + 4 * threadIdx.x, own.rho, own.u, own.v, own.phi);
   }
   __syncthreads();
-  if (k >= d.cell_begin + d.n_cells) return;
+  if (failed || k >= d.cell_begin + d.n_cells) return;
   const T rho = own.rho, u = own.u, v = own.v, phi = own.phi;
   T g0 = 0, g1 = 0, g2 = 0, g3 = 0, sx = 0, sy = 0, rx = 0, ry = 0;
   const int end = a.term_off[k + 1];
@@ -587,7 +595,7 @@ void k_flux(ArenaPtrs<T> a) {
   if (tile < 0) return;
   const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
-  if (!pass_on(c, BASE ? PASS_BASE : PASS_STAGE)) return;
+  if (!pass_live(c, BASE ? PASS_BASE : PASS_STAGE)) return;
   const MeshDesc<T>& d = a.desc[m];
   const int nc = a.nc;
   const int k = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
@@ -600,8 +608,11 @@ void k_flux(ArenaPtrs<T> a) {
   const int staged = min(tn, d.cell_begin + d.n_cells - first);  // valid cells of the tile
   const TileCells<T> lds(tn);
   const FaceSlots<T> slots(tn, BASE);
+  __shared__ int failed;  // block snapshot of the slot's admissibility bits
+  if (tid == 0) failed = c->bad;
   if (valid) stage_cell(lds, tid, load_prim(a, rb, k));
   __syncthreads();
+  if (failed) return;
   // Face items of the tile, one chunk at a time: (A) each face touching the tile is evaluated
   // once, by one thread, into its LDS slot; (B) each cell sums its faces' slots in its cell-face
   // order.  A face shared with another tile is evaluated by both tiles from the same inputs, so
@@ -733,7 +744,6 @@ void k_flux(ArenaPtrs<T> a) {
           const T rate = diffusion * iv;
           if (rate > T(0)) dt = fmin(dt, d.transport.viscous_cfl / rate);
         } else {
-          a.CELL_DT[k] = dt;
           a.DRATE[k] = diffusion * iv;
         }
       }
@@ -741,6 +751,9 @@ void k_flux(ArenaPtrs<T> a) {
         const T rate = T(2) * sa_spec / (rho * a.vol[k]) + a.SRATE[k];
         if (rate > T(0)) dt = fmin(dt, d.transport.sa_cfl / rate);
       }
+      // SRD-grouped viscous limit: the cell's convective (and SA) limit waits for the group
+      // rates (k_dt_group), as the reference's cell_dt does (src/static_amr2d_cuda.cu:159-161).
+      if (laminar && d.visc_mode == VISC_SRD_GROUP) a.CELL_DT[k] = dt;
       local_dt = static_cast<double>(dt);
     } else if (stage_deltas(d)) {
       // SSP-RK3 on increments: U2 = U + D2 with D2 = (D1 + dt r) / 4, then
@@ -799,7 +812,7 @@ __global__ __launch_bounds__(kMaxTile) void k_diffusion_group(ArenaPtrs<T> a) {
   if (tile < 0) return;
   const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
-  if (!pass_on(c, PASS_BASE)) return;
+  if (!pass_live(c, PASS_BASE)) return;  // uniform over the block: it meets a barrier in publish_dt
   const MeshDesc<T>& d = a.desc[m];
   if (d.visc_mode != VISC_SRD_GROUP) return;
   const int g = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
@@ -815,7 +828,7 @@ __global__ __launch_bounds__(kMaxTile) void k_dt_group(ArenaPtrs<T> a) {
   if (tile < 0) return;
   const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
-  if (!pass_on(c, PASS_BASE)) return;
+  if (!pass_live(c, PASS_BASE)) return;  // uniform over the block: it meets a barrier in publish_dt
   const MeshDesc<T>& d = a.desc[m];
   if (d.visc_mode != VISC_SRD_GROUP) return;
   const int k = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
