@@ -231,8 +231,13 @@ typedef struct amr_kernel_time { char name[32]; double total_ms; int64_t launche
 int amr_hip_profile(amr_session* s, double tend, int32_t trials, amr_kernel_time* out, int32_t capacity,
                     int32_t* count, char* err, size_t err_len);
 
-/* STREAM-copy probe: `repeats` launches of a plain device copy of `bytes` (read + write);
- * gbps = 2 * bytes / time.  The achievable-bandwidth reference and the HBM counter calibration. */
+/* STREAM-copy probe: `repeats` launches of a device copy of `bytes` (read + write) in each of
+ * four kernel variants (16 B per lane; 4 x 16 B per lane; 4 x 16 B non-temporal; 8 x 16 B
+ * non-temporal), gbps[v] = 2 * bytes / time (capacity >= 4).  The achievable-bandwidth reference
+ * and the HBM counter calibration. */
+int amr_hip_copy_variants(uint64_t bytes, int32_t repeats, double* gbps, int32_t capacity, char* err,
+                          size_t err_len);
+/* The best rate of amr_hip_copy_variants and its time per copy. */
 int amr_hip_copy_bandwidth(uint64_t bytes, int32_t repeats, double* gbps, double* ms_per_copy, char* err,
                            size_t err_len);
 
@@ -259,6 +264,9 @@ typedef struct amr_batch_options {
   int32_t tile;         /* launch tile: 64, 128 or 256 threads per block */
   int32_t graph;        /* 1: replay captured hipGraphs (one per branch) of one trial of every slot */
   int32_t batch_trials; /* trials per host poll, 1..1024 */
+  int32_t separate_stage0; /* 0: a viscous arena without SRD opens each trial's first RK stage
+                            * inside its stage-1 gradient pass; 1: in a launch of its own (the
+                            * same bits, one launch more per trial) */
 } amr_batch_options;
 typedef struct amr_slot_state {
   double time, target;

@@ -16,27 +16,30 @@ from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_a
 
 HERE = Path(__file__).resolve().parent
 ORACLE_PATH = HERE / "liboracle.so"
+ORACLE_OMP_PATH = HERE / "liboracle_omp.so"  # the same sources with the reference's OpenMP loops on
 REF_PATH = HERE / "_ref" / "libamr_ref.so"
 REFERENCE_SOURCES = Path("/root/reference/src")
 
 
 def build(reference: bool = True) -> None:
     """Compile the oracle (and the reference shim when the reference checkout exists)."""
-    subprocess.run(["make", "-s", "-C", str(HERE), "all"], check=True)
+    subprocess.run(["make", "-s", "-C", str(HERE), "all", "omp"], check=True)
     if reference and REFERENCE_SOURCES.is_dir():
         subprocess.run(["make", "-s", "-C", str(HERE), "ref"], check=True)
 
 
-_oracle = None
+_oracle = {}
 _ref = None
 
 
-def oracle_lib():
-    global _oracle
-    if _oracle is None:
-        if not ORACLE_PATH.exists():
+def oracle_lib(threaded: bool = False):
+    """The oracle library: single-threaded (the checker), or `threaded` with the reference's
+    OpenMP parallel loops (OMP_NUM_THREADS threads; the same results, for CPU timing)."""
+    path = ORACLE_OMP_PATH if threaded else ORACLE_PATH
+    if path not in _oracle:
+        if not path.exists():
             build(reference=False)
-        lib = C.CDLL(str(ORACLE_PATH))
+        lib = C.CDLL(str(path))
         V, P = C.POINTER(N.amr_mesh_view), C.POINTER(N.amr_params)
         lib.oracle_rhs.restype = C.c_int
         lib.oracle_rhs.argtypes = [V, P, N.c_double_p, C.c_uint64, N.c_double_p, C.c_uint64]
@@ -61,8 +64,8 @@ def oracle_lib():
         lib.oracle_viscosity.argtypes = [C.POINTER(N.amr_transport), C.c_double]
         lib.oracle_time_rhs.restype = C.c_double
         lib.oracle_time_rhs.argtypes = [V, P, N.c_double_p, C.c_uint64, C.c_int]
-        _oracle = lib
-    return _oracle
+        _oracle[path] = lib
+    return _oracle[path]
 
 
 def ref_available() -> bool:
@@ -110,13 +113,13 @@ def rhs(mesh, params, u: np.ndarray) -> np.ndarray:
 
 
 def run(mesh, params, u: np.ndarray, tend: float, maxsteps: int = 50_000_000, time: float = 0.0, steps: int = 0,
-        rejected: int = 0):
-    """Advance to tend; returns (u, meta dict).  Raises on failure with the reference message."""
+        rejected: int = 0, threaded: bool = False):
+    """Advance to tend; returns (u, meta dict) with the status and the reference message."""
     u = np.array(u, dtype=np.float64, copy=True)
     meta = N.amr_state_meta(time, steps, rejected)
     err = N.err_buffer()
     view = mesh.view
-    status = oracle_lib().oracle_run(C.byref(view), C.byref(params), tend, maxsteps, N.dptr(u), u.size,
+    status = oracle_lib(threaded).oracle_run(C.byref(view), C.byref(params), tend, maxsteps, N.dptr(u), u.size,
                                      C.byref(meta), err, N.ERR_LEN)
     out = dict(time=meta.time, steps=meta.steps, rejected=meta.rejected, status=status, error=err.value.decode())
     return u, out

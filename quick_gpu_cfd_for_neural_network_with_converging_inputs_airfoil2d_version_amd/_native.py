@@ -158,7 +158,8 @@ class amr_batch_features(C.Structure):
 
 
 class amr_batch_options(C.Structure):
-    _fields_ = [("precision", C.c_int32), ("tile", C.c_int32), ("graph", C.c_int32), ("batch_trials", C.c_int32)]
+    _fields_ = [("precision", C.c_int32), ("tile", C.c_int32), ("graph", C.c_int32), ("batch_trials", C.c_int32),
+                ("separate_stage0", C.c_int32)]
 
 
 class amr_slot_state(C.Structure):
@@ -304,6 +305,8 @@ def load_library(path: os.PathLike | None = None):
     lib.amr_hip_profile.restype = C.c_int
     lib.amr_hip_profile.argtypes = [H, C.c_double, C.c_int32, C.POINTER(amr_kernel_time), C.c_int32,
                                     C.POINTER(C.c_int32), C.c_char_p, C.c_size_t]
+    lib.amr_hip_copy_variants.restype = C.c_int
+    lib.amr_hip_copy_variants.argtypes = [C.c_uint64, C.c_int32, c_double_p, C.c_int32, C.c_char_p, C.c_size_t]
     lib.amr_hip_copy_bandwidth.restype = C.c_int
     lib.amr_hip_copy_bandwidth.argtypes = [C.c_uint64, C.c_int32, c_double_p, c_double_p, C.c_char_p, C.c_size_t]
     lib.amr_hip_state.restype = C.c_int

@@ -62,6 +62,9 @@ class Arena {
   virtual uint64_t geometry_bytes() const = 0;
   virtual uint64_t state_bytes() const = 0;
   virtual int launch_trial(hipStream_t s, LaunchHook* hook = nullptr) = 0;
+  // Trial schedule: the first RK stage inside the stage-1 gradient pass (default) or in its own
+  // launch.  Set before a trial is captured in a graph.
+  virtual void set_stage0_in_gradient(bool on) = 0;
   virtual int launch_residual(hipStream_t s, LaunchHook* hook = nullptr) = 0;
   // Base primitives of every running slot whose host-written state has none yet; follows every
   // launch_target before trials are launched.
@@ -102,9 +105,11 @@ class Arena {
 void sum_monitor(const Arena& arena, int m, const std::vector<double>& cells, const std::vector<double>& walls,
                  double* out9);
 
-// Arena with one slot per input, each sized to its mesh, all admitted.
+// Arena with one slot per input, each sized to its mesh, all admitted.  With `slab` the arena
+// is laid out in that device memory (slab_bytes long; an error if it needs more) and allocates
+// no device or pinned memory, staging through pageable host memory instead.
 std::unique_ptr<Arena> make_arena(const std::vector<MeshInput>& inputs, int precision, int tile, hipStream_t stream,
-                                  std::string* error);
+                                  std::string* error, void* slab = nullptr, size_t slab_bytes = 0);
 // Arena of empty slots with the given capacities and kernel set.
 std::unique_ptr<Arena> make_slot_arena(const std::vector<SlotCapacity>& caps, const ArenaFeatures& features,
                                        int precision, int tile, hipStream_t stream, std::string* error);

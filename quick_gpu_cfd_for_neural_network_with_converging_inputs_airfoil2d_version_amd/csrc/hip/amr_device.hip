@@ -1,6 +1,7 @@
 // amr_device.hip -- slot layout of an arena, one-allocation upload, admission, state transfer.
 //
-// Every array lives in one hipMalloc'ed slab (256-byte aligned sub-allocations).  Host copies
+// Every array lives in one slab (256-byte aligned sub-allocations), hipMalloc'ed by the arena or
+// handed to it (make_arena's `slab`).  Host copies
 // of the geometry tables are kept so that admitting a mesh into a slot rewrites and uploads
 // only that slot's ranges.  Index spaces and their per-slot ranges:
 //   cells   [cell_begin, cell_begin + cap.cells)   (tile multiple, >= n_cells + 1)
@@ -22,6 +23,12 @@ namespace {
 constexpr int BC_TRANSMISSIVE = 1, BC_REFLECT = 2, BC_FARFIELD = 3, BC_TOTAL_INLET = 4, BC_PRESSURE_OUTLET = 5,
               BC_NOSLIP = 6;
 constexpr unsigned long long kInf = 0x7ff0000000000000ull;
+
+// Zero `n` 32-bit words (grid-stride): the one-time clear of an arena's work arrays.
+__global__ void k_clear_words(uint32_t* p, size_t n) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
+}
 
 bool state_ok(const double* w) {
   return std::isfinite(w[0]) && w[0] > 0 && std::isfinite(w[1]) && std::isfinite(w[2]) && std::isfinite(w[3]) &&
@@ -290,8 +297,18 @@ class DeviceArena final : public Arena {
  public:
   ~DeviceArena() override {
     if (slab_) (void)hipFree(slab_);
-    if (pinned_) (void)hipHostFree(pinned_);
-    if (pinned_frames_) (void)hipHostFree(pinned_frames_);
+    if (!external_) {
+      if (pinned_) (void)hipHostFree(pinned_);
+      if (pinned_frames_) (void)hipHostFree(pinned_frames_);
+    }
+  }
+
+  // Lay the slab out in caller-provided device memory of `bytes` bytes instead of allocating
+  // it, with host staging in pageable memory: building and using such an arena allocates no
+  // device or pinned memory.  Set before upload_all; the memory is not freed.
+  void use_external(void* slab, size_t bytes) {
+    external_ = slab;
+    external_bytes_ = bytes;
   }
 
   // Slots, host tables with every slot empty; no device calls.
@@ -727,8 +744,15 @@ class DeviceArena final : public Arena {
       return "AMR device query failed";
     if (flux_lds_bytes<T>(tile_, true) > static_cast<size_t>(lds_max))
       return "AMR tile needs more LDS per block than the device provides";
-    if (hipMalloc(reinterpret_cast<void**>(&base), slab_bytes_) != hipSuccess) return "AMR hipMalloc of the arena failed";
-    slab_ = base;
+    if (external_) {
+      if (slab_bytes_ > external_bytes_)
+        return "AMR arena needs " + std::to_string(slab_bytes_) + " bytes; the device scratch holds " +
+               std::to_string(external_bytes_);
+      base = static_cast<char*>(external_);
+    } else {
+      if (hipMalloc(reinterpret_cast<void**>(&base), slab_bytes_) != hipSuccess) return "AMR hipMalloc of the arena failed";
+      slab_ = base;  // owned: freed with the arena
+    }
     size_t offset = 0;
     for (Column& c : columns_) {
       c.dev = base + offset;
@@ -826,6 +850,7 @@ class DeviceArena final : public Arena {
     a.any_sa = features_.sa;
     a.any_small = features_.small;
     a.any_visc_group = features_.visc_group;
+    a.stage0_in_gradient = 1;
     for (const Column& c : columns_)
       if (hipMemcpyAsync(c.dev, c.host, c.elem * c.count, hipMemcpyHostToDevice, stream) != hipSuccess)
         return "AMR geometry H2D failed";
@@ -833,16 +858,27 @@ class DeviceArena final : public Arena {
     for (size_t m = 0; m < nm; ++m) ctrl[m] = initial_control(m < metas.size() ? metas[m] : amr_state_meta{0, 0, 0});
     if (hipMemcpyAsync(a.ctrl, ctrl.data(), nm * sizeof(Control), hipMemcpyHostToDevice, stream) != hipSuccess)
       return "AMR control H2D failed";
-    // Zero the work arrays once so padding lanes never hold garbage.
-    if (hipMemsetAsync(base + tables_end, 0, slab_bytes_ - tables_end, stream) != hipSuccess ||
-        hipStreamSynchronize(stream) != hipSuccess)
-      return "AMR work clear failed";
+    // Zero the work arrays once so padding lanes never hold garbage (a fill kernel: an arena in
+    // reused memory starts from zero as well, and the clear is a plain kernel node if captured).
+    const size_t words = (slab_bytes_ - tables_end) / sizeof(uint32_t);
+    if (words) {
+      const unsigned blocks = static_cast<unsigned>(std::min<size_t>((words + 255) / 256, 65536));
+      hipLaunchKernelGGL(k_clear_words, dim3(blocks), dim3(256), 0, stream,
+                         reinterpret_cast<uint32_t*>(base + tables_end), words);
+    }
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(stream) != hipSuccess) return "AMR work clear failed";
     size_t max_state = 1;
     for (const MeshSlot& s : slots_) max_state = std::max(max_state, static_cast<size_t>(s.nv) * s.cap.cells);
-    if (hipHostMalloc(&pinned_, max_state * sizeof(double), hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc(reinterpret_cast<void**>(&pinned_frames_), nm * 6 * sizeof(double), hipHostMallocDefault) !=
-            hipSuccess)
+    if (external_) {
+      stage_.assign(max_state, 0.0);
+      stage_frames_.assign(nm * 6, 0.0);
+      pinned_ = stage_.data();
+      pinned_frames_ = stage_frames_.data();
+    } else if (hipHostMalloc(&pinned_, max_state * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+               hipHostMalloc(reinterpret_cast<void**>(&pinned_frames_), nm * 6 * sizeof(double),
+                             hipHostMallocDefault) != hipSuccess) {
       return "AMR pinned staging allocation failed";
+    }
     return "";
   }
 
@@ -904,6 +940,7 @@ class DeviceArena final : public Arena {
   uint64_t geometry_bytes() const override { return geometry_bytes_; }
   uint64_t state_bytes() const override { return state_bytes_; }
   int launch_trial(hipStream_t s, LaunchHook* hook) override { return hipamr::launch_trial<T>(ptrs_, s, hook); }
+  void set_stage0_in_gradient(bool on) override { ptrs_.stage0_in_gradient = on ? 1 : 0; }
   int launch_residual(hipStream_t s, LaunchHook* hook) override { return hipamr::launch_residual<T>(ptrs_, s, hook); }
   int n_branches() const override { return static_cast<int>(table_offsets_.size()) == 2 ? 1 : static_cast<int>(table_offsets_.size()) - 2; }
   int launch_trial_branch(int j, hipStream_t s) override {
@@ -1139,7 +1176,10 @@ class DeviceArena final : public Arena {
   std::vector<MeshSlot> slots_;
   std::vector<double> sa_nu_inf_;
   ArenaFeatures features_{};
-  char* slab_ = nullptr;
+  char* slab_ = nullptr;  // owned slab (nullptr for an external one)
+  void* external_ = nullptr;
+  size_t external_bytes_ = 0;
+  std::vector<double> stage_, stage_frames_;  // pageable staging of an external arena
   void* pinned_ = nullptr;
   double* targets_ = nullptr;
   long long* maxsteps_ = nullptr;
@@ -1154,8 +1194,10 @@ class DeviceArena final : public Arena {
 
 template <class T>
 std::unique_ptr<Arena> build_arena(const std::vector<SlotCapacity>& caps, const ArenaFeatures& f, int tile,
-                                   const std::vector<MeshInput>* inputs, hipStream_t stream, std::string* error) {
+                                   const std::vector<MeshInput>* inputs, hipStream_t stream, std::string* error,
+                                   void* slab = nullptr, size_t slab_bytes = 0) {
   auto a = std::make_unique<DeviceArena<T>>();
+  if (slab) a->use_external(slab, slab_bytes);
   std::string e = a->layout(caps, f, tile);
   std::vector<amr_state_meta> metas;
   if (e.empty() && inputs)
@@ -1215,7 +1257,7 @@ std::string validate_input(const MeshInput& in) {
 }
 
 std::unique_ptr<Arena> make_arena(const std::vector<MeshInput>& inputs, int precision, int tile, hipStream_t stream,
-                                  std::string* error) {
+                                  std::string* error, void* slab, size_t slab_bytes) {
   if (tile != 64 && tile != 128 && tile != 256) {
     *error = "tile must be 64, 128 or 256";
     return nullptr;
@@ -1235,8 +1277,8 @@ std::unique_ptr<Arena> make_arena(const std::vector<MeshInput>& inputs, int prec
     f.small |= g.small;
     f.visc_group |= g.visc_group;
   }
-  if (precision == 64) return build_arena<double>(caps, f, tile, &inputs, stream, error);
-  if (precision == 32) return build_arena<float>(caps, f, tile, &inputs, stream, error);
+  if (precision == 64) return build_arena<double>(caps, f, tile, &inputs, stream, error, slab, slab_bytes);
+  if (precision == 32) return build_arena<float>(caps, f, tile, &inputs, stream, error, slab, slab_bytes);
   *error = "precision must be 64 or 32";
   return nullptr;
 }

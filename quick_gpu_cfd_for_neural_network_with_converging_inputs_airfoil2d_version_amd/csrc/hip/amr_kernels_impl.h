@@ -320,17 +320,102 @@ __global__ __launch_bounds__(kMaxTile) void k_base_prim(ArenaPtrs<T> a) {
 }
 
 // ------------------------------------------------------------------------------------------
+// Step size of the trial (reference choose_dt_kernel semantics, src/static_amr2d_cuda.cu:177-187).
+__device__ __forceinline__ bool choose_dt(const Control* c, double fixed, double* dt_out) {
+  double dt = c->retrying ? c->dt : (fixed > 0 ? fixed : __longlong_as_double(static_cast<long long>(c->dt_min)));
+  const double remaining = c->target - c->time;
+  dt = fmin(dt, remaining);
+  const double tol = 32 * DBL_EPSILON * fmax(fmax(fabs(c->target), fabs(c->time)), dt);
+  if (remaining - dt <= tol) dt = remaining;
+  *dt_out = dt;
+  return isfinite(dt) && dt > 0 && c->time + dt != c->time;
+}
+
+// The dt choice that opens the first RK stage, evaluated by every block of the slot from the
+// same control fields; the slot's first block records the trial step or the fatal code (2: the
+// base residual failed, 4: dt underflow).  False: no stage this trial.
+template <class T>
+__device__ __forceinline__ bool open_stage0(Control* c, const MeshDesc<T>& d, int tile, double* dt) {
+  const bool leader = tile == d.first_cell_block && threadIdx.x == 0;
+  if (c->bad & BAD_BASE) {  // the base residual failed: nonphysical base state
+    if (leader) c->fatal = 2;
+    return false;
+  }
+  if (!choose_dt(c, d.dt_fixed, dt)) {
+    if (leader) c->fatal = 4;
+    return false;
+  }
+  if (leader) {
+    c->trial_dt = *dt;
+    if (!c->retrying) c->attempt = 0;
+  }
+  return true;
+}
+
+// First SSP-RK3 stage of cell k, U1 = U + dt R0, as conserved values q (FP64).  With FP32
+// increments (stage_deltas) the increment is rounded to T first, D1 = T(dt R0), exactly as the
+// second stage re-forms it; otherwise U1 is also stored in U[cur ^ 1] when `store`.
+template <class T>
+__device__ __forceinline__ void first_stage(const ArenaPtrs<T>& a, const MeshDesc<T>& d, const Control* c, int k,
+                                            double dt, bool store, double (&q)[5]) {
+  const int nc = a.nc, cur = c->cur;
+  const double* u = a.U[cur];
+  for (int v = 0; v < 5; ++v) q[v] = 0.0;
+  if (stage_deltas(d)) {
+    for (int v = 0; v < d.nv; ++v)
+      q[v] = u[v * nc + k] + static_cast<double>(static_cast<T>(dt * static_cast<double>(a.R0[v * nc + k])));
+  } else {
+    for (int v = 0; v < d.nv; ++v) {
+      q[v] = u[v * nc + k] + dt * static_cast<double>(a.R0[v * nc + k]);
+      if (store) a.U[cur ^ 1][v * nc + k] = q[v];
+    }
+  }
+}
+
+// Density, velocity and SA working variable of the first-stage state of a cell another tile
+// owns: the same operations as make_prim_q, so the bits equal the ones that tile stores.
+template <class T>
+__device__ __forceinline__ CellVel<T> first_stage_vel(const ArenaPtrs<T>& a, const MeshDesc<T>& d, const Control* c,
+                                                      int k, double dt) {
+  const int nc = a.nc, cur = c->cur;
+  const double* u = a.U[cur];
+  const bool deltas = stage_deltas(d);
+  const auto q_of = [&](int v) {
+    const double r = static_cast<double>(a.R0[v * nc + k]);
+    return u[v * nc + k] + (deltas ? static_cast<double>(static_cast<T>(dt * r)) : dt * r);
+  };
+  const double rho = q_of(0), mx = q_of(1), my = q_of(2);
+  const double ir = 1.0 / rho;
+  CellVel<T> w;
+  w.rho = static_cast<T>(rho);
+  w.u = static_cast<T>(mx * ir);
+  w.v = static_cast<T>(my * ir);
+  w.phi = d.sa ? static_cast<T>(q_of(4) * ir) : T(0);
+  return w;
+}
+
+// ------------------------------------------------------------------------------------------
 // LSQ gradients of the stage's state (and the SA-neg point source) into the gradient vector
-// GREC.  STAGE 0 = base pass, 1 and 2 = RK stages.
-template <class T, int STAGE>
+// GREC.  STAGE 0 = base pass, 1 and 2 = RK stages.  FIRST (stage 1 only): the kernel also
+// opens the trial as k_stage0 does -- dt choice, first RK stage of its own cells and their
+// primitives (stored for the flux pass) -- and evaluates the first-stage velocities of
+// neighbours outside its tile itself, so the trial needs no separate stage-0 launch.  Meshes
+// without viscous terms take only the stage-0 part.
+template <class T, int STAGE, bool FIRST = false>
 __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
+  static_assert(!FIRST || STAGE == 1, "the first stage opens only the stage-1 gradients");
   const int tile = cell_tile(a);
   if (tile < 0) return;
   const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
-  if (!pass_live(c, STAGE == 0 ? PASS_BASE : PASS_STAGE)) return;
   const MeshDesc<T>& d = a.desc[m];
-  if (!d.laminar) return;
+  double dt = 0.0;
+  if (FIRST) {
+    if (!running(c) || !open_stage0(c, d, tile, &dt)) return;
+  } else {
+    if (!pass_live(c, STAGE == 0 ? PASS_BASE : PASS_STAGE)) return;
+    if (!d.laminar) return;
+  }
   __shared__ int failed;  // block snapshot of the slot's admissibility bits
   if (threadIdx.x == 0) failed = c->bad;
   const int first = tile * static_cast<int>(blockDim.x);
@@ -341,9 +426,17 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
   T* const vel = reinterpret_cast<T*>(tile_lds);  // the tile's own velocity vectors (4 per cell)
   CellPrim<T> own{};
   if (k < d.cell_begin + d.n_cells) {
-    own = load_prim(a, rb, k);
+    if (FIRST) {
+      double q[5];
+      first_stage(a, d, c, k, dt, true, q);
+      if (!make_prim_q(q, d, own)) flag_bad(c, BAD_STAGE);
+      else store_prim(a, write_buffer(c, 0), k, own);
+    } else {
+      own = load_prim(a, rb, k);
+    }
     store4(vel + 4 * threadIdx.x, own.rho, own.u, own.v, own.phi);
   }
+  if (FIRST && !d.laminar) return;  // the whole block: no barrier below is reached
   __syncthreads();
   if (failed || k >= d.cell_begin + d.n_cells) return;
   const T rho = own.rho, u = own.u, v = own.v, phi = own.phi;
@@ -358,6 +451,8 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
       const int local = src - first;
       if (local >= 0 && local < staged)
         load4(vel + 4 * local, nb.rho, nb.u, nb.v, nb.phi);
+      else if (FIRST)
+        nb = first_stage_vel(a, d, c, src, dt);
       else
         nb = load_vel(a, rb, src);
       uu = nb.u;
@@ -845,17 +940,9 @@ __global__ __launch_bounds__(kMaxTile) void k_dt_group(ArenaPtrs<T> a) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Step size of the trial (reference choose_dt_kernel semantics) and the first RK stage.
-__device__ __forceinline__ bool choose_dt(const Control* c, double fixed, double* dt_out) {
-  double dt = c->retrying ? c->dt : (fixed > 0 ? fixed : __longlong_as_double(static_cast<long long>(c->dt_min)));
-  const double remaining = c->target - c->time;
-  dt = fmin(dt, remaining);
-  const double tol = 32 * DBL_EPSILON * fmax(fmax(fabs(c->target), fabs(c->time)), dt);
-  if (remaining - dt <= tol) dt = remaining;
-  *dt_out = dt;
-  return isfinite(dt) && dt > 0 && c->time + dt != c->time;
-}
-
+// Step size of the trial and the first RK stage (reference choose_dt_kernel and rk_kernel,
+// src/static_amr2d_cuda.cu:177-194), for arenas whose trial does not open in k_grad<1> (SRD
+// meshes, whose stage state is redistributed before its primitives exist; inviscid arenas).
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
   const int tile = cell_tile(a);
@@ -864,42 +951,17 @@ __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
   Control* c = a.ctrl + m;
   if (!running(c)) return;
   const MeshDesc<T>& d = a.desc[m];
-  const bool leader = tile == d.first_cell_block && threadIdx.x == 0;
-  if (c->bad & BAD_BASE) {  // the base residual failed: nonphysical base state
-    if (leader) c->fatal = 2;
-    return;
-  }
   double dt;
-  if (!choose_dt(c, d.dt_fixed, &dt)) {
-    if (leader) c->fatal = 4;
-    return;
-  }
-  if (leader) {
-    c->trial_dt = dt;
-    if (!c->retrying) c->attempt = 0;
-  }
+  if (!open_stage0(c, d, tile, &dt)) return;
   const int k = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
   if (k >= d.cell_begin + d.n_cells) return;
-  const int nc = a.nc, cur = c->cur;
-  const double* u = a.U[cur];
-  if (stage_deltas(d)) {  // U1 = U + D1, D1 = dt R0 in T (not stored: the next stage re-forms it)
-    double q[5] = {0, 0, 0, 0, 0};
-    for (int v = 0; v < d.nv; ++v) {
-      const T dv = static_cast<T>(dt * static_cast<double>(a.R0[v * nc + k]));
-      q[v] = u[v * nc + k] + static_cast<double>(dv);
-    }
-    CellPrim<T> w;
-    if (!make_prim_q(q, d, w)) flag_bad(c, BAD_STAGE);
-    else store_prim(a, write_buffer(c, 0), k, w);
-    return;
-  }
-  double* out = a.U[cur ^ 1];
-  for (int v = 0; v < d.nv; ++v) out[v * nc + k] = u[v * nc + k] + dt * static_cast<double>(a.R0[v * nc + k]);
-  if (!d.small) {
-    CellPrim<T> w;
-    if (!make_prim(out, nc, k, d, w)) flag_bad(c, BAD_STAGE);
-    else store_prim(a, write_buffer(c, 0), k, w);
-  }
+  // U1 = U + D1, D1 = T(dt R0) not stored (the next stage re-forms it); or U1 in U[cur ^ 1].
+  double q[5];
+  first_stage(a, d, c, k, dt, true, q);
+  if (d.small && !stage_deltas(d)) return;  // primitives after the redistribution
+  CellPrim<T> w;
+  if (!make_prim_q(q, d, w)) flag_bad(c, BAD_STAGE);
+  else store_prim(a, write_buffer(c, 0), k, w);
 }
 
 // SRD: q_g = sum_k w_gk u_k, then u_k = mean of q over the groups holding k, then primitives.
@@ -1148,12 +1210,17 @@ int launch_trial(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
   const dim3 tile(a.tile), cells(a.n_grid_blocks);
   const size_t stage_lds = flux_lds_bytes<T>(a.tile, false), vel_lds = grad_lds_bytes<T>(a.tile);
   int launches = launch_base_pass(a, s, hook);
-  HIPAMR_LAUNCH("stage0", k_stage0<T>, cells, a);
-  if (a.any_small) {
-    HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 0);
-    HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 0);
+  if (a.any_laminar && !a.any_small && a.stage0_in_gradient) {
+    // The first stage opens inside the stage-1 gradient pass (one launch fewer per trial).
+    HIPAMR_LAUNCH_LDS("stage0_gradient", (k_grad<T, 1, true>), cells, vel_lds, a);
+  } else {
+    HIPAMR_LAUNCH("stage0", k_stage0<T>, cells, a);
+    if (a.any_small) {
+      HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 0);
+      HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 0);
+    }
+    if (a.any_laminar) HIPAMR_LAUNCH_LDS("gradient_stage", (k_grad<T, 1>), cells, vel_lds, a);
   }
-  if (a.any_laminar) HIPAMR_LAUNCH_LDS("gradient_stage", (k_grad<T, 1>), cells, vel_lds, a);
   HIPAMR_LAUNCH_LDS("flux_rk1", (k_flux<T, 1>), cells, stage_lds, a);
   if (a.any_small) {
     HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 1);

@@ -347,17 +347,52 @@ int amr_hip_profile(amr_session* s, double tend, int32_t trials, amr_kernel_time
 }  // extern "C"
 
 namespace {
-// Plain device copy, 16 bytes per lane per iteration (the STREAM "copy" kernel).
-__global__ __launch_bounds__(256) void k_stream_copy(const double2* __restrict__ src, double2* __restrict__ dst,
-                                                     size_t n) {
-  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
-  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+// Device copy (the STREAM "copy" kernel): each lane moves UNROLL 16-byte vectors per chunk of
+// 256 * UNROLL vectors, the block's lanes side by side (coalesced), chunks grid-strided; NT uses
+// non-temporal loads and stores (streamed past the caches).
+template <int UNROLL, bool NT>
+__global__ __launch_bounds__(256) void k_stream_copy(const d2* __restrict__ src, d2* __restrict__ dst, size_t n) {
+  const size_t chunk = static_cast<size_t>(256) * UNROLL;
+  for (size_t c = static_cast<size_t>(blockIdx.x) * chunk; c < n; c += static_cast<size_t>(gridDim.x) * chunk) {
+    d2 v[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+      const size_t i = c + static_cast<size_t>(u) * 256 + threadIdx.x;
+      if (i < n) v[u] = NT ? __builtin_nontemporal_load(src + i) : src[i];
+    }
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+      const size_t i = c + static_cast<size_t>(u) * 256 + threadIdx.x;
+      if (i < n) {
+        if (NT) __builtin_nontemporal_store(v[u], dst + i);
+        else dst[i] = v[u];
+      }
+    }
+  }
+}
+
+// The probe's variants: plain 1 x 16 B per lane, 4 x 16 B, 4 x 16 B non-temporal, 8 x 16 B
+// non-temporal.
+constexpr int kCopyVariants = 4;
+void launch_copy(int variant, const d2* a, d2* b, size_t n, hipStream_t s) {
+  const int unroll = variant == 0 ? 1 : (variant == 3 ? 8 : 4);
+  const size_t chunks = (n + 256 * unroll - 1) / (256 * unroll);
+  const dim3 grid(static_cast<unsigned>(std::min<size_t>(chunks, 256 * 32))), block(256);
+  switch (variant) {
+    case 0: hipLaunchKernelGGL((k_stream_copy<1, false>), grid, block, 0, s, a, b, n); break;
+    case 1: hipLaunchKernelGGL((k_stream_copy<4, false>), grid, block, 0, s, a, b, n); break;
+    case 2: hipLaunchKernelGGL((k_stream_copy<4, true>), grid, block, 0, s, a, b, n); break;
+    default: hipLaunchKernelGGL((k_stream_copy<8, true>), grid, block, 0, s, a, b, n); break;
+  }
 }
 }  // namespace
 
-extern "C" int amr_hip_copy_bandwidth(uint64_t bytes, int32_t repeats, double* gbps, double* ms_per_copy, char* err,
-                                      size_t err_len) {
-  if (!gbps || !ms_per_copy || repeats < 1 || bytes < 4096 || bytes % 16) return fail(err, err_len, "invalid copy probe");
+extern "C" int amr_hip_copy_variants(uint64_t bytes, int32_t repeats, double* gbps, int32_t capacity, char* err,
+                                     size_t err_len) {
+  if (!gbps || capacity < kCopyVariants || repeats < 1 || bytes < 4096 || bytes % 16)
+    return fail(err, err_len, "invalid copy probe");
   void *a = nullptr, *b = nullptr;
   hipStream_t s = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -378,18 +413,30 @@ extern "C" int amr_hip_copy_bandwidth(uint64_t bytes, int32_t repeats, double* g
     return 1;
   }
   const size_t n = bytes / 16;
-  const dim3 grid(static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 256 * 64))), block(256);
-  hipLaunchKernelGGL(k_stream_copy, grid, block, 0, s, static_cast<const double2*>(a), static_cast<double2*>(b), n);
-  (void)hipEventRecord(e0, s);
-  for (int r = 0; r < repeats; ++r)
-    hipLaunchKernelGGL(k_stream_copy, grid, block, 0, s, static_cast<const double2*>(a), static_cast<double2*>(b), n);
-  (void)hipEventRecord(e1, s);
-  float ms = 0;
-  const bool ok = checked(hipEventSynchronize(e1), err, err_len, "copy probe sync") &&
-                  checked(hipEventElapsedTime(&ms, e0, e1), err, err_len, "copy probe elapsed");
+  const d2* src = static_cast<const d2*>(a);
+  d2* dst = static_cast<d2*>(b);
+  bool ok = true;
+  for (int v = 0; v < kCopyVariants && ok; ++v) {
+    launch_copy(v, src, dst, n, s);  // warm-up
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < repeats; ++r) launch_copy(v, src, dst, n, s);
+    (void)hipEventRecord(e1, s);
+    float ms = 0;
+    ok = checked(hipGetLastError(), err, err_len, "copy probe launch") &&
+         checked(hipEventSynchronize(e1), err, err_len, "copy probe sync") &&
+         checked(hipEventElapsedTime(&ms, e0, e1), err, err_len, "copy probe elapsed");
+    if (ok) gbps[v] = 2.0 * static_cast<double>(bytes) / (ms / repeats * 1e-3) / 1e9;
+  }
   cleanup();
-  if (!ok) return 1;
-  *ms_per_copy = ms / repeats;
-  *gbps = 2.0 * static_cast<double>(bytes) / (*ms_per_copy * 1e-3) / 1e9;
+  return ok ? 0 : 1;
+}
+
+extern "C" int amr_hip_copy_bandwidth(uint64_t bytes, int32_t repeats, double* gbps, double* ms_per_copy, char* err,
+                                      size_t err_len) {
+  if (!gbps || !ms_per_copy) return fail(err, err_len, "invalid copy probe");
+  double rates[kCopyVariants] = {};
+  if (amr_hip_copy_variants(bytes, repeats, rates, kCopyVariants, err, err_len)) return 1;
+  *gbps = *std::max_element(rates, rates + kCopyVariants);
+  *ms_per_copy = 2.0 * static_cast<double>(bytes) / (*gbps * 1e9) * 1e3;
   return 0;
 }

@@ -28,6 +28,8 @@ struct amr_batch_driver {
   std::vector<Clock::time_point> started;
   std::vector<char> stepped;                     // the solve has advanced at least once
   std::vector<amr_slot_state> states;
+  Clock::time_point reported = Clock::now();  // last progress line
+  uint64_t check_rounds = 0;
 };
 
 namespace {
@@ -163,6 +165,15 @@ int amr_driver_poll(amr_batch_driver* d, double wall_budget_seconds, amr_steady_
       if (amr_batch_monitor(d->batch, mask.data(), frames.data(), mach.data(), aoa.data(), mon.data(), status.data(),
                             err, err_len))
         return 1;
+      ++d->check_rounds;
+      if (seconds_since(d->reported) >= 10) {  // progress of long solves: at most one line per 10 s of checks
+        double tmin = INFINITY, tmax = 0;
+        for (int m = 0; m < n; ++m)
+          if (d->runs[m]) tmin = std::min(tmin, d->states[m].time), tmax = std::max(tmax, d->states[m].time);
+        std::fprintf(stderr, "amr-driver: %d solves running, %llu check rounds, t* %.4g..%.4g\n", running,
+                     static_cast<unsigned long long>(d->check_rounds), tmin, tmax);
+        d->reported = Clock::now();
+      }
       for (int m = 0; m < n; ++m) {
         if (!mask[m]) continue;
         const amr_slot_state& s = d->states[m];
