@@ -264,9 +264,6 @@ typedef struct amr_batch_options {
   int32_t tile;         /* launch tile: 64, 128 or 256 threads per block */
   int32_t graph;        /* 1: replay captured hipGraphs (one per branch) of one trial of every slot */
   int32_t batch_trials; /* trials per host poll, 1..1024 */
-  int32_t separate_stage0; /* 0: a viscous arena without SRD opens each trial's first RK stage
-                            * inside its stage-1 gradient pass; 1: in a launch of its own (the
-                            * same bits, one launch more per trial) */
 } amr_batch_options;
 typedef struct amr_slot_state {
   double time, target;

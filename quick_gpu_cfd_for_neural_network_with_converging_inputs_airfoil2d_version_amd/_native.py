@@ -158,8 +158,7 @@ class amr_batch_features(C.Structure):
 
 
 class amr_batch_options(C.Structure):
-    _fields_ = [("precision", C.c_int32), ("tile", C.c_int32), ("graph", C.c_int32), ("batch_trials", C.c_int32),
-                ("separate_stage0", C.c_int32)]
+    _fields_ = [("precision", C.c_int32), ("tile", C.c_int32), ("graph", C.c_int32), ("batch_trials", C.c_int32)]
 
 
 class amr_slot_state(C.Structure):

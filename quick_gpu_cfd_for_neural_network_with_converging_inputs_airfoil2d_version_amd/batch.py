@@ -83,14 +83,13 @@ class Batch:
         self._state_buf = (N.amr_slot_state * n)()
 
     @staticmethod
-    def _options(precision, tile, graph, batch_trials, separate_stage0=False):
-        return N.amr_batch_options(precision, tile, 1 if graph else 0, batch_trials, 1 if separate_stage0 else 0)
+    def _options(precision, tile, graph, batch_trials):
+        return N.amr_batch_options(precision, tile, 1 if graph else 0, batch_trials)
 
     @classmethod
     def of_meshes(cls, meshes, params, states, *, metas=None, precision: int = 64, tile: int = 64,
-                  graph: bool = True, batch_trials: int = 32, separate_stage0: bool = False) -> "Batch":
-        """One slot per mesh, each sized to its mesh and admitted with its initial state
-        (separate_stage0: the first RK stage in a launch of its own, amr_batch_options)."""
+                  graph: bool = True, batch_trials: int = 32) -> "Batch":
+        """One slot per mesh, each sized to its mesh and admitted with its initial state."""
         lib = N.load_library()
         n = len(meshes)
         views = (N.amr_mesh_view * n)(*[m.view for m in meshes])
@@ -102,7 +101,7 @@ class Batch:
             meta = (N.amr_state_meta * n)(*[N.amr_state_meta(*m) for m in metas])
         handle = C.c_void_p()
         err = N.err_buffer()
-        opts = cls._options(precision, tile, graph, batch_trials, separate_stage0)
+        opts = cls._options(precision, tile, graph, batch_trials)
         if lib.amr_batch_create(views, prm, ptrs, meta, n, C.byref(opts), C.byref(handle), err, N.ERR_LEN):
             raise BackendError(1, err.value.decode())
         nv = 5 if meshes[0].view.transport.sa_model else 4
@@ -112,14 +111,14 @@ class Batch:
 
     @classmethod
     def with_slots(cls, caps, features, *, precision: int = 64, tile: int = 64, graph: bool = True,
-                   batch_trials: int = 32, separate_stage0: bool = False) -> "Batch":
+                   batch_trials: int = 32) -> "Batch":
         """Empty slots of the given capacities; the kernel set is fixed by `features`."""
         lib = N.load_library()
         n = len(caps)
         arr = (N.amr_slot_capacity * n)(*caps)
         handle = C.c_void_p()
         err = N.err_buffer()
-        opts = cls._options(precision, tile, graph, batch_trials, separate_stage0)
+        opts = cls._options(precision, tile, graph, batch_trials)
         if lib.amr_batch_create_slots(arr, n, C.byref(features), C.byref(opts), C.byref(handle), err, N.ERR_LEN):
             raise BackendError(1, err.value.decode())
         return cls(handle, n, 5 if features.sa else 4, precision, tile)

@@ -154,9 +154,6 @@ struct ArenaPtrs {
   T* CELL_DT; // deferred local dt (SRD-grouped viscous limit only)
   T* DRATE;
   T* DGROUP;
-  // 1: a viscous arena without SRD opens each trial's first RK stage inside the stage-1
-  // gradient pass (k_grad<T, 1, true>); 0: in its own launch (k_stage0).  The same bits.
-  int stage0_in_gradient;
 };
 
 // Optional observer of the launch sequence (per-kernel timing): before(name) is called right

@@ -72,8 +72,7 @@ using Clock = std::chrono::steady_clock;
 
 bool options_ok(const amr_batch_options* o) {
   return o && (o->precision == 32 || o->precision == 64) && (o->tile == 64 || o->tile == 128 || o->tile == 256) &&
-         o->batch_trials >= 1 && o->batch_trials <= 1024 && (o->graph == 0 || o->graph == 1) &&
-         (o->separate_stage0 == 0 || o->separate_stage0 == 1);
+         o->batch_trials >= 1 && o->batch_trials <= 1024 && (o->graph == 0 || o->graph == 1);
 }
 
 hipamr::SlotCapacity to_cap(const amr_slot_capacity& c) {
@@ -295,7 +294,6 @@ int amr_batch_create(const amr_mesh_view* meshes, const amr_params* params, cons
   std::string message;
   b->arena = hipamr::make_arena(inputs, options->precision, options->tile, b->stream, &message);
   if (!b->arena) return fail(err, err_len, message);
-  b->arena->set_stage0_in_gradient(options->separate_stage0 == 0);
   b->stats.state_h2d_bytes = state_bytes;
   b->stats.control_h2d_bytes = n * sizeof(Control);
   if (finish_setup(b.get(), t0, err, err_len)) return 1;
@@ -321,7 +319,6 @@ int amr_batch_create_slots(const amr_slot_capacity* capacities, int32_t n, const
   std::string message;
   b->arena = hipamr::make_slot_arena(caps, f, options->precision, options->tile, b->stream, &message);
   if (!b->arena) return fail(err, err_len, message);
-  b->arena->set_stage0_in_gradient(options->separate_stage0 == 0);
   b->stats.control_h2d_bytes = n * sizeof(Control);
   if (finish_setup(b.get(), t0, err, err_len)) return 1;
   *out = b.release();

@@ -62,9 +62,6 @@ class Arena {
   virtual uint64_t geometry_bytes() const = 0;
   virtual uint64_t state_bytes() const = 0;
   virtual int launch_trial(hipStream_t s, LaunchHook* hook = nullptr) = 0;
-  // Trial schedule: the first RK stage inside the stage-1 gradient pass (default) or in its own
-  // launch.  Set before a trial is captured in a graph.
-  virtual void set_stage0_in_gradient(bool on) = 0;
   virtual int launch_residual(hipStream_t s, LaunchHook* hook = nullptr) = 0;
   // Base primitives of every running slot whose host-written state has none yet; follows every
   // launch_target before trials are launched.

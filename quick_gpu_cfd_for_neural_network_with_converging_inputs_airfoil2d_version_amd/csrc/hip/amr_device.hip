@@ -850,7 +850,6 @@ class DeviceArena final : public Arena {
     a.any_sa = features_.sa;
     a.any_small = features_.small;
     a.any_visc_group = features_.visc_group;
-    a.stage0_in_gradient = 1;
     for (const Column& c : columns_)
       if (hipMemcpyAsync(c.dev, c.host, c.elem * c.count, hipMemcpyHostToDevice, stream) != hipSuccess)
         return "AMR geometry H2D failed";
@@ -940,7 +939,6 @@ class DeviceArena final : public Arena {
   uint64_t geometry_bytes() const override { return geometry_bytes_; }
   uint64_t state_bytes() const override { return state_bytes_; }
   int launch_trial(hipStream_t s, LaunchHook* hook) override { return hipamr::launch_trial<T>(ptrs_, s, hook); }
-  void set_stage0_in_gradient(bool on) override { ptrs_.stage0_in_gradient = on ? 1 : 0; }
   int launch_residual(hipStream_t s, LaunchHook* hook) override { return hipamr::launch_residual<T>(ptrs_, s, hook); }
   int n_branches() const override { return static_cast<int>(table_offsets_.size()) == 2 ? 1 : static_cast<int>(table_offsets_.size()) - 2; }
   int launch_trial_branch(int j, hipStream_t s) override {

@@ -97,6 +97,16 @@ struct CellVel {
   T rho, u, v, phi;
 };
 
+// The last read of a per-cell plane in a trial (the state in the last stage, the residual and
+// the increment where they are consumed): a non-temporal load, so that data no later kernel
+// reads does not displace what the next kernels gather.  (Non-temporal stores of the records and
+// planes that the next kernel reads were measured and dropped: they bypass the device's last-level
+// cache, and the kernel after them got slower, stage0 49 -> 56 us on 256 LF2 meshes.)
+template <class X>
+__device__ __forceinline__ X ld_last(const X* p) {
+  return __builtin_nontemporal_load(p);
+}
+
 // 16-byte vector loads and stores of 4 T (float4, or two double2).
 __device__ __forceinline__ void load4(const float* p, float& a, float& b, float& c, float& d) {
   const float4 v = *reinterpret_cast<const float4*>(p);
@@ -372,50 +382,18 @@ __device__ __forceinline__ void first_stage(const ArenaPtrs<T>& a, const MeshDes
   }
 }
 
-// Density, velocity and SA working variable of the first-stage state of a cell another tile
-// owns: the same operations as make_prim_q, so the bits equal the ones that tile stores.
-template <class T>
-__device__ __forceinline__ CellVel<T> first_stage_vel(const ArenaPtrs<T>& a, const MeshDesc<T>& d, const Control* c,
-                                                      int k, double dt) {
-  const int nc = a.nc, cur = c->cur;
-  const double* u = a.U[cur];
-  const bool deltas = stage_deltas(d);
-  const auto q_of = [&](int v) {
-    const double r = static_cast<double>(a.R0[v * nc + k]);
-    return u[v * nc + k] + (deltas ? static_cast<double>(static_cast<T>(dt * r)) : dt * r);
-  };
-  const double rho = q_of(0), mx = q_of(1), my = q_of(2);
-  const double ir = 1.0 / rho;
-  CellVel<T> w;
-  w.rho = static_cast<T>(rho);
-  w.u = static_cast<T>(mx * ir);
-  w.v = static_cast<T>(my * ir);
-  w.phi = d.sa ? static_cast<T>(q_of(4) * ir) : T(0);
-  return w;
-}
-
 // ------------------------------------------------------------------------------------------
 // LSQ gradients of the stage's state (and the SA-neg point source) into the gradient vector
-// GREC.  STAGE 0 = base pass, 1 and 2 = RK stages.  FIRST (stage 1 only): the kernel also
-// opens the trial as k_stage0 does -- dt choice, first RK stage of its own cells and their
-// primitives (stored for the flux pass) -- and evaluates the first-stage velocities of
-// neighbours outside its tile itself, so the trial needs no separate stage-0 launch.  Meshes
-// without viscous terms take only the stage-0 part.
-template <class T, int STAGE, bool FIRST = false>
+// GREC.  STAGE 0 = base pass, 1 and 2 = RK stages.
+template <class T, int STAGE>
 __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
-  static_assert(!FIRST || STAGE == 1, "the first stage opens only the stage-1 gradients");
   const int tile = cell_tile(a);
   if (tile < 0) return;
   const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
+  if (!pass_live(c, STAGE == 0 ? PASS_BASE : PASS_STAGE)) return;
   const MeshDesc<T>& d = a.desc[m];
-  double dt = 0.0;
-  if (FIRST) {
-    if (!running(c) || !open_stage0(c, d, tile, &dt)) return;
-  } else {
-    if (!pass_live(c, STAGE == 0 ? PASS_BASE : PASS_STAGE)) return;
-    if (!d.laminar) return;
-  }
+  if (!d.laminar) return;
   __shared__ int failed;  // block snapshot of the slot's admissibility bits
   if (threadIdx.x == 0) failed = c->bad;
   const int first = tile * static_cast<int>(blockDim.x);
@@ -426,17 +404,9 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
   T* const vel = reinterpret_cast<T*>(tile_lds);  // the tile's own velocity vectors (4 per cell)
   CellPrim<T> own{};
   if (k < d.cell_begin + d.n_cells) {
-    if (FIRST) {
-      double q[5];
-      first_stage(a, d, c, k, dt, true, q);
-      if (!make_prim_q(q, d, own)) flag_bad(c, BAD_STAGE);
-      else store_prim(a, write_buffer(c, 0), k, own);
-    } else {
-      own = load_prim(a, rb, k);
-    }
+    own = load_prim(a, rb, k);
     store4(vel + 4 * threadIdx.x, own.rho, own.u, own.v, own.phi);
   }
-  if (FIRST && !d.laminar) return;  // the whole block: no barrier below is reached
   __syncthreads();
   if (failed || k >= d.cell_begin + d.n_cells) return;
   const T rho = own.rho, u = own.u, v = own.v, phi = own.phi;
@@ -451,8 +421,6 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
       const int local = src - first;
       if (local >= 0 && local < staged)
         load4(vel + 4 * local, nb.rho, nb.u, nb.v, nb.phi);
-      else if (FIRST)
-        nb = first_stage_vel(a, d, c, src, dt);
       else
         nb = load_vel(a, rb, src);
       uu = nb.u;
@@ -862,14 +830,14 @@ void k_flux(ArenaPtrs<T> a) {
       double q[5] = {0, 0, 0, 0, 0};
       for (int v = 0; v < d.nv; ++v) {
         const int i = v * nc + k;
-        const T prev = STAGE == 1 ? static_cast<T>(dt * static_cast<double>(a.R0[i])) : a.DU[i];
+        const T prev = STAGE == 1 ? static_cast<T>(dt * static_cast<double>(ld_last(a.R0 + i))) : ld_last(a.DU + i);
         const double inc = static_cast<double>(prev) + dt * static_cast<double>(rv[v]);
         if (STAGE == 1) {
           const T dn = static_cast<T>(0.25 * inc);
           a.DU[i] = dn;
           q[v] = u[i] + static_cast<double>(dn);
         } else {
-          q[v] = u[i] + (2.0 / 3.0) * inc;
+          q[v] = ld_last(u + i) + (2.0 / 3.0) * inc;
           a.U[cur ^ 1][i] = q[v];
         }
       }
@@ -941,8 +909,10 @@ __global__ __launch_bounds__(kMaxTile) void k_dt_group(ArenaPtrs<T> a) {
 
 // ------------------------------------------------------------------------------------------
 // Step size of the trial and the first RK stage (reference choose_dt_kernel and rk_kernel,
-// src/static_amr2d_cuda.cu:177-194), for arenas whose trial does not open in k_grad<1> (SRD
-// meshes, whose stage state is redistributed before its primitives exist; inviscid arenas).
+// src/static_amr2d_cuda.cu:177-194).  (Opening the first stage inside the stage-1 gradient pass
+// instead, which then forms the first-stage velocities of neighbours outside its tile from U and
+// R0 itself, was measured and dropped: 146.8 us against 49.4 + 79.8 us for the two launches on
+// 256 LF2 meshes, and no gain in the latency of a single-mesh trial, 70.5 against 70.4 us.)
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
   const int tile = cell_tile(a);
@@ -1210,17 +1180,12 @@ int launch_trial(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
   const dim3 tile(a.tile), cells(a.n_grid_blocks);
   const size_t stage_lds = flux_lds_bytes<T>(a.tile, false), vel_lds = grad_lds_bytes<T>(a.tile);
   int launches = launch_base_pass(a, s, hook);
-  if (a.any_laminar && !a.any_small && a.stage0_in_gradient) {
-    // The first stage opens inside the stage-1 gradient pass (one launch fewer per trial).
-    HIPAMR_LAUNCH_LDS("stage0_gradient", (k_grad<T, 1, true>), cells, vel_lds, a);
-  } else {
-    HIPAMR_LAUNCH("stage0", k_stage0<T>, cells, a);
-    if (a.any_small) {
-      HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 0);
-      HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 0);
-    }
-    if (a.any_laminar) HIPAMR_LAUNCH_LDS("gradient_stage", (k_grad<T, 1>), cells, vel_lds, a);
+  HIPAMR_LAUNCH("stage0", k_stage0<T>, cells, a);
+  if (a.any_small) {
+    HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 0);
+    HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 0);
   }
+  if (a.any_laminar) HIPAMR_LAUNCH_LDS("gradient_stage", (k_grad<T, 1>), cells, vel_lds, a);
   HIPAMR_LAUNCH_LDS("flux_rk1", (k_flux<T, 1>), cells, stage_lds, a);
   if (a.any_small) {
     HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 1);

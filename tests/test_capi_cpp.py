@@ -103,7 +103,7 @@ def test_cpp_api_matches_session_bitwise(tmp_path):
         assert (int(status), int(down)) == (0, 0)
         assert (float(time), int(steps), int(rejected)) == (m["time"], m["steps"], m["rejected"])
     polls, downloads, d2h, allocated, nodes = map(int, lines["stats"].split())
-    assert polls >= 2 and downloads == 2 and d2h == 2 * u0.size * 8 and allocated > 0 and nodes >= 8
+    assert polls >= 2 and downloads == 2 and d2h == 2 * u0.size * 8 and allocated > 0 and nodes >= 7
     # Wall budget: 2 after the first poll (32 of the 41 steps), then resumed to the same bits.
     b = lines["budget"].split()
     assert (int(b[0]), int(b[1])) == (0, 2) and float(b[2]) < t1 and int(b[3]) == 32

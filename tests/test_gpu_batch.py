@@ -143,27 +143,6 @@ def test_fp32_sa_branches_at_production_tiles_match_single_slot_bitwise(tile):
             assert np.array_equal(u, ref_u)
 
 
-@pytest.mark.parametrize("precision,tile", [(32, 128), (64, 64)])
-def test_first_stage_inside_the_gradient_pass_matches_its_own_launch(precision, tile):
-    """The default trial opens the first RK stage inside the stage-1 gradient pass, which also
-    forms the first-stage velocities of neighbours outside its tile; a trial with k_stage0 in a
-    launch of its own must give the same bits (states, times, counters), SA-neg and laminar."""
-    laminar = four_variable_meshes()
-    for group in (sa_meshes(), [laminar[0], laminar[2]]):  # no SRD mesh: it takes the stage-0 launch anyway
-        runs = []
-        for separate in (False, True):
-            with B.Batch.of_meshes([m for _, m, _, _ in group], [p for _, _, p, _ in group],
-                                   [u for _, _, _, u in group], precision=precision, tile=tile,
-                                   separate_stage0=separate) as b:
-                b.set_targets(0.02)
-                assert b.advance("all") == 0
-                runs.append([b.download(k) for k in range(len(group))])
-        for (u, meta), (v, vmeta) in zip(*runs):
-            print(precision, tile, meta, vmeta, flush=True)
-            assert meta == vmeta
-            assert np.array_equal(u, v)
-
-
 def test_refill_and_release_keep_neighbours_bitwise():
     items = four_variable_meshes()
     (ca, ma, pa, ua), (cb, mb, pb, ub), (cc, mc, pc, uc) = items
