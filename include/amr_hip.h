@@ -231,6 +231,23 @@ typedef struct amr_kernel_time { char name[32]; double total_ms; int64_t launche
 int amr_hip_profile(amr_session* s, double tend, int32_t trials, amr_kernel_time* out, int32_t capacity,
                     int32_t* count, char* err, size_t err_len);
 
+/* One solve that allocates nothing.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (harnesses that forbid hipMalloc / hipFree, tight serving loops that
+ * must not touch the allocator).  The same result as amr_hip_create (graph 0, the given
+ * precision) + amr_hip_advance(tend, maxsteps, wall_budget_seconds) + amr_hip_download +
+ * amr_hip_destroy, bit for bit: u (host, nv * n_cells) holds the initial state on entry and the
+ * last accepted state on return, state carries time / steps / rejected in and out; returns 0,
+ * 1 (error, or a fatal code with the last accepted state written back) or 2 (wall budget).
+ * The arena lives in a static device array of the library, 64 MiB per device (about 70k cells
+ * in FP64); a larger mesh is rejected with a message.  No device, pinned or graph allocation:
+ * staging goes through pageable host memory on the null stream.  Every call on a device reuses
+ * that device's array and overwrites it (nothing carries over from one call to the next); calls
+ * that overlapped on one device would race for it, so calls from threads of one process take
+ * turns behind a per-device lock. */
+int amr_hip_run_noalloc(const amr_mesh_view* mesh, const amr_params* params, double* u, uint64_t u_len,
+                        amr_state_meta* state, double tend, int64_t maxsteps, double wall_budget_seconds,
+                        int32_t precision, char* err, size_t err_len);
+
 /* STREAM-copy probe: `repeats` launches of a device copy of `bytes` (read + write) in each of
  * four kernel variants (16 B per lane; 4 x 16 B per lane; 4 x 16 B non-temporal; 8 x 16 B
  * non-temporal), gbps[v] = 2 * bytes / time (capacity >= 4).  The achievable-bandwidth reference

@@ -304,6 +304,10 @@ def load_library(path: os.PathLike | None = None):
     lib.amr_hip_profile.restype = C.c_int
     lib.amr_hip_profile.argtypes = [H, C.c_double, C.c_int32, C.POINTER(amr_kernel_time), C.c_int32,
                                     C.POINTER(C.c_int32), C.c_char_p, C.c_size_t]
+    lib.amr_hip_run_noalloc.restype = C.c_int
+    lib.amr_hip_run_noalloc.argtypes = [C.POINTER(amr_mesh_view), C.POINTER(amr_params), c_double_p, C.c_uint64,
+                                        C.POINTER(amr_state_meta), C.c_double, C.c_int64, C.c_double, C.c_int32,
+                                        C.c_char_p, C.c_size_t]
     lib.amr_hip_copy_variants.restype = C.c_int
     lib.amr_hip_copy_variants.argtypes = [C.c_uint64, C.c_int32, c_double_p, C.c_int32, C.c_char_p, C.c_size_t]
     lib.amr_hip_copy_bandwidth.restype = C.c_int

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -439,4 +440,83 @@ extern "C" int amr_hip_copy_bandwidth(uint64_t bytes, int32_t repeats, double* g
   *gbps = *std::max_element(rates, rates + kCopyVariants);
   *ms_per_copy = 2.0 * static_cast<double>(bytes) / (*gbps * 1e9) * 1e3;
   return 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// amr_hip_run_noalloc: create + advance + download + destroy of one solve, with the arena laid
+// out in a file-scope static device array instead of hipMalloc'ed memory.  No device or pinned
+// allocation: the caller passes the mesh, parameters and state in host memory; staging goes
+// through pageable host memory on the null stream; trials are launched directly (no graph).
+namespace {
+
+// 64 MiB per device, held until the process ends: one arena of up to about 70k cells in FP64
+// (an LF2 mesh takes 8.5 MiB, a level-7 mesh 26 MiB) or twice that in FP32.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on each device (its host-side name is not a device address), and
+// one lock per device: calls from threads of this process on one device take turns, since each
+// rebuilds the arena in the same memory.
+std::mutex scratch_lock[kMaxDevices];
+void* scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+extern "C" int amr_hip_run_noalloc(const amr_mesh_view* mesh, const amr_params* params, double* u, uint64_t u_len,
+                                   amr_state_meta* state, double tend, int64_t maxsteps, double wall_budget_seconds,
+                                   int32_t precision, char* err, size_t err_len) {
+  if (!mesh || !params || !u || !state) return fail(err, err_len, "invalid AMR GPU configuration");
+  MeshInput in{};
+  in.view = mesh;
+  in.params = *params;
+  in.u = u;
+  in.meta = *state;
+  const uint64_t nv = mesh->transport.sa_model ? 5 : 4;
+  std::string message = hipamr::validate_input(in);
+  if (message.empty() && u_len != nv * mesh->n_cells) message = "invalid AMR GPU configuration";
+  if (message.empty() && precision != 64 && precision != 32) message = "precision must be 64 or 32";
+  if (message.empty() && (!std::isfinite(tend) || tend < state->time || maxsteps < 1 ||
+                          !std::isfinite(wall_budget_seconds) || wall_budget_seconds < 0))
+    message = "invalid AMR GPU advance";
+  if (!message.empty()) return fail(err, err_len, message);
+  if (amr_hip_device_count() < 1) return fail(err, err_len, "no HIP device available for the AMR backend");
+  int device = 0;
+  if (!checked(hipGetDevice(&device), err, err_len, "AMR device")) return 1;
+  if (device < 0 || device >= kMaxDevices) return fail(err, err_len, "AMR device index beyond the scratch table");
+  std::lock_guard<std::mutex> guard(scratch_lock[device]);
+  if (!scratch_ptr[device]) {  // once per device, with that device current
+    void* p = nullptr;
+    if (!checked(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), err, err_len, "AMR scratch address")) return 1;
+    scratch_ptr[device] = p;
+  }
+  const auto t0 = Clock::now();
+  const hipStream_t stream = nullptr;
+  // The same launch tile as a session of this mesh, so the two give the same bits.
+  const int tile = mesh->n_cells < 200000 ? 64 : 256;
+  std::unique_ptr<Arena> arena =
+      hipamr::make_arena({in}, precision, tile, stream, &message, scratch_ptr[device], kScratchBytes);
+  if (!arena) return fail(err, err_len, message);
+  Control host{};
+  hipamr::launch_target(arena->control(), 1, tend, maxsteps, nullptr, nullptr, stream);
+  arena->launch_prims(stream);
+  bool expired = false;
+  for (;;) {
+    for (int trial = 0; trial < 32; ++trial) arena->launch_trial(stream);
+    if (!checked(hipGetLastError(), err, err_len, "AMR trial kernels") ||
+        !checked(hipMemcpyAsync(&host, arena->control(), sizeof(Control), hipMemcpyDeviceToHost, stream), err, err_len,
+                 "AMR control D2H") ||
+        !checked(hipStreamSynchronize(stream), err, err_len, "AMR batch sync"))
+      return 1;
+    if (host.fatal || host.finished) break;
+    if (wall_budget_seconds > 0 && std::chrono::duration<double>(Clock::now() - t0).count() >= wall_budget_seconds) {
+      expired = true;
+      break;
+    }
+  }
+  if (!checked(arena->read_state(0, host.cur, u, stream), err, err_len, "AMR state D2H")) return 1;
+  to_meta(host, state);
+  if (host.fatal)
+    return fail(err, err_len, "AMR GPU stopped with status " + std::to_string(host.fatal) + "; last accepted state retained");
+  return expired ? 2 : 0;
 }
