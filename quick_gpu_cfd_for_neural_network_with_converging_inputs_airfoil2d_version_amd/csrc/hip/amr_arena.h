@@ -30,8 +30,12 @@ struct Control {
   int w_valid;               // W/PHI hold primitives of U[cur]
   int r0_valid;              // R0 holds L(U[cur]) (set after a rejection)
   int wbuf;                  // primitive-vector buffer (0..2) holding the primitives of U[cur]
+  // Blocks of the last stage pass that have finished this trial (low 32 bits) and how many of
+  // them found the candidate inadmissible (high 32 bits); the last one commits (zero between
+  // trials).
+  unsigned long long arrivals;
 };
-static_assert(sizeof(Control) == 104, "Control layout");
+static_assert(sizeof(Control) == 112, "Control layout");
 
 enum : int { VISC_VOLUME = 0, VISC_SRD_RATE = 1, VISC_SRD_GROUP = 2 };
 
@@ -68,6 +72,7 @@ template <class T>
 struct MeshDesc {
   int cell_begin, n_cells, face_begin, n_faces, wall_begin, n_walls;
   int first_cell_block;  // block index (in cell launches) of the mesh's first tile
+  int n_tiles;           // tiles of the slot: blocks of a cell launch that serve it
   int nv;                // 4, or 5 with SA-neg
   int laminar, sa, flux, small, visc_mode, agglomerated;
   double dt_fixed;
@@ -84,6 +89,9 @@ struct ArenaPtrs {
   int n_grid_blocks;  // blocks of a cell launch (>= n_cell_blocks, a multiple of 8)
   int commit_first, commit_stride;  // slots a trial's commit covers: first, first + stride, ...
   int any_laminar, any_sa, any_small, any_visc_group;
+  // The last stage pass commits the trial (its last block per slot); without SRD meshes only,
+  // whose last stage is redistributed after it (then k_commit follows the redistribution).
+  int commit_in_flux;
   const MeshDesc<T>* desc;
   Control* ctrl;
   const int* cell_block_mesh;  // n_cell_blocks

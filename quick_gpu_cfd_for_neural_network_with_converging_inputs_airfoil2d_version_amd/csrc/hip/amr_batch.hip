@@ -39,8 +39,9 @@ struct amr_batch {
   std::vector<hipEvent_t> joined;      // per side stream: its part of a batch is done
   Control* snap[2] = {nullptr, nullptr};  // pinned control snapshots of the pipelined advance
   hipEvent_t snap_done[2] = {nullptr, nullptr};
-  std::vector<hipGraph_t> graphs;      // one per branch
-  std::vector<hipGraphExec_t> execs;
+  std::vector<hipGraph_t> graphs;      // per branch: one trial, then batch_trials trials
+  std::vector<hipGraphExec_t> execs;   // per branch: one trial
+  std::vector<hipGraphExec_t> execs_batch;  // per branch: batch_trials trials in one graph
   Control* host_control = nullptr;  // pinned, n slots
   double* host_targets = nullptr;   // pinned, n slots
   long long* host_maxsteps = nullptr;
@@ -50,6 +51,7 @@ struct amr_batch {
     if (stream) (void)hipStreamSynchronize(stream);
     for (hipStream_t s : sides) (void)hipStreamSynchronize(s);
     for (hipGraphExec_t e : execs) (void)hipGraphExecDestroy(e);
+    for (hipGraphExec_t e : execs_batch) (void)hipGraphExecDestroy(e);
     for (hipGraph_t g : graphs) (void)hipGraphDestroy(g);
     arena.reset();
     if (host_control) (void)hipHostFree(host_control);
@@ -132,16 +134,15 @@ int queue_batch(amr_batch* b, int k) {
     const Control& c = b->host_control[m];
     busy[m % branches] |= !c.finished && !c.fatal;
   }
-  for (int trial = 0; trial < b->options.batch_trials; ++trial)
-    for (int j = 0; j < branches; ++j) {
-      if (!busy[j]) continue;
-      hipStream_t s = branch_stream(b, j);
-      if (!b->execs.empty()) {
-        if (hipGraphLaunch(b->execs[j], s) != hipSuccess) return 1;
-      } else {
-        b->arena->launch_trial_branch(j, s);
-      }
+  for (int j = 0; j < branches; ++j) {
+    if (!busy[j]) continue;
+    hipStream_t s = branch_stream(b, j);
+    if (!b->execs_batch.empty()) {  // the batch's trials as one graph
+      if (hipGraphLaunch(b->execs_batch[j], s) != hipSuccess) return 1;
+    } else {
+      for (int trial = 0; trial < b->options.batch_trials; ++trial) b->arena->launch_trial_branch(j, s);
     }
+  }
   for (size_t j = 0; j < b->sides.size(); ++j)
     if (hipEventRecord(b->joined[j], b->sides[j]) != hipSuccess ||
         hipStreamWaitEvent(b->stream, b->joined[j], 0) != hipSuccess)
@@ -230,6 +231,23 @@ int finish_setup(amr_batch* b, Clock::time_point t0, char* err, size_t err_len) 
       if (!checked(hipGraphGetNodes(g, nullptr, &nodes), err, err_len, "AMR graph node count")) return 1;
       b->stats.graph_nodes_per_trial += nodes;
     }
+    // The trials of one poll as a single graph per branch: consecutive graph launches on a
+    // stream leave a gap between them (about 6 us on one LF2 mesh, where a trial takes 70 us),
+    // consecutive kernels inside one graph none.
+    if (b->options.batch_trials > 1)
+      for (int j = 0; j < branches; ++j) {
+        hipStream_t s = branch_stream(b, j);
+        hipGraph_t g = nullptr;
+        hipGraphExec_t e = nullptr;
+        if (!checked(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), err, err_len, "AMR graph begin"))
+          return 1;
+        for (int trial = 0; trial < b->options.batch_trials; ++trial) b->arena->launch_trial_branch(j, s);
+        if (!checked(hipStreamEndCapture(s, &g), err, err_len, "AMR graph end")) return 1;
+        b->graphs.push_back(g);
+        if (!checked(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), err, err_len, "AMR graph instantiate"))
+          return 1;
+        b->execs_batch.push_back(e);
+      }
   }
   b->stats.setup_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
   return 0;
@@ -436,7 +454,20 @@ int amr_batch_advance(amr_batch* b, int32_t until, double wall_budget_seconds, a
 
 int amr_batch_run(amr_batch* b, int32_t trials, amr_slot_state* states, char* err, size_t err_len) {
   if (!b || trials < 1) return fail(err, err_len, "invalid AMR batch run");
-  for (int i = 0; i < trials; ++i)
+  int i = 0;
+  if (!b->execs_batch.empty() && trials >= b->options.batch_trials) {
+    if (!b->sides.empty() && b->side_must_wait) {  // as launch_one: side streams after host-side work
+      if (hipEventRecord(b->host_done, b->stream) != hipSuccess) return fail(err, err_len, "AMR trial launch failed");
+      for (hipStream_t s : b->sides)
+        if (hipStreamWaitEvent(s, b->host_done, 0) != hipSuccess) return fail(err, err_len, "AMR trial launch failed");
+      b->side_must_wait = false;
+    }
+    for (; i + b->options.batch_trials <= trials; i += b->options.batch_trials)
+      for (int j = 0; j < b->arena->n_branches(); ++j)
+        if (hipGraphLaunch(b->execs_batch[j], branch_stream(b, j)) != hipSuccess)
+          return fail(err, err_len, "AMR trial launch failed");
+  }
+  for (; i < trials; ++i)
     if (launch_one(b)) return fail(err, err_len, "AMR trial launch failed");
   if (!checked(hipGetLastError(), err, err_len, "AMR trial kernels") || poll(b, err, err_len)) return 1;
   fill_states(b, states);

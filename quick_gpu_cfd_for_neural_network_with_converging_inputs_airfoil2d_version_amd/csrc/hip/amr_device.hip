@@ -519,6 +519,7 @@ class DeviceArena final : public Arena {
     d.wall_begin = wb;
     d.n_walls = nw;
     d.first_cell_block = cb / tile_;
+    d.n_tiles = s.cap.cells / tile_;
     d.nv = sa ? 5 : 4;
     d.laminar = laminar;
     d.sa = sa;
@@ -681,6 +682,19 @@ class DeviceArena final : public Arena {
     for (int b = 0; b < n_cell_blocks_; ++b) {
       const MeshSlot& s = slots_[h.cell_block_mesh[b]];
       if (b * tile_ < s.cell_begin || b * tile_ >= s.cell_begin + s.cap.cells) return "layout: cell block range";
+    }
+    // The last stage pass commits a slot when n_tiles of its blocks have reported: every table
+    // must give each slot exactly that many blocks.
+    std::vector<int> per_slot(slots_.size() * tables, 0);
+    for (int t = 0; t < tables; ++t)
+      for (int i = table_offsets_[t]; i < table_offsets_[t + 1]; ++i)
+        if (h.block_tile[i] >= 0) ++per_slot[t * slots_.size() + h.cell_block_mesh[h.block_tile[i]]];
+    for (size_t m = 0; m < slots_.size(); ++m) {
+      if (h.desc[m].n_tiles != slots_[m].cap.cells / tile_ || h.desc[m].n_tiles != per_slot[m])
+        return "layout: slot tile count";
+      for (int t = 1; t < tables; ++t)
+        if (per_slot[t * slots_.size() + m] != (static_cast<int>(m) % branches == t - 1 ? h.desc[m].n_tiles : 0))
+          return "layout: slot tile count in a branch table";
     }
     for (int b = 0; b < n_face_blocks_ + n_wall_blocks_; ++b) {
       const MeshSlot& s = slots_[h.face_block_mesh[b]];
@@ -850,6 +864,7 @@ class DeviceArena final : public Arena {
     a.any_sa = features_.sa;
     a.any_small = features_.small;
     a.any_visc_group = features_.visc_group;
+    a.commit_in_flux = features_.small ? 0 : 1;
     for (const Column& c : columns_)
       if (hipMemcpyAsync(c.dev, c.host, c.elem * c.count, hipMemcpyHostToDevice, stream) != hipSuccess)
         return "AMR geometry H2D failed";
@@ -1159,6 +1174,7 @@ class DeviceArena final : public Arena {
     d.face_begin = s.face_begin;
     d.wall_begin = s.wall_begin;
     d.first_cell_block = s.cell_begin / tile_;
+    d.n_tiles = s.cap.cells / tile_;
     d.nv = features_.sa ? 5 : 4;
     d.gamma = T(1.4);
     h.desc[m] = d;

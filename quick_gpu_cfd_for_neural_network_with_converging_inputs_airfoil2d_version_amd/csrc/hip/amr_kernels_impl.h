@@ -644,6 +644,68 @@ __device__ __forceinline__ void wall_shear(const CellPrim<T>& donor, T nx, T ny,
 }
 
 // ------------------------------------------------------------------------------------------
+// Start the next trial: finished when the target is reached, fatal 3 at the step cap.
+__device__ __forceinline__ void begin_trial(Control* c) {
+  c->active = 0;
+  if (c->fatal || c->finished) return;
+  if (c->time >= c->target) {
+    c->finished = 1;
+    return;
+  }
+  if (c->steps >= c->maxsteps) {
+    c->fatal = 3;
+    return;
+  }
+  c->bad = 0;
+  c->active = 1;
+  if (!c->r0_valid) c->dt_min = kInfBits;
+}
+
+// Accept or reject the finished trial of a slot (reference commit_control_kernel,
+// src/static_amr2d_cuda.cu:212-216) and start the next; `bad`: the candidate's admissibility.
+__device__ __forceinline__ void commit_trial(Control* c, bool bad) {
+  if (c->active) {
+    if (c->fatal) {
+      c->w_valid = 0;
+    } else if (bad) {  // reject: same base state, half the step
+      ++c->rejected;
+      ++c->attempt;
+      c->retrying = 1;
+      c->dt = c->trial_dt * 0.5;
+      if (c->attempt >= 21) c->fatal = 5;
+      c->r0_valid = 1;  // R0 and the base primitives (buffer wbuf) still describe U[cur]
+    } else {  // accept: the candidate buffers become the state
+      const double remaining = c->target - c->time;
+      c->time = c->trial_dt == remaining ? c->target : c->time + c->trial_dt;
+      ++c->steps;
+      c->dt = c->trial_dt;
+      c->retrying = 0;
+      c->attempt = 0;
+      c->cur ^= 1;
+      c->wbuf = (c->wbuf + 1) % 3;
+      c->w_valid = 1;
+      c->r0_valid = 0;
+      c->finished = c->time >= c->target;
+    }
+  }
+  begin_trial(c);
+}
+
+// A block of the last stage pass has finished (`failed`: some thread of it found the candidate
+// inadmissible).  One 64-bit atomic per block counts the blocks (low word) and the failed ones
+// (high word); the block that completes the count commits the trial and clears the count.  Every
+// other block has made its last access to the slot's data and control by then, and the flags
+// raised in earlier launches are visible, so the commit needs no memory fence.
+__device__ __forceinline__ void arrive_and_commit(Control* c, int n_blocks, bool failed) {
+  const unsigned long long add = 1ull | (failed ? (1ull << 32) : 0ull);
+  const unsigned long long before = atomicAdd(&c->arrivals, add);
+  if (static_cast<int>(before & 0xffffffffull) + 1 != n_blocks) return;
+  const bool any_failed = (before >> 32) != 0 || failed;
+  c->arrivals = 0;
+  commit_trial(c, c->bad != 0 || any_failed);
+}
+
+// ------------------------------------------------------------------------------------------
 // Residual of every cell from its faces and wall fragments.  STAGE 0: base residual R0 and the
 // local dt.  1, 2: second and third SSP-RK3 combinations, then the candidate's primitives.
 // FP32 register budget: 6 waves per SIMD in the base pass (80 VGPRs; it needs 83) and 7 in the
@@ -658,7 +720,11 @@ void k_flux(ArenaPtrs<T> a) {
   if (tile < 0) return;
   const int m = a.cell_block_mesh[tile];
   Control* c = a.ctrl + m;
-  if (!pass_live(c, BASE ? PASS_BASE : PASS_STAGE)) return;
+  // The last stage pass commits the trial: every block of a slot with an active trial reports
+  // (also when the trial has already failed), the last one commits.
+  const bool commits = STAGE == 2 && a.commit_in_flux;
+  if (commits ? !c->active : !pass_live(c, BASE ? PASS_BASE : PASS_STAGE)) return;
+  const bool live = running(c);
   const MeshDesc<T>& d = a.desc[m];
   const int nc = a.nc;
   const int k = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
@@ -671,11 +737,13 @@ void k_flux(ArenaPtrs<T> a) {
   const int staged = min(tn, d.cell_begin + d.n_cells - first);  // valid cells of the tile
   const TileCells<T> lds(tn);
   const FaceSlots<T> slots(tn, BASE);
-  __shared__ int failed;  // block snapshot of the slot's admissibility bits
-  if (tid == 0) failed = c->bad;
-  if (valid) stage_cell(lds, tid, load_prim(a, rb, k));
+  __shared__ int failed;  // block snapshot of the slot's admissibility bits (or the trial ended)
+  if (tid == 0) failed = live ? c->bad : 1;
+  if (valid && live) stage_cell(lds, tid, load_prim(a, rb, k));
   __syncthreads();
-  if (failed) return;
+  int bad = 0;  // admissibility bits this thread raises
+  double local_dt = INFINITY;
+  if (!failed) {
   // Face items of the tile, one chunk at a time: (A) each face touching the tile is evaluated
   // once, by one thread, into its LDS slot; (B) each cell sums its faces' slots in its cell-face
   // order.  A face shared with another tile is evaluated by both tiles from the same inputs, so
@@ -750,9 +818,8 @@ void k_flux(ArenaPtrs<T> a) {
     }
     __syncthreads();  // the next chunk reuses the slots
   }
-  if (!ok) flag_bad(c, bit);  // a face of the tile, whichever thread evaluated it
+  if (!ok) bad |= bit;  // a face of the tile, whichever thread evaluated it
   ok = true;
-  double local_dt = INFINITY;
   if (valid) {
     // The cell's own record is read back from LDS rather than kept in registers across the
     // face loop (fewer live registers there).
@@ -785,13 +852,13 @@ void k_flux(ArenaPtrs<T> a) {
         }
       }
     }
-    if (!ok) flag_bad(c, bit);
+    if (!ok) bad |= bit;
     const T iv = a.inv_vol[k];
     const T r0 = -s0 * iv, r1 = -s1 * iv, r2 = -s2 * iv, r3 = -s3 * iv;
     T r4 = T(0);
     if (sa) {
       r4 = -sa_net * iv + a.GREC[8 * k + 7];
-      if (!isfinite(r4)) flag_bad(c, bit);
+      if (!isfinite(r4)) bad |= bit;
     }
     if (BASE) {
       a.R0[k] = r0;
@@ -842,7 +909,7 @@ void k_flux(ArenaPtrs<T> a) {
         }
       }
       CellPrim<T> w;
-      if (!make_prim_q(q, d, w)) flag_bad(c, BAD_STAGE);
+      if (!make_prim_q(q, d, w)) bad |= BAD_STAGE;
       else store_prim(a, write_buffer(c, STAGE), k, w);
     } else {
       // SSP-RK3: u2 = 3/4 u + 1/4 (u1 + dt r)  or  u1 = 1/3 u + 2/3 (u2 + dt r).
@@ -859,12 +926,19 @@ void k_flux(ArenaPtrs<T> a) {
       if (sa) out[4 * nc + k] = wu * u[4 * nc + k] + wp * (prev[4 * nc + k] + dt * static_cast<double>(r4));
       if (!d.small) {
         CellPrim<T> w;
-        if (!make_prim(out, nc, k, d, w)) flag_bad(c, BAD_STAGE);
+        if (!make_prim(out, nc, k, d, w)) bad |= BAD_STAGE;
         else store_prim(a, write_buffer(c, STAGE), k, w);
       }
     }
   }
+  }  // !failed
   if (BASE && d.visc_mode != VISC_SRD_GROUP) publish_dt(c, local_dt);
+  if (commits) {
+    const int any = __syncthreads_or(bad);
+    if (tid == 0) arrive_and_commit(c, d.n_tiles, any != 0);
+  } else if (bad) {
+    flag_bad(c, bad);
+  }
 }
 
 // SRD-grouped viscous limit: group means of the diffusion rate, then the per-cell limit
@@ -996,53 +1070,12 @@ __global__ __launch_bounds__(kMaxTile) void k_redist(ArenaPtrs<T> a, int stage) 
 }
 
 // ------------------------------------------------------------------------------------------
-// Start the next trial: finished when the target is reached, fatal 3 at the step cap.
-__device__ __forceinline__ void begin_trial(Control* c) {
-  c->active = 0;
-  if (c->fatal || c->finished) return;
-  if (c->time >= c->target) {
-    c->finished = 1;
-    return;
-  }
-  if (c->steps >= c->maxsteps) {
-    c->fatal = 3;
-    return;
-  }
-  c->bad = 0;
-  c->active = 1;
-  if (!c->r0_valid) c->dt_min = kInfBits;
-}
-
 // Commits slots first, first + stride, ... (< n): a branch of the trial commits its own slots.
 __global__ void k_commit(Control* ctrl, int n, int first, int stride) {
   const int m = first + stride * static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   if (m >= n) return;
   Control* c = ctrl + m;
-  if (c->active) {
-    if (c->fatal) {
-      c->w_valid = 0;
-    } else if (c->bad) {  // reject: same base state, half the step
-      ++c->rejected;
-      ++c->attempt;
-      c->retrying = 1;
-      c->dt = c->trial_dt * 0.5;
-      if (c->attempt >= 21) c->fatal = 5;
-      c->r0_valid = 1;  // R0 and the base primitives (buffer wbuf) still describe U[cur]
-    } else {  // accept: the candidate buffers become the state
-      const double remaining = c->target - c->time;
-      c->time = c->trial_dt == remaining ? c->target : c->time + c->trial_dt;
-      ++c->steps;
-      c->dt = c->trial_dt;
-      c->retrying = 0;
-      c->attempt = 0;
-      c->cur ^= 1;
-      c->wbuf = (c->wbuf + 1) % 3;
-      c->w_valid = 1;
-      c->r0_valid = 0;
-      c->finished = c->time >= c->target;
-    }
-  }
-  begin_trial(c);
+  commit_trial(c, c->bad != 0);
 }
 
 __global__ void k_target(Control* ctrl, int n, double target, long long maxsteps, const double* targets,
@@ -1197,7 +1230,7 @@ int launch_trial(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
     HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 2);
     HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 2);
   }
-  {
+  if (!a.commit_in_flux) {
     const dim3 tile(256);
     const int count = (a.n_meshes - a.commit_first + a.commit_stride - 1) / a.commit_stride;
     HIPAMR_LAUNCH("commit", k_commit, dim3(blocks_of(count, 256)), a.ctrl, a.n_meshes, a.commit_first,

@@ -29,6 +29,8 @@ struct amr_session {
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipGraph_t graph_batch = nullptr;      // batch_trials trials in one graph
+  hipGraphExec_t exec_batch = nullptr;
   hipEvent_t start = nullptr, end = nullptr;
   Control* host_control = nullptr;  // pinned
   int device = 0;
@@ -39,6 +41,8 @@ struct amr_session {
     if (stream) (void)hipStreamSynchronize(stream);
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (exec_batch) (void)hipGraphExecDestroy(exec_batch);
+    if (graph_batch) (void)hipGraphDestroy(graph_batch);
     if (start) (void)hipEventDestroy(start);
     if (end) (void)hipEventDestroy(end);
     arena.reset();
@@ -65,6 +69,18 @@ int launch_one(amr_session* s) {
   if (s->options.graph && s->exec) return hipGraphLaunch(s->exec, s->stream) == hipSuccess ? 0 : 1;
   s->arena->launch_trial(s->stream);
   return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+// `trials` trials: whole poll batches as one graph each (no gap between their trials), the rest
+// one by one.
+int launch_trials(amr_session* s, int trials) {
+  int i = 0;
+  if (s->options.graph && s->exec_batch)
+    for (; i + s->options.batch_trials <= trials; i += s->options.batch_trials)
+      if (hipGraphLaunch(s->exec_batch, s->stream) != hipSuccess) return 1;
+  for (; i < trials; ++i)
+    if (launch_one(s)) return 1;
+  return 0;
 }
 
 }  // namespace
@@ -133,6 +149,17 @@ int amr_hip_create(const amr_mesh_view* mesh, const amr_params* params, const do
     size_t nodes = 0;
     if (!checked(hipGraphGetNodes(s->graph, nullptr, &nodes), err, err_len, "AMR graph node count")) return 1;
     s->stats.graph_nodes_per_trial = nodes;
+    // The trials of one poll as a single graph (consecutive graph launches leave a gap between
+    // them, consecutive kernels inside one graph none).
+    if (options->batch_trials > 1) {
+      if (!checked(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), err, err_len, "AMR graph begin"))
+        return 1;
+      for (int t = 0; t < options->batch_trials; ++t) s->arena->launch_trial(s->stream);
+      if (!checked(hipStreamEndCapture(s->stream, &s->graph_batch), err, err_len, "AMR graph end") ||
+          !checked(hipGraphInstantiate(&s->exec_batch, s->graph_batch, nullptr, nullptr, 0), err, err_len,
+                   "AMR graph instantiate"))
+        return 1;
+    }
   }
   s->stats.setup_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
   *out = s.release();
@@ -152,8 +179,7 @@ int amr_hip_advance(amr_session* s, double tend, int64_t maxsteps, double wall_b
   for (;;) {
     if (s->options.event_timing && !checked(hipEventRecord(s->start, s->stream), err, err_len, "AMR timing record start"))
       return 1;
-    for (int trial = 0; trial < s->options.batch_trials; ++trial)
-      if (launch_one(s)) return fail(err, err_len, "AMR trial launch failed");
+    if (launch_trials(s, s->options.batch_trials)) return fail(err, err_len, "AMR trial launch failed");
     if (s->options.event_timing && !checked(hipEventRecord(s->end, s->stream), err, err_len, "AMR timing record end"))
       return 1;
     if (!checked(hipGetLastError(), err, err_len, "AMR trial kernels") ||
@@ -317,8 +343,7 @@ int amr_hip_run(amr_session* s, double tend, int32_t trials, amr_state_meta* sta
     return fail(err, err_len, "invalid AMR GPU run");
   hipamr::launch_target(s->arena->control(), 1, tend, INT64_MAX, nullptr, nullptr, s->stream);
   s->arena->launch_prims(s->stream);  // base primitives when the state was written by the host
-  for (int i = 0; i < trials; ++i)
-    if (launch_one(s)) return fail(err, err_len, "AMR trial launch failed");
+  if (launch_trials(s, trials)) return fail(err, err_len, "AMR trial launch failed");
   if (!checked(hipGetLastError(), err, err_len, "AMR trial kernels") ||
       !checked(copy_sync(s->host_control, s->arena->control(), sizeof(Control), hipMemcpyDeviceToHost, s->stream), err,
                err_len, "AMR control D2H"))

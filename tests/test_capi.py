@@ -79,3 +79,8 @@ def test_device_layout_is_consistent(tile):
     assert status == 0, message
     status, message = check_layout(items[1:], tile)
     assert status == 0, message
+    # Four slots: three trial branches, the first with two slots (per-branch block tables, and
+    # every slot's tile count in each, which the last stage pass's commit counts on).
+    status, message = check_layout(items[1:] + [case_mesh("G0100", "LF1", LAMINAR), case_mesh("G0010", "LF1", LAMINAR)],
+                                   tile)
+    assert status == 0, message
