@@ -3,7 +3,7 @@
 // One arena of slots.  With two or more slots the trial runs as independent branches (slot m in
 // branch m % branches), each captured as its own graph and replayed on its own stream, so that
 // one branch's kernels fill the GPU while another's drain; the streams join at every poll.  The
-// host polls the array of control records (104 bytes per slot) every batch_trials trials; in
+// host polls the array of control records (112 bytes per slot) every batch_trials trials; in
 // amr_batch_advance the polls are pipelined through two pinned snapshots (queue_batch).  Slot
 // independence: every kernel reads only its own slot's ranges and control record, the only cross-block
 // reductions (minimum dt, monitor partials) are per slot and exact (atomicMin on the bit

@@ -2,7 +2,7 @@
 //
 // Semantics follow the reference backend (src/static_amr2d_cuda.cu:219-366): create validates,
 // uploads and captures one trial as a graph; advance replays it batch_trials times per host
-// poll of the 104-byte control record until the target time is reached (0), a fatal code is
+// poll of the 112-byte control record until the target time is reached (0), a fatal code is
 // raised (1, last accepted state retained) or the wall budget expires (2); download copies
 // the accepted state; stats reports the same counters as AmrGpuStats.
 #include <algorithm>
