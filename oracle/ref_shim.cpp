@@ -10,10 +10,21 @@
 //   ref_wall_flux    static_amr2d_viscous_diagnostics (src/static_amr2d_viscous.cpp:137-142)
 //   ref_boundary_state / ref_boundary_valid
 //                    amr_bc_state / amr_bc_valid (src/static_amr2d_boundary.h:32-102)
+// and, for the legacy tensor-product grid solver (include/ugrid_hip.h):
+//   ref_grid_geometry  solver2d_init's grid (src/euler2d.cpp:38-151)
+//   ref_grid_apply_bc  solver2d_apply_bc (src/euler2d.cpp:221-247)
+//   ref_grid_rhs       solver2d_apply_bc + solver2d_residual (src/euler2d.cpp:249-489)
+//   ref_grid_run       solver2d_run, the CPU SSP-RK3 driver (src/euler2d.cpp:595-698)
+//   ref_flux           flux_hllc_t / flux_ausmpup_t (src/flux.h:41-176)
+#include <cstddef>
 #include <cstring>
+#include <memory>
 #include <string>
 
 #include "amr_hip.h"
+#include "ugrid_hip.h"
+#include "euler2d.h"
+#include "flux.h"
 #include "static_amr2d.h"
 #include "static_amr2d_boundary.h"
 #include "static_amr2d_sa.h"
@@ -240,6 +251,118 @@ int ref_boundary_valid(const amr_boundary* b, int legacy, int side) {
   AmrBoundary2D rb;
   std::memcpy(&rb, b, sizeof(rb));
   return amr_bc_valid(rb, legacy, side) ? 1 : 0;
+}
+
+
+// ------------------------------------------------------------ legacy grid solver --
+static Params2D ref_grid_params(const ugrid_params& c, const int* stretch_type, const double* stretch_param) {
+  Params2D p = params2d_default();
+  p.gamma = c.gamma;
+  p.cfl = c.cfl;
+  p.flux = c.flux;
+  p.limiter = c.limiter;
+  p.ausm_mco = c.ausm_mco;
+  for (int k = 0; k < 4; ++k) p.bc[k] = c.bc[k];
+  p.dt_fixed = c.dt_fixed;
+  p.time_integrator = c.time_integrator;
+  p.pseudo_cfl = c.pseudo_cfl;
+  p.implicit_maxiter = c.implicit_maxiter;
+  p.implicit_rtol = c.implicit_rtol;
+  p.implicit_atol = c.implicit_atol;
+  p.sx.type = stretch_type ? stretch_type[0] : STRETCH_UNIFORM;
+  p.sx.param = stretch_param ? stretch_param[0] : 0.0;
+  p.sy.type = stretch_type ? stretch_type[1] : STRETCH_UNIFORM;
+  p.sy.param = stretch_param ? stretch_param[1] : 0.0;
+  return p;
+}
+
+// box = xmin, xmax, ymin, ymax
+static std::unique_ptr<Solver2D> ref_grid_solver(int nx, int ny, const double* box, int ncase, const ugrid_params* p,
+                                                 const int* stretch_type, const double* stretch_param) {
+  std::unique_ptr<Solver2D> s(new Solver2D());
+  solver2d_init(s.get(), nx, ny, box[0], box[1], box[2], box[3], ncase, ref_grid_params(*p, stretch_type, stretch_param));
+  return s;
+}
+
+int ref_grid_geometry(int nx, int ny, const double* box, const ugrid_params* p, const int* stretch_type,
+                      const double* stretch_param, double* xf, double* xc, double* dx, double* yf, double* yc,
+                      double* dy) {
+  auto s = ref_grid_solver(nx, ny, box, 1, p, stretch_type, stretch_param);
+  const Grid2D& g = s->grid;
+  std::memcpy(xf, g.xf.data(), g.xf.size() * sizeof(double));
+  std::memcpy(xc, g.xc.data(), g.xc.size() * sizeof(double));
+  std::memcpy(dx, g.dx.data(), g.dx.size() * sizeof(double));
+  std::memcpy(yf, g.yf.data(), g.yf.size() * sizeof(double));
+  std::memcpy(yc, g.yc.data(), g.yc.size() * sizeof(double));
+  std::memcpy(dy, g.dy.data(), g.dy.size() * sizeof(double));
+  return 0;
+}
+
+void ref_grid_apply_bc(int nx, int ny, int ncase, const ugrid_params* p, double* u) {
+  const double box[4] = {0.0, 1.0, 0.0, 1.0};
+  auto s = ref_grid_solver(nx, ny, box, ncase, p, nullptr, nullptr);
+  std::memcpy(s->U.data(), u, s->U.size() * sizeof(double));
+  solver2d_apply_bc(s.get(), s->U.data());
+  std::memcpy(u, s->U.data(), s->U.size() * sizeof(double));
+}
+
+int ref_grid_rhs(int nx, int ny, const double* box, int ncase, const ugrid_params* p, const int* stretch_type,
+                 const double* stretch_param, const double* mco, const double* u, double* r) {
+  auto s = ref_grid_solver(nx, ny, box, ncase, p, stretch_type, stretch_param);
+  if (mco)
+    for (int c = 0; c < ncase; ++c) solver2d_set_mco(s.get(), c, mco[c]);
+  std::memcpy(s->U.data(), u, s->U.size() * sizeof(double));
+  solver2d_apply_bc(s.get(), s->U.data());
+  solver2d_residual(s.get(), s->U.data(), s->R.data(), s->W.data(), s->SX.data(), s->SY.data(), s->FX.data(),
+                    s->FY.data());
+  std::memcpy(r, s->R.data(), s->R.size() * sizeof(double));
+  return 0;
+}
+
+int ref_grid_run(int nx, int ny, const double* box, int ncase, const ugrid_params* p, const int* stretch_type,
+                 const double* stretch_param, const double* mco, double* u, double* t, double* dt, double tend,
+                 int64_t maxsteps, int64_t* nsteps, int64_t* nreject) {
+  auto s = ref_grid_solver(nx, ny, box, ncase, p, stretch_type, stretch_param);
+  for (int c = 0; c < ncase; ++c) {
+    if (mco) solver2d_set_mco(s.get(), c, mco[c]);
+    s->t[c] = t[c];
+    s->dt[c] = dt[c];
+  }
+  std::memcpy(s->U.data(), u, s->U.size() * sizeof(double));
+  const int status = solver2d_run(s.get(), tend, static_cast<long>(maxsteps));
+  std::memcpy(u, s->U.data(), s->U.size() * sizeof(double));
+  for (int c = 0; c < ncase; ++c) {
+    t[c] = s->t[c];
+    dt[c] = s->dt[c];
+  }
+  *nsteps = s->nsteps;
+  *nreject = s->nreject;
+  return status;
+}
+
+// The same list as ugrid_solver2d_layout (include/ugrid_hip.h), measured on the reference's types.
+int ref_solver2d_layout(int64_t* out, int32_t capacity) {
+  const int64_t v[] = {
+      sizeof(Solver2D), sizeof(Grid2D), sizeof(Array), sizeof(Params2D), sizeof(ImplicitStats2D),
+      offsetof(Solver2D, grid), offsetof(Grid2D, nx), offsetof(Grid2D, ng), offsetof(Grid2D, mx), offsetof(Grid2D, my),
+      offsetof(Grid2D, xmin), offsetof(Grid2D, xf), offsetof(Grid2D, yf), offsetof(Grid2D, xc), offsetof(Grid2D, dx),
+      offsetof(Grid2D, yc), offsetof(Grid2D, dy), offsetof(Solver2D, par), offsetof(Solver2D, ncase),
+      offsetof(Solver2D, U), offsetof(Solver2D, U1), offsetof(Solver2D, U2), offsetof(Solver2D, R),
+      offsetof(Solver2D, W), offsetof(Solver2D, SX), offsetof(Solver2D, SY), offsetof(Solver2D, FX),
+      offsetof(Solver2D, FY), offsetof(Solver2D, t), offsetof(Solver2D, dt), offsetof(Solver2D, mco),
+      offsetof(Solver2D, nsteps), offsetof(Solver2D, nreject), offsetof(Solver2D, implicit), offsetof(Solver2D, ib),
+      offsetof(Solver2D, cut)};
+  const int32_t n = static_cast<int32_t>(sizeof(v) / sizeof(v[0]));
+  for (int32_t k = 0; k < n && k < capacity; ++k) out[k] = v[k];
+  return n;
+}
+
+// kind 0 HLLC, 1 AUSM+-up; states (rho, normal velocity, tangential velocity, p).
+void ref_flux(int kind, const double* L, const double* R, double gamma, double mco, double* F) {
+  if (kind == 1)
+    flux_ausmpup_t(L[0], L[1], L[2], L[3], R[0], R[1], R[2], R[3], gamma, mco, F);
+  else
+    flux_hllc_t(L[0], L[1], L[2], L[3], R[0], R[1], R[2], R[3], gamma, F);
 }
 
 }  // extern "C"

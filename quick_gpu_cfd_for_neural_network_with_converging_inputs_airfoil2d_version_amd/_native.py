@@ -267,6 +267,47 @@ def _bind_batch_api(lib):
         fn.argtypes = argtypes
 
 
+class ugrid_params(C.Structure):
+    """ugrid_hip.h: the solver fields of the reference's Params2D (src/euler2d.h:38-52)."""
+    _fields_ = [("gamma", C.c_double), ("cfl", C.c_double), ("flux", C.c_int32), ("limiter", C.c_int32),
+                ("ausm_mco", C.c_double), ("bc", C.c_int32 * 4), ("dt_fixed", C.c_double),
+                ("time_integrator", C.c_int32), ("implicit_maxiter", C.c_int32), ("pseudo_cfl", C.c_double),
+                ("implicit_rtol", C.c_double), ("implicit_atol", C.c_double)]
+
+
+class ugrid_setup(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("ncase", C.c_int32), ("pad", C.c_int32),
+                ("dx", c_double_p), ("dy", c_double_p), ("params", ugrid_params), ("mco", c_double_p)]
+
+
+class ugrid_meta(C.Structure):
+    _fields_ = [("nsteps", C.c_int64), ("nreject", C.c_int64)]
+
+
+class ugrid_launch_info(C.Structure):
+    _fields_ = [("rows", C.c_int32), ("strips", C.c_int32), ("blocks", C.c_int32), ("threads", C.c_int32)]
+
+
+def _bind_ugrid_api(lib):
+    H = C.c_void_p
+    sig = {
+        "ugrid_axis": (C.c_int, [C.c_int32, C.c_double, C.c_double, C.c_int32, C.c_double, C.c_int32, C.c_int32,
+                                 c_double_p, c_double_p, c_double_p, C.c_char_p, C.c_size_t]),
+        "ugrid_create": (C.c_int, [C.POINTER(ugrid_setup), c_double_p, C.c_uint64, c_double_p, c_double_p,
+                                   C.POINTER(H), C.c_char_p, C.c_size_t]),
+        "ugrid_run": (C.c_int, [H, C.c_double, C.c_int64, C.POINTER(ugrid_meta), C.c_char_p, C.c_size_t]),
+        "ugrid_download": (C.c_int, [H, c_double_p, C.c_uint64, c_double_p, c_double_p, C.c_char_p, C.c_size_t]),
+        "ugrid_rhs": (C.c_int, [H, c_double_p, C.c_uint64, c_double_p, C.c_char_p, C.c_size_t]),
+        "ugrid_destroy": (None, [H]),
+        "ugrid_get_launch": (C.c_int, [H, C.POINTER(ugrid_launch_info)]),
+        "ugrid_device_available": (C.c_int, [C.c_char_p, C.c_size_t]),
+    }
+    for name, (restype, argtypes) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = restype
+        fn.argtypes = argtypes
+
+
 _LIB = None
 
 
@@ -321,6 +362,7 @@ def load_library(path: os.PathLike | None = None):
     lib.amr_hip_solve.argtypes = [H, C.POINTER(amr_steady_options), C.POINTER(amr_steady_result),
                                   C.POINTER(amr_steady_check), C.c_int32, C.c_char_p, C.c_size_t]
     _bind_batch_api(lib)
+    _bind_ugrid_api(lib)
     lib.amr_hip_check_layout.restype = C.c_int
     lib.amr_hip_check_layout.argtypes = [C.POINTER(amr_mesh_view), C.POINTER(amr_params), C.c_int32, C.c_int32,
                                          C.c_char_p, C.c_size_t]

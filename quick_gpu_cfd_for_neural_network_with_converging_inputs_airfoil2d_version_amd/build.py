@@ -21,18 +21,25 @@ DRIVER = PACKAGE / "quick_airfoil_hip"
 DRIVER_SOURCE = "host/quick_airfoil_hip.cpp"
 CALLER_SOURCE = ROOT / "tests" / "capi" / "api_caller.cpp"
 CALLER = ROOT / "tests" / "capi" / "api_caller"
+# and of the legacy-grid backend API (euler2d_gpu.h)
+GRID_CALLER_SOURCE = ROOT / "tests" / "capi" / "ugrid_caller.cpp"
+GRID_CALLER = ROOT / "tests" / "capi" / "ugrid_caller"
 OBJ = ROOT / "_build" / "obj"
 
 HOST_SOURCES = ["host/cutcell.cpp", "host/mesher.cpp", "host/viscous_prep.cpp", "host/mesh_capi.cpp",
-                "host/cfd_api.cpp", "host/steady.cpp", "host/batch_steady.cpp"]
+                "host/cfd_api.cpp", "host/steady.cpp", "host/batch_steady.cpp", "host/ugrid_axis.cpp",
+                "host/euler2d_api.cpp"]
 HIP_SOURCES = ["hip/amr_kernels_f64.hip", "hip/amr_kernels_f32.hip", "hip/amr_device.hip", "hip/amr_session.hip",
-               "hip/amr_batch.hip"]
+               "hip/amr_batch.hip", "hip/ugrid.hip"]
 # Per-source extra flags: FP32 quotients and square roots of the FP32 kernels are not correctly
 # rounded (about 1 ulp: reciprocal and multiply) instead of the scaled IEEE sequences, and FP32
 # denormals flush to zero, so that a quotient is v_rcp + v_mul without the frexp / ldexp range
 # scaling (8% fewer VALU instructions in the flux kernels).  Flow quantities never come near
 # 1e-38; the FP64 state, control and reductions keep full IEEE behaviour.
-HIP_EXTRA = {"hip/amr_kernels_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"]}
+# The tensor-product grid solver is compiled without FMA contraction: its arithmetic follows the
+# reference's formulas operation for operation, so results are bitwise its CPU restatement's.
+HIP_EXTRA = {"hip/amr_kernels_f32.hip": ["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"],
+             "hip/ugrid.hip": ["-ffp-contract=off"]}
 INCLUDES = ["-I", str(ROOT / "include"), "-I", str(CSRC / "host"), "-I", str(CSRC / "hip")]
 ARCH = os.environ.get("HIPAMR_ARCH", "gfx950")
 
@@ -82,14 +89,16 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
     _run(["g++", str(driver_obj), "-L", str(PACKAGE), "-l:libamr_hip.so", "-L", "/opt/rocm/lib",
           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib", "-o", str(tmp)], verbose)
     tmp.replace(DRIVER)
-    # The test suite's caller of the C++ backend API (a reference-style caller of
-    # static_amr2d_gpu.h linked against the library), when the tests are present.
-    if CALLER_SOURCE.exists():
-        tmp = CALLER.with_suffix(".tmp")
-        _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", str(ROOT / "include"), str(CALLER_SOURCE), "-L", str(PACKAGE),
+    # The test suite's callers of the C++ backend APIs (reference-style callers of
+    # static_amr2d_gpu.h and euler2d_gpu.h linked against the library), when the tests are present.
+    for source, target in ((CALLER_SOURCE, CALLER), (GRID_CALLER_SOURCE, GRID_CALLER)):
+        if not source.exists():
+            continue
+        tmp = target.with_suffix(".tmp")
+        _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", str(ROOT / "include"), str(source), "-L", str(PACKAGE),
               "-l:libamr_hip.so", "-L", "/opt/rocm/lib", f"-Wl,-rpath,$ORIGIN/../../{PACKAGE.name}",
               "-Wl,-rpath,/opt/rocm/lib", "-o", str(tmp)], verbose)
-        tmp.replace(CALLER)
+        tmp.replace(target)
     return OUT
 
 

@@ -1,0 +1,895 @@
+// ugrid.hip -- batched first-order Euler solver on a shared tensor-product grid (gfx950, FP64):
+// the device side of ugrid_hip.h, replacing the reference's legacy CUDA backend
+// (src/euler2d_cuda.cu:153-447 kernels, :530-708 driver).
+//
+// Design.  The reference launches, per RK stage, separate primitive, x-flux, y-flux, collect,
+// RK-combine and admissibility kernels with a host round trip per stage (flag copy) and per
+// step (smax copy, dt upload).  Here one kernel per RK stage does all of it and the step
+// control lives on the device:
+//   * a wave owns a strip of 62 columns x `rows` rows of one case.  Lane k holds column
+//     i0 - 1 + k, so lanes 0 and 63 are the strip's x halo; each lane computes the flux of the
+//     face west of its cell (left state from lane k - 1 by a cross-lane shuffle) and takes its
+//     east flux from lane k + 1, so every x face of the strip is evaluated once by 64 lanes at
+//     a time.  The wave marches north through its rows, keeping the previous row's primitives,
+//     x-residual and y-face flux in registers, so each y face is evaluated once and a cell's
+//     state is read once per stage (plus one halo row per strip).  No LDS, no barriers.
+//   * boundary ghosts are never materialised: a halo lane or row outside the domain reads the
+//     mapped interior cell (periodic wrap, transmissive clamp, reflect mirror) and negates the
+//     normal velocity for a reflection -- bit for bit the primitives of the reference's filled
+//     ghost (negation is exact).
+//   * the RK update, the admissibility test of the candidate and, in the last stage, the
+//     candidate's wave-speed bound for the next step's dt are fused into the same pass.  The
+//     last block of the last stage (one counted atomic) commits or rejects the trial for all
+//     cases: t += dt, step counters, buffer swap (U[cur] and the other buffer alternate, so no
+//     accept copy), next dt from the bound, or dt halving and the 12-retry limit.
+//   * a trial is three launches; the host replays a graph of several trials and polls the
+//     control record between replays.
+// The arithmetic of every formula follows the reference's GPU kernels operation for operation
+// and this file is compiled without FMA contraction, so results are bitwise those of the CPU
+// restatement of that backend.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ugrid_hip.h"
+
+namespace ugrid {
+
+constexpr int kLanes = 64;
+constexpr int kStripCells = kLanes - 2;  // columns a strip updates
+constexpr int kWaves = 4;                // strips per block
+constexpr int kThreads = kLanes * kWaves;
+constexpr int kGhosts = 2;
+constexpr int kXcds = 8;
+constexpr int kMaxRetries = 12;
+enum : int { BC_PERIODIC = 0, BC_TRANSMISSIVE = 1, BC_REFLECT = 2 };
+enum : int { RUNNING = -1 };
+
+// Step control of a batch (one record, device memory).
+struct Control {
+  double tend;
+  long long maxsteps;  // steps allowed in the current run call
+  long long steps;     // accepted in the current run call
+  long long nsteps, nreject;
+  int status;      // RUNNING, or the run's result 0 / 1 / 2 (also 0 before the first run)
+  int bad;         // a stage found a nonphysical candidate cell (this trial)
+  int retries;     // rejections of the current step
+  int cur;         // U[cur] holds the accepted state
+  int ghost_from;  // download ghosts: 0 as uploaded, 1 boundary fill of U[cur], 2 of U[cur ^ 1]
+  unsigned arrivals;  // blocks of the last stage that finished this trial
+};
+
+struct Grid {
+  int nx, ny, mx, my, ncase;
+  int rows;            // rows per strip
+  int strips_x, strips_y, strips;
+  int blocks;          // blocks that own strips (the launch is padded to a multiple of 8)
+  int bc[4];
+  int ausm;
+  double gamma, cfl, dt_fixed;
+  size_t plane;        // mx * my
+  const double* dx;    // mx widths (ghosts included)
+  const double* dy;    // my
+  const double* mco;   // ncase
+  double* U[2];
+  double* S2;          // second-stage candidate
+  double* t;
+  double* dt;          // step of the running trial
+  double* dt_out;      // last accepted step
+  unsigned long long* smax;  // per case: bit pattern of the largest wave-speed bound (non-negative)
+  Control* ctrl;
+};
+
+struct Prim {
+  double r, u, v, p;
+};
+struct Flux {
+  double m, x, y, e;  // mass, x momentum, y momentum, energy
+};
+
+// ---------------------------------------------------------------------------------------------
+// Pointwise formulas (operation order of src/euler2d_cuda.cu:71-146, 169-205, 248-281).
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ double pressure(double r, double mu, double mv, double E, double gamma) {
+  return (gamma - 1.0) * (E - 0.5 * (mu * mu + mv * mv) / r);
+}
+
+__device__ __forceinline__ Prim primitive(double r, double mu, double mv, double E, double gamma) {
+  return {r, mu / r, mv / r, pressure(r, mu, mv, E, gamma)};
+}
+
+// Exact flux along the normal velocity un (tangential ut): mass, normal, tangential, energy.
+__device__ __forceinline__ void exact_flux(double r, double un, double ut, double p, double gm, double f[4]) {
+  const double E = p / (gm - 1.0) + 0.5 * r * (un * un + ut * ut);
+  f[0] = r * un;
+  f[1] = r * un * un + p;
+  f[2] = r * un * ut;
+  f[3] = un * (E + p);
+}
+
+__device__ void hllc(double rL, double uL, double vL, double pL, double rR, double uR, double vR, double pR, double gm,
+                     double f[4]) {
+  const double aL = sqrt(gm * pL / rL), aR = sqrt(gm * pR / rR);
+  const double EL = pL / (gm - 1.0) + 0.5 * rL * (uL * uL + vL * vL);
+  const double ER = pR / (gm - 1.0) + 0.5 * rR * (uR * uR + vR * vR);
+  const double qL = sqrt(rL), qR = sqrt(rR);
+  const double HL = (EL + pL) / rL, HR = (ER + pR) / rR;
+  const double uT = (qL * uL + qR * uR) / (qL + qR);
+  const double vT = (qL * vL + qR * vR) / (qL + qR);
+  const double HT = (qL * HL + qR * HR) / (qL + qR);
+  const double aT = sqrt(fmax((gm - 1.0) * (HT - 0.5 * (uT * uT + vT * vT)), 1e-300));
+  const double SL = fmin(uL - aL, uT - aT), SR = fmax(uR + aR, uT + aT);
+  double fL[4], fR[4];
+  exact_flux(rL, uL, vL, pL, gm, fL);
+  exact_flux(rR, uR, vR, pR, gm, fR);
+  if (SL >= 0.0) {
+    for (int k = 0; k < 4; ++k) f[k] = fL[k];
+    return;
+  }
+  if (SR <= 0.0) {
+    for (int k = 0; k < 4; ++k) f[k] = fR[k];
+    return;
+  }
+  const double Ss = (pR - pL + rL * uL * (SL - uL) - rR * uR * (SR - uR)) / (rL * (SL - uL) - rR * (SR - uR));
+  // Star state of the upwind side of the contact, then F = F_side + S_side (U* - U_side).
+  const bool left = Ss >= 0.0;
+  const double r = left ? rL : rR, u = left ? uL : uR, v = left ? vL : vR, p = left ? pL : pR;
+  const double E = left ? EL : ER, S = left ? SL : SR;
+  const double* fs = left ? fL : fR;
+  const double a = r * (S - u) / (S - Ss);
+  const double star[4] = {a, a * Ss, a * v, a * (E / r + (Ss - u) * (Ss + p / (r * (S - u))))};
+  const double side[4] = {r, r * u, r * v, E};
+  for (int k = 0; k < 4; ++k) f[k] = fs[k] + S * (star[k] - side[k]);
+}
+
+// AUSM+-up split functions (Liou 2006) with the reference's constants (beta 1/8, Kp 1/4,
+// Ku 3/4, sigma 1).  s is the side sign +1 / -1.
+__device__ __forceinline__ double m1(double m, int s) { return 0.5 * (m + s * fabs(m)); }
+__device__ __forceinline__ double m2(double m, int s) { return s * 0.25 * (m + s) * (m + s); }
+__device__ __forceinline__ double m4(double m, int s) {
+  if (fabs(m) >= 1.0) return m1(m, s);
+  return m2(m, s) * (1.0 - s * 2.0 * m2(m, -s));
+}
+__device__ __forceinline__ double p5(double m, int s, double alpha) {
+  if (fabs(m) >= 1.0) return 0.5 * (1.0 + s * (m > 0.0 ? 1.0 : (m < 0.0 ? -1.0 : 0.0)));
+  return m2(m, s) * ((s * 2.0 - m) - s * 16.0 * alpha * m * m2(m, -s));
+}
+
+__device__ void ausm(double rL, double uL, double vL, double pL, double rR, double uR, double vR, double pR, double gm,
+                     double mco, double f[4]) {
+  const double a12 = 0.5 * (sqrt(gm * pL / rL) + sqrt(gm * pR / rR));
+  const double rho12 = 0.5 * (rL + rR), ML = uL / a12, MR = uR / a12;
+  const double mb2 = 0.5 * (uL * uL + uR * uR) / (a12 * a12);
+  const double m02 = fmin(1.0, fmax(mb2, mco * mco));
+  const double fa = sqrt(m02) * (2.0 - sqrt(m02));
+  const double alpha = 3.0 / 16.0 * (-4.0 + 5.0 * fa * fa);
+  const double m12 =
+      m4(ML, 1) + m4(MR, -1) - (0.25 / fa) * fmax(1.0 - mb2, 0.0) * (pR - pL) / (rho12 * a12 * a12);
+  const double pp = p5(ML, 1, alpha), pm = p5(MR, -1, alpha);
+  const double p12 = pp * pL + pm * pR - 0.75 * pp * pm * (rL + rR) * fa * a12 * (uR - uL);
+  const bool left = m12 > 0.0;
+  const double r = left ? rL : rR, u = left ? uL : uR;
+  const double v = left ? vL : vR, p = left ? pL : pR;
+  const double mdot = a12 * m12 * r;
+  const double E = p / (gm - 1.0) + 0.5 * r * (u * u + v * v);
+  f[0] = mdot;
+  f[1] = mdot * u + p12;
+  f[2] = mdot * v;
+  f[3] = mdot * (E + p) / r;
+}
+
+// Flux through an x face (normal u) or a y face (normal v), returned in x / y momentum order.
+template <bool XFACE>
+__device__ __forceinline__ Flux face_flux(const Grid& g, double mco, const Prim& L, const Prim& R) {
+  double f[4];
+  const double nL = XFACE ? L.u : L.v, tL = XFACE ? L.v : L.u;
+  const double nR = XFACE ? R.u : R.v, tR = XFACE ? R.v : R.u;
+  if (g.ausm)
+    ausm(L.r, nL, tL, L.p, R.r, nR, tR, R.p, g.gamma, mco, f);
+  else
+    hllc(L.r, nL, tL, L.p, R.r, nR, tR, R.p, g.gamma, f);
+  return XFACE ? Flux{f[0], f[1], f[2], f[3]} : Flux{f[0], f[2], f[1], f[3]};
+}
+
+__device__ __forceinline__ double wave_bound(double r, double mu, double mv, double E, const Grid& g, double dx,
+                                             double dy) {
+  const double u = mu / r, v = mv / r;
+  const double p = pressure(r, mu, mv, E, g.gamma);
+  const double a = sqrt(g.gamma * p / r);
+  return (fabs(u) + a) / dx + (fabs(v) + a) / dy;
+}
+
+__device__ __forceinline__ bool inadmissible(double r, double mu, double mv, double E, double gamma) {
+  const double p = pressure(r, mu, mv, E, gamma);
+  return !(r > 0.0) || !(p > 0.0) || !isfinite(r) || !isfinite(p);
+}
+
+// Ghost index map of one axis (src/euler2d_cuda.cu:153-180): the interior cell a ghost copies
+// and whether its normal momentum is negated.  Indices beyond the first ghost (unused lanes)
+// are clamped.
+__device__ __forceinline__ int map_axis(int i, int n, int lo, int hi, bool* flip) {
+  if (i < 0) {
+    if (lo == BC_PERIODIC) return i + n;
+    if (lo == BC_REFLECT) {
+      *flip = true;
+      return -i - 1;
+    }
+    return 0;
+  }
+  if (i >= n) {
+    if (i > n) return n - 1;
+    if (hi == BC_PERIODIC) return i - n;
+    if (hi == BC_REFLECT) {
+      *flip = true;
+      return 2 * n - 1 - i;
+    }
+    return n - 1;
+  }
+  return i;
+}
+
+__device__ __forceinline__ double shfl_up1(double x) { return __shfl_up(x, 1, kLanes); }
+__device__ __forceinline__ double shfl_down1(double x) { return __shfl_down(x, 1, kLanes); }
+
+__device__ __forceinline__ unsigned long long wave_max(unsigned long long x) {
+  for (int d = kLanes / 2; d > 0; d >>= 1) {
+    const unsigned long long y = __shfl_xor(x, d, kLanes);
+    x = y > x ? y : x;
+  }
+  return x;
+}
+
+__device__ __forceinline__ double next_dt(const Grid& g, double smax_value, double t, double tend) {
+  double d = g.dt_fixed > 0.0 ? g.dt_fixed : g.cfl / smax_value;
+  const double remain = tend - t;
+  if (remain <= 0.0)
+    d = 0.0;
+  else if (d > remain)
+    d = remain;
+  return d;
+}
+
+__device__ __forceinline__ unsigned long long load_relaxed(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Logical block of a launch: hardware dispatches consecutive blocks round-robin over the 8 XCDs;
+// give each XCD a contiguous range of strips so that halo rows and columns shared by neighbours
+// stay in one L2.  The launch is a multiple of 8 blocks.
+__device__ __forceinline__ int logical_block() {
+  const int per_xcd = gridDim.x / kXcds;
+  return (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Commit of a trial (the last block of the last stage): src/euler2d_cuda.cu:670-693.
+// ---------------------------------------------------------------------------------------------
+__device__ void commit(const Grid& g) {
+  Control* c = g.ctrl;
+  __shared__ int s_flag;
+  __shared__ int s_maxed;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_flag = 0;
+  __syncthreads();
+  if (__hip_atomic_load(&c->bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+    // Rejected: halve every positive step; give up after 12 retries or on an unusable step.
+    for (int k = tid; k < g.ncase; k += blockDim.x) {
+      double d = g.dt[k];
+      if (d > 0.0) d *= 0.5;
+      g.dt[k] = d;
+      if (d < 0.0 || !isfinite(d)) s_flag = 1;
+      g.smax[k] = 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const int retries = c->retries + 1;
+      c->retries = retries;
+      c->nreject += 1;
+      if (retries > kMaxRetries || s_flag) c->status = 2;
+      c->bad = 0;
+      c->arrivals = 0;
+    }
+    return;
+  }
+  // Accepted: advance the clocks of the cases that moved, then either stop (maxsteps reached,
+  // or every case at tend) or choose the next step from the candidate's wave-speed bound.
+  const double tend = c->tend;
+  if (tid == 0) {
+    const long long steps = c->steps + 1;
+    c->steps = steps;
+    s_maxed = steps >= c->maxsteps;
+  }
+  for (int k = tid; k < g.ncase; k += blockDim.x) {
+    const double d = g.dt[k];
+    g.dt_out[k] = d;
+    double tk = g.t[k];
+    if (d > 0.0) {
+      tk += d;
+      g.t[k] = tk;
+    }
+    if (tk < tend) s_flag = 1;
+  }
+  __syncthreads();
+  const bool next = !s_maxed && s_flag;
+  for (int k = tid; k < g.ncase; k += blockDim.x) {
+    if (next) g.dt[k] = next_dt(g, __longlong_as_double(static_cast<long long>(load_relaxed(&g.smax[k]))), g.t[k], tend);
+    g.smax[k] = 0;
+  }
+  if (tid == 0) {
+    c->cur ^= 1;
+    c->nsteps += 1;
+    c->retries = 0;
+    c->ghost_from = next ? 1 : 2;
+    if (!next) c->status = s_maxed ? 1 : 0;
+    c->arrivals = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// One RK stage (MODE 1..3) or the residual alone (MODE 0), fused over a wave strip.
+//   MODE 1: U[cur^1] = U + dt L(U)                    (in U[cur])
+//   MODE 2: S2 = 3/4 U + 1/4 (U1 + dt L(U1))          (U1 in U[cur^1])
+//   MODE 3: U[cur^1] = 1/3 U + 2/3 (S2 + dt L(S2)), then the commit
+//   MODE 0: R = L(in)
+// ---------------------------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_stage(Grid g, const double* __restrict__ rhs_in,
+                                                     double* __restrict__ rhs_out) {
+  Control* ctrl = g.ctrl;
+  bool skip = false;
+  int cur = 0;
+  if (MODE > 0) {
+    if (__hip_atomic_load(&ctrl->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != RUNNING) return;
+    skip = __hip_atomic_load(&ctrl->bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    cur = ctrl->cur;
+  }
+  const double* base = MODE > 0 ? g.U[cur] : nullptr;
+  const double* in = MODE == 0 ? rhs_in : MODE == 1 ? g.U[cur] : MODE == 2 ? g.U[cur ^ 1] : g.S2;
+  double* out = MODE == 0 ? rhs_out : MODE == 2 ? g.S2 : g.U[cur ^ 1];
+  constexpr double A = MODE == 2 ? 0.75 : MODE == 3 ? 1.0 / 3.0 : 0.0;
+  constexpr double B = MODE == 2 ? 0.25 : MODE == 3 ? 2.0 / 3.0 : 1.0;
+
+  const int lane = threadIdx.x % kLanes;
+  const int strip = logical_block() * kWaves + threadIdx.x / kLanes;
+  bool wave_bad = false;
+  unsigned long long bound = 0;
+  const bool work = !skip && strip < g.strips;
+  int icase = 0;
+  if (work) {
+    icase = strip / (g.strips_x * g.strips_y);
+    const int rem = strip - icase * g.strips_x * g.strips_y;
+    const int sy = rem / g.strips_x, sx = rem - sy * g.strips_x;
+    const int i = sx * kStripCells - 1 + lane;
+    const int j0 = sy * g.rows;
+    const int jl = min(j0 + g.rows, g.ny) - 1;
+    const bool owner = lane >= 1 && lane <= kStripCells && i < g.nx;
+    bool flip_u = false;
+    const int ic = map_axis(i, g.nx, g.bc[0], g.bc[1], &flip_u);
+    const size_t plane = g.plane;
+    const size_t case_base = static_cast<size_t>(icase) * 4 * plane + (ic + kGhosts);
+    const double mco = g.mco[icase];
+    const double dxi = g.dx[min(max(i, -kGhosts), g.nx + kGhosts - 1) + kGhosts];
+    const double dt = MODE > 0 ? g.dt[icase] : 0.0;
+
+    Prim wprev{};
+    Flux gprev{}, xres{};
+    double q0 = 0, q1 = 0, q2 = 0, q3 = 0;  // conserved state of the owned cell of the previous row
+    for (int j = j0 - 1; j <= jl + 1; ++j) {
+      bool flip_v = false;
+      const int jc = map_axis(j, g.ny, g.bc[2], g.bc[3], &flip_v);
+      const size_t k = case_base + static_cast<size_t>(jc + kGhosts) * g.mx;
+      const double c0 = in[k], c1 = in[k + plane], c2 = in[k + 2 * plane], c3 = in[k + 3 * plane];
+      Prim w = primitive(c0, c1, c2, c3, g.gamma);
+      if (flip_u) w.u = -w.u;
+      if (flip_v) w.v = -w.v;
+      if (j >= j0) {
+        const Flux gf = face_flux<false>(g, mco, wprev, w);
+        if (j > j0 && owner) {
+          // Residual and update of the owned cell of row j - 1.
+          const double dyj = g.dy[j - 1 + kGhosts];
+          const double r0 = xres.m - (gf.m - gprev.m) / dyj;
+          const double r1 = xres.x - (gf.x - gprev.x) / dyj;
+          const double r2 = xres.y - (gf.y - gprev.y) / dyj;
+          const double r3 = xres.e - (gf.e - gprev.e) / dyj;
+          const size_t ko = case_base + static_cast<size_t>(j - 1 + kGhosts) * g.mx;
+          if (MODE == 0) {
+            out[ko] = r0;
+            out[ko + plane] = r1;
+            out[ko + 2 * plane] = r2;
+            out[ko + 3 * plane] = r3;
+          } else {
+            double a0 = q0, a1 = q1, a2 = q2, a3 = q3;
+            if (MODE > 1) {
+              a0 = __builtin_nontemporal_load(base + ko);
+              a1 = __builtin_nontemporal_load(base + ko + plane);
+              a2 = __builtin_nontemporal_load(base + ko + 2 * plane);
+              a3 = __builtin_nontemporal_load(base + ko + 3 * plane);
+            }
+            const double n0 = A * a0 + B * (q0 + dt * r0);
+            const double n1 = A * a1 + B * (q1 + dt * r1);
+            const double n2 = A * a2 + B * (q2 + dt * r2);
+            const double n3 = A * a3 + B * (q3 + dt * r3);
+            out[ko] = n0;
+            out[ko + plane] = n1;
+            out[ko + 2 * plane] = n2;
+            out[ko + 3 * plane] = n3;
+            if (inadmissible(n0, n1, n2, n3, g.gamma)) wave_bad = true;
+            if (MODE == 3) {
+              const double b = wave_bound(n0, n1, n2, n3, g, dxi, dyj);
+              const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(b));
+              bound = bits > bound ? bits : bound;
+            }
+          }
+        }
+        gprev = gf;
+      }
+      if (j >= j0 && j <= jl) {
+        // x faces of row j: lane k evaluates the face west of its cell.
+        Prim wl;
+        wl.r = shfl_up1(w.r);
+        wl.u = shfl_up1(w.u);
+        wl.v = shfl_up1(w.v);
+        wl.p = shfl_up1(w.p);
+        if (lane == 0) wl = w;
+        const Flux fw = face_flux<true>(g, mco, wl, w);
+        const Flux fe{shfl_down1(fw.m), shfl_down1(fw.x), shfl_down1(fw.y), shfl_down1(fw.e)};
+        xres = Flux{-(fe.m - fw.m) / dxi, -(fe.x - fw.x) / dxi, -(fe.y - fw.y) / dxi, -(fe.e - fw.e) / dxi};
+      }
+      wprev = w;
+      q0 = c0, q1 = c1, q2 = c2, q3 = c3;
+    }
+  }
+  if (MODE == 0) return;
+  if (__any(wave_bad) && lane == 0) atomicOr(&ctrl->bad, 1);
+  if (MODE == 3) {
+    bound = wave_max(bound);
+    if (work && lane == 0 && bound) atomicMax(&g.smax[icase], bound);
+    __shared__ int s_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence();
+      const unsigned prev = atomicAdd(&ctrl->arrivals, 1u);
+      s_last = prev == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    commit(g);
+  }
+}
+
+// Wave-speed bound of every case at U[cur] (the first step of a run, src/euler2d_cuda.cu:223-237).
+__global__ __launch_bounds__(kThreads) void k_bound(Grid g) {
+  const int cur = g.ctrl->cur;
+  const double* u = g.U[cur];
+  const size_t inner = static_cast<size_t>(g.nx) * g.ny;
+  const size_t n = inner * g.ncase;
+  for (size_t q = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; q < n;
+       q += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const int icase = static_cast<int>(q / inner);
+    const int ij = static_cast<int>(q - icase * inner);
+    const int i = ij % g.nx, j = ij / g.nx;
+    const size_t k = static_cast<size_t>(icase) * 4 * g.plane + static_cast<size_t>(j + kGhosts) * g.mx + i + kGhosts;
+    const double b = wave_bound(u[k], u[k + g.plane], u[k + 2 * g.plane], u[k + 3 * g.plane], g, g.dx[i + kGhosts],
+                                g.dy[j + kGhosts]);
+    atomicMax(&g.smax[icase], static_cast<unsigned long long>(__double_as_longlong(b)));
+  }
+}
+
+// Start of a run call (src/euler2d_cuda.cu:605-619 for the first step): done / maxsteps tests
+// and the first step's dt of every case.  One block.
+__global__ __launch_bounds__(kThreads) void k_open(Grid g, double tend, long long maxsteps) {
+  Control* c = g.ctrl;
+  __shared__ int s_open;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_open = 0;
+  __syncthreads();
+  for (int k = tid; k < g.ncase; k += blockDim.x)
+    if (g.t[k] < tend) s_open = 1;
+  __syncthreads();
+  const bool run = maxsteps > 0 && s_open;
+  for (int k = tid; k < g.ncase; k += blockDim.x) {
+    if (run) g.dt[k] = next_dt(g, __longlong_as_double(static_cast<long long>(g.smax[k])), g.t[k], tend);
+    g.smax[k] = 0;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    c->tend = tend;
+    c->maxsteps = maxsteps;
+    c->steps = 0;
+    c->retries = 0;
+    c->bad = 0;
+    c->arrivals = 0;
+    if (maxsteps <= 0)
+      c->status = 1;
+    else if (!s_open)
+      c->status = 0;
+    else {
+      c->status = RUNNING;
+      c->ghost_from = 1;
+    }
+  }
+}
+
+__global__ void k_zero(double* p, size_t n) {
+  for (size_t q = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; q < n;
+       q += static_cast<size_t>(gridDim.x) * blockDim.x)
+    p[q] = 0.0;
+}
+
+}  // namespace ugrid
+
+// =============================================================================================
+// Host side
+// =============================================================================================
+using namespace ugrid;
+
+struct ugrid_session {
+  int device = 0;
+  Grid g{};
+  ugrid_params par{};
+  size_t states = 0;      // doubles per state array (ghosts included)
+  int launch_blocks = 0;  // padded to a multiple of 8
+  double* R = nullptr;    // residual output of ugrid_rhs (allocated on first use)
+  Control* host_ctrl = nullptr;  // pinned mirror of the control record
+  hipStream_t stream = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_trials = 0;
+  std::vector<double> uploaded;  // the state as uploaded (its ghost cells are the download's
+                                 // ghosts until a step begins)
+  std::vector<void*> owned;
+};
+
+namespace {
+
+int fail(char* err, size_t len, const std::string& message) {
+  if (err && len) {
+    const size_t k = message.size() < len - 1 ? message.size() : len - 1;
+    std::memcpy(err, message.data(), k);
+    err[k] = 0;
+  }
+  return 3;
+}
+
+bool ok(hipError_t e, char* err, size_t len, const char* what) {
+  if (e == hipSuccess) return true;
+  fail(err, len, std::string(what) + ": " + hipGetErrorString(e));
+  return false;
+}
+
+template <class T>
+bool device_alloc(ugrid_session* s, T** p, size_t count, char* err, size_t len) {
+  void* q = nullptr;
+  if (!ok(hipMalloc(&q, (count ? count : 1) * sizeof(T)), err, len, "hipMalloc")) return false;
+  s->owned.push_back(q);
+  *p = static_cast<T*>(q);
+  return true;
+}
+
+void release(ugrid_session* s) {
+  if (!s) return;
+  if (s->exec) (void)hipGraphExecDestroy(s->exec);
+  for (void* p : s->owned) (void)hipFree(p);
+  if (s->host_ctrl) (void)hipHostFree(s->host_ctrl);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+// Fill the two ghost layers of every plane from the interior as the reference's boundary kernel
+// does (src/euler2d_cuda.cu:153-180): both indices mapped, normal momentum negated on reflection.
+void fill_ghosts(const Grid& g, double* u) {
+  const int ng = kGhosts;
+  for (int c = 0; c < g.ncase; ++c)
+    for (int v = 0; v < 4; ++v) {
+      double* q = u + (static_cast<size_t>(c) * 4 + v) * g.plane;
+      for (int j = -ng; j < g.ny + ng; ++j)
+        for (int i = -ng; i < g.nx + ng; ++i) {
+          if (i >= 0 && i < g.nx && j >= 0 && j < g.ny) continue;
+          int ii = i, jj = j;
+          double sign = 1.0;
+          if (ii < 0) {
+            if (g.bc[0] == BC_PERIODIC) ii += g.nx;
+            else if (g.bc[0] == BC_REFLECT) { ii = -ii - 1; if (v == 1) sign = -sign; }
+            else ii = 0;
+          } else if (ii >= g.nx) {
+            if (g.bc[1] == BC_PERIODIC) ii -= g.nx;
+            else if (g.bc[1] == BC_REFLECT) { ii = 2 * g.nx - 1 - ii; if (v == 1) sign = -sign; }
+            else ii = g.nx - 1;
+          }
+          if (jj < 0) {
+            if (g.bc[2] == BC_PERIODIC) jj += g.ny;
+            else if (g.bc[2] == BC_REFLECT) { jj = -jj - 1; if (v == 2) sign = -sign; }
+            else jj = 0;
+          } else if (jj >= g.ny) {
+            if (g.bc[3] == BC_PERIODIC) jj -= g.ny;
+            else if (g.bc[3] == BC_REFLECT) { jj = 2 * g.ny - 1 - jj; if (v == 2) sign = -sign; }
+            else jj = g.ny - 1;
+          }
+          q[static_cast<size_t>(j + ng) * g.mx + i + ng] = sign * q[static_cast<size_t>(jj + ng) * g.mx + ii + ng];
+        }
+    }
+}
+
+int launch_trial(ugrid_session* s, hipStream_t stream) {
+  hipLaunchKernelGGL(k_stage<1>, dim3(s->launch_blocks), dim3(kThreads), 0, stream, s->g, nullptr, nullptr);
+  hipLaunchKernelGGL(k_stage<2>, dim3(s->launch_blocks), dim3(kThreads), 0, stream, s->g, nullptr, nullptr);
+  hipLaunchKernelGGL(k_stage<3>, dim3(s->launch_blocks), dim3(kThreads), 0, stream, s->g, nullptr, nullptr);
+  return 3;
+}
+
+bool build_graph(ugrid_session* s, int trials, char* err, size_t len) {
+  if (s->exec) {
+    (void)hipGraphExecDestroy(s->exec);
+    s->exec = nullptr;
+  }
+  hipGraph_t graph = nullptr;
+  if (!ok(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), err, len, "begin capture")) return false;
+  for (int k = 0; k < trials; ++k) launch_trial(s, s->stream);
+  const hipError_t launched = hipGetLastError();
+  const hipError_t ended = hipStreamEndCapture(s->stream, &graph);
+  if (!ok(launched, err, len, "trial launch") || !ok(ended, err, len, "end capture")) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return false;
+  }
+  const hipError_t inst = hipGraphInstantiate(&s->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (!ok(inst, err, len, "graph instantiate")) return false;
+  s->graph_trials = trials;
+  return true;
+}
+
+// Rows per strip: 32, halved (down to 4) while the launch would hold fewer than 16 waves per
+// CU (256 CUs), and never more than the grid has.
+int choose_rows(int nx, int ny, int ncase) {
+  const long long sx = (nx + kStripCells - 1) / kStripCells;
+  int rows = 32;
+  while (rows > 4 && sx * ((ny + rows - 1) / rows) * ncase < 16LL * 256) rows /= 2;
+  return rows < ny ? rows : ny;
+}
+
+}  // namespace
+
+extern "C" int ugrid_create(const ugrid_setup* setup, const double* u, uint64_t u_len, const double* t,
+                            const double* dt, ugrid_session** out, char* err, size_t err_len) {
+  if (!out) return fail(err, err_len, "ugrid_create: null output handle");
+  *out = nullptr;
+  if (!setup || !u || !setup->dx || !setup->dy) return fail(err, err_len, "ugrid_create: null argument");
+  const ugrid_params& p = setup->params;
+  if (setup->nx < 1 || setup->ny < 1 || setup->ncase < 1)
+    return fail(err, err_len, "ugrid_create: nx, ny and ncase must be positive");
+  if (p.time_integrator != 0)
+    return fail(err, err_len, "ugrid_create: only the SSP-RK3 integrator (time_integrator 0) is implemented");
+  if (p.limiter != 3)
+    return fail(err, err_len,
+                "ugrid_create: the device backend requires the first-order limiter (3); refusing to change the method "
+                "implicitly");
+  for (int k = 0; k < 4; ++k)
+    if (p.bc[k] < 0 || p.bc[k] > 2) return fail(err, err_len, "ugrid_create: boundary kinds are 0, 1 or 2");
+  if ((p.bc[0] == BC_PERIODIC) != (p.bc[1] == BC_PERIODIC) || (p.bc[2] == BC_PERIODIC) != (p.bc[3] == BC_PERIODIC))
+    return fail(err, err_len, "ugrid_create: periodic boundary must be set on both sides of an axis");
+  const long long mx = setup->nx + 2LL * kGhosts, my = setup->ny + 2LL * kGhosts;
+  if (mx * my > (1LL << 40) / setup->ncase) return fail(err, err_len, "ugrid_create: grid too large");
+  const size_t states = static_cast<size_t>(setup->ncase) * 4 * static_cast<size_t>(mx * my);
+  if (u_len != states) return fail(err, err_len, "ugrid_create: u length must be ncase * 4 * (nx + 4) * (ny + 4)");
+
+  ugrid_session* s = new ugrid_session();
+  s->par = p;
+  s->states = states;
+  Grid& g = s->g;
+  g.nx = setup->nx;
+  g.ny = setup->ny;
+  g.mx = static_cast<int>(mx);
+  g.my = static_cast<int>(my);
+  g.ncase = setup->ncase;
+  g.plane = static_cast<size_t>(mx * my);
+  for (int k = 0; k < 4; ++k) g.bc[k] = p.bc[k];
+  g.ausm = p.flux == 1;
+  g.gamma = p.gamma;
+  g.cfl = p.cfl;
+  g.dt_fixed = p.dt_fixed;
+  g.rows = choose_rows(g.nx, g.ny, g.ncase);
+  g.strips_x = (g.nx + kStripCells - 1) / kStripCells;
+  g.strips_y = (g.ny + g.rows - 1) / g.rows;
+  const long long strips = static_cast<long long>(g.strips_x) * g.strips_y * g.ncase;
+  if (strips > (1LL << 30)) {
+    release(s);
+    return fail(err, err_len, "ugrid_create: grid too large");
+  }
+  g.strips = static_cast<int>(strips);
+  g.blocks = (g.strips + kWaves - 1) / kWaves;
+  s->launch_blocks = (g.blocks + kXcds - 1) / kXcds * kXcds;
+
+  std::vector<double> mco(g.ncase, p.ausm_mco);
+  if (setup->mco)
+    for (int c = 0; c < g.ncase; ++c) mco[c] = setup->mco[c];
+  std::vector<double> t0(g.ncase, 0.0), dt0(g.ncase, 0.0);
+  if (t)
+    for (int c = 0; c < g.ncase; ++c) t0[c] = t[c];
+  if (dt)
+    for (int c = 0; c < g.ncase; ++c) dt0[c] = dt[c];
+
+  double *dxd = nullptr, *dyd = nullptr, *mcod = nullptr;
+  if (!ok(hipGetDevice(&s->device), err, err_len, "hipGetDevice") ||
+      !ok(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), err, err_len, "hipStreamCreate") ||
+      !ok(hipHostMalloc(reinterpret_cast<void**>(&s->host_ctrl), sizeof(Control), hipHostMallocDefault), err, err_len,
+          "hipHostMalloc") ||
+      !device_alloc(s, &g.U[0], states, err, err_len) || !device_alloc(s, &g.U[1], states, err, err_len) ||
+      !device_alloc(s, &g.S2, states, err, err_len) || !device_alloc(s, &dxd, g.mx, err, err_len) ||
+      !device_alloc(s, &dyd, g.my, err, err_len) || !device_alloc(s, &mcod, g.ncase, err, err_len) ||
+      !device_alloc(s, &g.t, g.ncase, err, err_len) || !device_alloc(s, &g.dt, g.ncase, err, err_len) ||
+      !device_alloc(s, &g.dt_out, g.ncase, err, err_len) || !device_alloc(s, &g.smax, g.ncase, err, err_len) ||
+      !device_alloc(s, &g.ctrl, 1, err, err_len)) {
+    release(s);
+    return 3;
+  }
+  g.dx = dxd;
+  g.dy = dyd;
+  g.mco = mcod;
+  Control c0{};
+  c0.status = 0;
+  const hipMemcpyKind h2d = hipMemcpyHostToDevice;
+  const size_t bytes = states * sizeof(double);
+  if (!ok(hipMemcpyAsync(g.U[0], u, bytes, h2d, s->stream), err, err_len, "upload state") ||
+      !ok(hipMemcpyAsync(g.U[1], u, bytes, h2d, s->stream), err, err_len, "upload state") ||
+      !ok(hipMemcpyAsync(dxd, setup->dx, g.mx * sizeof(double), h2d, s->stream), err, err_len, "upload dx") ||
+      !ok(hipMemcpyAsync(dyd, setup->dy, g.my * sizeof(double), h2d, s->stream), err, err_len, "upload dy") ||
+      !ok(hipMemcpyAsync(mcod, mco.data(), g.ncase * sizeof(double), h2d, s->stream), err, err_len, "upload mco") ||
+      !ok(hipMemcpyAsync(g.t, t0.data(), g.ncase * sizeof(double), h2d, s->stream), err, err_len, "upload t") ||
+      !ok(hipMemcpyAsync(g.dt, dt0.data(), g.ncase * sizeof(double), h2d, s->stream), err, err_len, "upload dt") ||
+      !ok(hipMemcpyAsync(g.dt_out, dt0.data(), g.ncase * sizeof(double), h2d, s->stream), err, err_len,
+          "upload dt") ||
+      !ok(hipMemsetAsync(g.smax, 0, g.ncase * sizeof(unsigned long long), s->stream), err, err_len, "clear bound") ||
+      !ok(hipMemcpyAsync(g.ctrl, &c0, sizeof(Control), h2d, s->stream), err, err_len, "upload control") ||
+      !ok(hipStreamSynchronize(s->stream), err, err_len, "upload")) {
+    release(s);
+    return 3;
+  }
+  s->uploaded.assign(u, u + states);
+  const int trials = g.nx * static_cast<long long>(g.ny) * g.ncase >= (1 << 22) ? 4 : 16;
+  if (!build_graph(s, trials, err, err_len)) {
+    release(s);
+    return 3;
+  }
+  *out = s;
+  return 0;
+}
+
+extern "C" int ugrid_run(ugrid_session* s, double tend, int64_t maxsteps, ugrid_meta* meta, char* err,
+                         size_t err_len) {
+  if (!s) return fail(err, err_len, "ugrid_run: null session");
+  const Grid& g = s->g;
+  const int bound_blocks = static_cast<int>(
+      std::min<long long>((static_cast<long long>(g.nx) * g.ny * g.ncase + kThreads - 1) / kThreads, 4096));
+  hipLaunchKernelGGL(k_bound, dim3(bound_blocks), dim3(kThreads), 0, s->stream, g);
+  hipLaunchKernelGGL(k_open, dim3(1), dim3(kThreads), 0, s->stream, g, tend, static_cast<long long>(maxsteps));
+  if (!ok(hipGetLastError(), err, err_len, "run start")) return 3;
+  // Every trial of a running batch ends in an accept or a rejection, so each replay of the
+  // trial graph must move nsteps + nreject; a replay that does not is reported, not waited on.
+  long long progress = -1;
+  for (;;) {
+    if (!ok(hipMemcpyAsync(s->host_ctrl, g.ctrl, sizeof(Control), hipMemcpyDeviceToHost, s->stream), err, err_len,
+            "control copy") ||
+        !ok(hipStreamSynchronize(s->stream), err, err_len, "trial"))
+      return 3;
+    if (s->host_ctrl->status != RUNNING) break;
+    const long long now = s->host_ctrl->nsteps + s->host_ctrl->nreject;
+    if (now == progress) return fail(err, err_len, "ugrid_run: device step control made no progress");
+    progress = now;
+    if (!ok(hipGraphLaunch(s->exec, s->stream), err, err_len, "trial graph")) return 3;
+  }
+  const Control& c = *s->host_ctrl;
+  if (meta) {
+    meta->nsteps = c.nsteps;
+    meta->nreject = c.nreject;
+  }
+  if (c.status == 2) {
+    char line[160];
+    std::snprintf(line, sizeof(line), "non-physical RK candidate after %d retries at accepted step %lld", c.retries,
+                  c.nsteps);
+    fail(err, err_len, line);
+  }
+  return c.status;
+}
+
+extern "C" int ugrid_download(ugrid_session* s, double* u, uint64_t u_len, double* t, double* dt, char* err,
+                              size_t err_len) {
+  if (!s) return fail(err, err_len, "ugrid_download: null session");
+  if (u && u_len != s->states) return fail(err, err_len, "ugrid_download: u length mismatch");
+  const Grid& g = s->g;
+  Control c{};
+  if (!ok(hipMemcpyAsync(&c, g.ctrl, sizeof(Control), hipMemcpyDeviceToHost, s->stream), err, err_len, "control") ||
+      !ok(hipStreamSynchronize(s->stream), err, err_len, "control"))
+    return 3;
+  const size_t bytes = s->states * sizeof(double);
+  if (u) {
+    if (!ok(hipMemcpyAsync(u, g.U[c.cur], bytes, hipMemcpyDeviceToHost, s->stream), err, err_len, "download") ||
+        !ok(hipStreamSynchronize(s->stream), err, err_len, "download"))
+      return 3;
+    std::vector<double> other;
+    const double* ghosts = s->uploaded.data();
+    if (c.ghost_from == 1) {
+      fill_ghosts(g, u);
+      ghosts = nullptr;
+    } else if (c.ghost_from == 2) {
+      other.resize(s->states);
+      if (!ok(hipMemcpyAsync(other.data(), g.U[c.cur ^ 1], bytes, hipMemcpyDeviceToHost, s->stream), err, err_len,
+              "download") ||
+          !ok(hipStreamSynchronize(s->stream), err, err_len, "download"))
+        return 3;
+      fill_ghosts(g, other.data());
+      ghosts = other.data();
+    }
+    if (ghosts)
+      for (size_t cv = 0; cv < static_cast<size_t>(g.ncase) * 4; ++cv)
+        for (int j = 0; j < g.my; ++j)
+          for (int i = 0; i < g.mx; ++i) {
+            if (i >= kGhosts && i < g.nx + kGhosts && j >= kGhosts && j < g.ny + kGhosts) continue;
+            const size_t k = cv * g.plane + static_cast<size_t>(j) * g.mx + i;
+            u[k] = ghosts[k];
+          }
+  }
+  if (t && !ok(hipMemcpyAsync(t, g.t, g.ncase * sizeof(double), hipMemcpyDeviceToHost, s->stream), err, err_len,
+                "download t"))
+    return 3;
+  if (dt && !ok(hipMemcpyAsync(dt, g.dt_out, g.ncase * sizeof(double), hipMemcpyDeviceToHost, s->stream), err,
+                 err_len, "download dt"))
+    return 3;
+  return ok(hipStreamSynchronize(s->stream), err, err_len, "download") ? 0 : 3;
+}
+
+extern "C" int ugrid_rhs(ugrid_session* s, const double* u, uint64_t u_len, double* r, char* err, size_t err_len) {
+  if (!s || !u || !r) return fail(err, err_len, "ugrid_rhs: null argument");
+  if (u_len != s->states) return fail(err, err_len, "ugrid_rhs: u length mismatch");
+  const size_t bytes = s->states * sizeof(double);
+  if (!s->R && !device_alloc(s, &s->R, s->states, err, err_len)) return 3;
+  // The second-stage buffer carries the input: it holds nothing between trials.
+  hipLaunchKernelGGL(k_zero, dim3(1024), dim3(kThreads), 0, s->stream, s->R, s->states);
+  if (!ok(hipMemcpyAsync(s->g.S2, u, bytes, hipMemcpyHostToDevice, s->stream), err, err_len, "upload state")) return 3;
+  hipLaunchKernelGGL(k_stage<0>, dim3(s->launch_blocks), dim3(kThreads), 0, s->stream, s->g, s->g.S2, s->R);
+  if (!ok(hipGetLastError(), err, err_len, "residual launch") ||
+      !ok(hipMemcpyAsync(r, s->R, bytes, hipMemcpyDeviceToHost, s->stream), err, err_len, "download residual") ||
+      !ok(hipStreamSynchronize(s->stream), err, err_len, "residual"))
+    return 3;
+  return 0;
+}
+
+extern "C" int ugrid_device_available(char* reason, size_t reason_len) {
+  int count = 0;
+  const hipError_t e = hipGetDeviceCount(&count);
+  char line[256];
+  if (e != hipSuccess || count <= 0) {
+    std::snprintf(line, sizeof(line), "hipGetDeviceCount: %s (devices=%d)", hipGetErrorString(e), count);
+    fail(reason, reason_len, line);
+    return 0;
+  }
+  int device = 0;
+  hipDeviceProp_t p{};
+  if (hipGetDevice(&device) != hipSuccess || hipGetDeviceProperties(&p, device) != hipSuccess) {
+    fail(reason, reason_len, "hipGetDeviceProperties failed");
+    return 0;
+  }
+  std::snprintf(line, sizeof(line), "%s, %s", p.name, p.gcnArchName);
+  fail(reason, reason_len, line);
+  return 1;
+}
+
+extern "C" int ugrid_get_launch(const ugrid_session* s, ugrid_launch_info* out) {
+  if (!s || !out) return 1;
+  out->rows = s->g.rows;
+  out->strips = s->g.strips;
+  out->blocks = s->launch_blocks;
+  out->threads = kThreads;
+  return 0;
+}
+
+extern "C" void ugrid_destroy(ugrid_session* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->device);
+  (void)hipStreamSynchronize(s->stream);
+  release(s);
+}

@@ -1,0 +1,185 @@
+"""GPU parity of the tensor-product grid solver (csrc/hip/ugrid.hip through include/ugrid_hip.h)
+against its oracle (oracle/grid_oracle.cpp, the reference's legacy GPU backend restated).
+
+The device kernels follow the backend's formulas operation for operation and are compiled
+without FMA contraction, so the bar is bit-for-bit: residuals, whole runs (state including the
+ghost cells the reference's device array ends with, clocks, last steps, step and rejection
+counts, return codes) on every boundary kind, both fluxes, stretched axes, several cases with
+their own reference Mach numbers, strip and row-band edges (grids not multiples of the 62-column
+strip or of the row band).  At sizes the oracle cannot follow, size-independent properties:
+conservation on a periodic box and the x / y mirror symmetry of a symmetric blast.
+"""
+import numpy as np
+import pytest
+
+from oracle import pygrid as G
+from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import _native as N
+from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import euler2d as E
+
+from .grid_cases import ALL_BCS, RIEMANN_SET, STRETCHES, blast, params, random_smooth, riemann, vortex
+
+pytestmark = pytest.mark.gpu
+
+
+def device_run(s, tend, maxsteps, calls=1):
+    """Run on the device (one session, `calls` consecutive run calls splitting maxsteps)."""
+    out = []
+    with s.device_session() as d:
+        per = [maxsteps // calls] * calls
+        per[-1] += maxsteps - sum(per)
+        for m in per:
+            status, msg = d.run(tend, m)
+            out.append(status)
+        u = np.empty_like(s.U)
+        t = np.empty(s.ncase)
+        dt = np.empty(s.ncase)
+        err = N.err_buffer()
+        assert d.lib.ugrid_download(d.handle, N.dptr(u), u.size, N.dptr(t), N.dptr(dt), err, N.ERR_LEN) == 0
+        return dict(u=u, t=t, dt=dt, status=out[-1], statuses=out, nsteps=d.meta.nsteps, nreject=d.meta.nreject,
+                    message=msg, launch=d.launch_info())
+
+
+def assert_same_run(dev, ora):
+    assert dev["status"] == ora["status"]
+    assert (dev["nsteps"], dev["nreject"]) == (ora["nsteps"], ora["nreject"])
+    np.testing.assert_array_equal(dev["t"], ora["t"])
+    np.testing.assert_array_equal(dev["dt"], ora["dt"])
+    np.testing.assert_array_equal(dev["u"], ora["u"])
+
+
+@pytest.mark.parametrize("flux", [E.FLUX_HLLC, E.FLUX_AUSMPUP])
+@pytest.mark.parametrize("bc", ALL_BCS)
+def test_residual_bitwise(flux, bc):
+    s = random_smooth(131, 45, 3, params(flux=flux, bc=bc, sx=STRETCHES[1][0], sy=STRETCHES[1][1]), seed=flux + 17)
+    print(f"residual flux={flux} bc={bc}", flush=True)
+    with s.device_session() as d:
+        r = d.rhs(s.U)
+        assert d.launch_info()["strips"] >= 3 * 3 * 2
+    np.testing.assert_array_equal(r, G.rhs(s, s.U))
+
+
+RUNS = {
+    "vortex_hllc_63x33": lambda: (vortex(63, ny=33, par=params(bc=ALL_BCS[0])), 0.5),
+    "vortex_ausm_stretched": lambda: (vortex(70, ny=41, par=params(flux=E.FLUX_AUSMPUP, bc=ALL_BCS[0], mco=0.3,
+                                                                   sx=(E.STRETCH_SINH_CENTER, 1.0)), ncase=2), 0.3),
+    "riemann_x_cases": lambda: (riemann(200, 5, "x", RIEMANN_SET), 0.05),
+    "riemann_y_ausm": lambda: (riemann(3, 150, "y", RIEMANN_SET[2:5], params(flux=E.FLUX_AUSMPUP, bc=ALL_BCS[4])),
+                               0.04),
+    "blast_reflect": lambda: (blast(64, states=(RIEMANN_SET[0], RIEMANN_SET[3])), 0.03),
+    "mixed_bc_stretched": lambda: (random_smooth(97, 70, 2, params(bc=ALL_BCS[5], sx=STRETCHES[2][0],
+                                                                   sy=STRETCHES[2][1]), seed=3), 0.05),
+    # One column: transmissive x sides.  (With a reflect or periodic side on an axis of fewer than
+    # two cells, the reference's fill of the outer ghost layer reads another ghost cell, which its
+    # one-thread-per-ghost kernel races on; that layer is never read by the first-order scheme.)
+    "one_column": lambda: (riemann(1, 64, "y", RIEMANN_SET[:2], params(bc=(1, 1, 0, 0))), 0.05),
+    "one_row": lambda: (riemann(90, 1, "x", RIEMANN_SET[:2]), 0.05),
+}
+
+
+@pytest.mark.parametrize("name", sorted(RUNS))
+def test_run_bitwise(name):
+    s, tend = RUNS[name]()
+    print(f"run {name} {s.grid.nx}x{s.grid.ny}x{s.ncase}", flush=True)
+    dev = device_run(s, tend, 5000)
+    ora = G.run(s, tend, 5000)
+    assert ora["status"] == 0 and ora["nsteps"] > 10
+    assert_same_run(dev, ora)
+
+
+def test_run_split_calls_bitwise():
+    s, tend = RUNS["riemann_x_cases"]()
+    ora = G.run(s, tend, 30)
+    dev = device_run(s, tend, 30, calls=3)
+    assert dev["statuses"] == [1, 1, 1]
+    assert_same_run(dev, ora)
+
+
+def test_unstable_fixed_step_status_2():
+    s = riemann(64, 4, "x", [RIEMANN_SET[3]], params(dt_fixed=1e6))
+    dev = device_run(s, 1e6, 100)
+    ora = G.run(s, 1e6, 100)
+    assert ora["status"] == 2 and ora["nreject"] == 13
+    assert_same_run(dev, ora)
+    assert "non-physical" in dev["message"]
+
+
+def test_fixed_step_rejections_then_acceptance():
+    s = riemann(64, 4, "x", RIEMANN_SET[3:5], params(dt_fixed=0.004))
+    dev = device_run(s, 1.0, 6)
+    ora = G.run(s, 1.0, 6)
+    assert ora["nreject"] >= 1 and ora["status"] == 1
+    assert_same_run(dev, ora)
+
+
+def test_control_semantics():
+    s = riemann(70, 3, "x", RIEMANN_SET[:2])
+    for t0, tend, maxsteps in (([0.0, 0.0], 0.05, 0), ([0.05, 0.06], 0.05, 10), ([0.0, 0.04], 0.05, 1000),
+                               ([0.0, 0.05], 0.05, 1000)):
+        s.t[:] = t0
+        ora = G.run(s, tend, maxsteps)
+        dev = device_run(s, tend, maxsteps)
+        assert_same_run(dev, ora)
+    s.t[:] = [0.0, 0.04]
+    full = G.run(s, 0.05, 1000)
+    dev = device_run(s, 0.05, full["nsteps"])
+    assert dev["status"] == 1
+    assert_same_run(dev, G.run(s, 0.05, full["nsteps"]))
+
+
+def test_python_mirror_run():
+    s = vortex(80, ny=50, par=params(flux=E.FLUX_AUSMPUP, bc=ALL_BCS[0], mco=0.2), ncase=2)
+    ora = G.run(s, 0.2, 10000)
+    assert E.solver2d_run_cuda(s, 0.2, 10000) == 0
+    np.testing.assert_array_equal(s.U, ora["u"])
+    np.testing.assert_array_equal(s.t, ora["t"])
+    np.testing.assert_array_equal(s.dt, ora["dt"])
+    assert (s.nsteps, s.nreject) == (ora["nsteps"], ora["nreject"])
+    # a second call continues from the written-back state
+    ora2 = G.run(s, 0.3, 10000)
+    assert E.solver2d_run_cuda(s, 0.3, 10000) == 0
+    np.testing.assert_array_equal(s.U, ora2["u"])
+    assert s.nsteps == ora["nsteps"] + ora2["nsteps"]
+    # refusals
+    s.par.limiter = E.LIM_VENKAT
+    assert E.solver2d_run_cuda(s, 0.4, 10) == 3
+
+
+def test_large_batch_bitwise_short():
+    s = random_smooth(517, 300, 3, params(flux=E.FLUX_HLLC, bc=ALL_BCS[3]), seed=8, amplitude=0.1)
+    ora = G.run(s, 10.0, 3)
+    dev = device_run(s, 10.0, 3)
+    rows = dev["launch"]["rows"]
+    assert dev["launch"]["strips"] == 3 * -(-517 // 62) * -(-300 // rows)
+    assert_same_run(dev, ora)
+
+
+def totals(s):
+    """sum U dx dy per case and variable (numpy order)."""
+    g = s.grid
+    vol = g.dy[2:2 + g.ny, None] * g.dx[None, 2:2 + g.nx]
+    q = s.U.reshape(s.ncase, 4, g.my, g.mx)[:, :, 2:2 + g.ny, 2:2 + g.nx]
+    return (q * vol).sum(axis=(2, 3))
+
+
+def test_large_grid_properties():
+    """Two cases of 1536 x 1536: conservation on the periodic box, and the symmetries of a
+    centred blast at rest in a reflecting box (mirror in x and y, transpose)."""
+    n = 1536
+    s = vortex(n, par=params(bc=ALL_BCS[0]), ncase=2)
+    before = totals(s)
+    print("large vortex", flush=True)
+    assert E.solver2d_run_cuda(s, 0.05, 100000) == 0
+    assert s.nsteps > 10
+    np.testing.assert_allclose(totals(s), before, rtol=1e-11, atol=1e-11)
+    strong = (1.0, 0.0, 0.0, 1000.0, 1.0, 0.0, 0.0, 0.01)
+    b = blast(n, states=(RIEMANN_SET[0], strong))
+    print("large blast", flush=True)
+    assert E.solver2d_run_cuda(b, 0.01, 100000) == 0
+    for c in range(2):
+        q = b.field(c)[:, 2:2 + n, 2:2 + n]
+        assert np.all(np.isfinite(q))
+        tol = dict(rtol=1e-8, atol=1e-10 * float(np.max(np.abs(q[3]))))
+        np.testing.assert_allclose(q[0], q[0][:, ::-1], **tol)
+        np.testing.assert_allclose(q[0], q[0][::-1, :], **tol)
+        np.testing.assert_allclose(q[1], -q[1][:, ::-1], **tol)
+        np.testing.assert_allclose(q[3], q[3].T, **tol)

@@ -1,0 +1,290 @@
+"""Pins of the tensor-product grid oracle (oracle/grid_oracle.cpp, a restatement of the
+reference's legacy GPU backend) and of the host grid geometry, before the oracle is trusted as
+the checker of the HIP ugrid path.
+
+1. Against the reference's own CPU code built from its sources (oracle/_ref): the grid geometry
+   of every stretching law and the boundary fill bit for bit; the legacy fluxes, the residual
+   and whole SSP-RK3 runs within round-off (the backend's formulas group the pressure, energy
+   and collect terms differently from the CPU solver's, src/euler2d_cuda.cu:71-146, 182-258
+   against src/flux.h and src/euler2d.cpp:345-485), with the same step and rejection counts
+   and return codes.
+2. Against closed forms: flux consistency F(W, W) = the Euler flux, supersonic upwinding, mirror
+   antisymmetry, a uniform state's zero residual on every boundary kind, discrete conservation
+   on a periodic box, x / y symmetry of a transposed problem, the 12-retry failure of an
+   unstable fixed step, and the control semantics (maxsteps, done, per-case clocks).
+A plausible slip (a dropped term, a wrong sign, a transposed index) fails at least one.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import pygrid as G
+from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import euler2d as E
+
+from .grid_cases import ALL_BCS, RIEMANN_SET, STRETCHES, blast, params, random_smooth, riemann, vortex
+
+needs_ref = pytest.mark.skipif(not G.ref_available(), reason="reference CPU build (oracle/_ref) not present")
+
+
+def interior(s, u):
+    g = s.grid
+    return u.reshape(s.ncase, 4, g.my, g.mx)[:, :, 2:2 + g.ny, 2:2 + g.nx]
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+# ------------------------------------------------------------------ geometry --
+@needs_ref
+@pytest.mark.parametrize("sx, sy", STRETCHES + [((E.STRETCH_TANH_ENDS, 0.0), (E.STRETCH_GEOMETRIC, 1.0))])
+@pytest.mark.parametrize("bc", [ALL_BCS[0], ALL_BCS[2], ALL_BCS[5]])
+def test_axis_geometry_matches_reference_bitwise(sx, sy, bc):
+    s = E.Solver2D(37, 23, -1.25, 2.5, 0.3, 1.1, 1, params(bc=bc, sx=sx, sy=sy))
+    ref = G.ref_geometry(s)
+    g = s.grid
+    for name in ("xf", "xc", "dx", "yf", "yc", "dy"):
+        np.testing.assert_array_equal(getattr(g, name), ref[name], err_msg=name)
+
+
+def test_axis_closed_forms():
+    xf = E.grid2d_stretch_faces(10, -1.0, 3.0, E.Stretch(E.STRETCH_UNIFORM, 0.0))
+    assert xf[0] == -1.0 and xf[-1] == 3.0
+    np.testing.assert_allclose(np.diff(xf), 0.4, rtol=1e-14)
+    r = 1.1
+    xf = E.grid2d_stretch_faces(12, 0.0, 1.0, E.Stretch(E.STRETCH_GEOMETRIC, r))
+    w = np.diff(xf)
+    np.testing.assert_allclose(w[1:] / w[:-1], r, rtol=1e-12)
+    xf = E.grid2d_stretch_faces(16, -2.0, 2.0, E.Stretch(E.STRETCH_TANH_ENDS, 2.0))
+    np.testing.assert_allclose(xf + xf[::-1], 0.0, atol=1e-15)  # symmetric, refined at both ends
+    assert np.diff(xf)[0] < np.diff(xf)[8]
+    xf = E.grid2d_stretch_faces(16, -2.0, 2.0, E.Stretch(E.STRETCH_SINH_CENTER, 2.0))
+    assert np.diff(xf)[8] < np.diff(xf)[0]
+    # ghosts: periodic translates, otherwise mirrors
+    p = params(bc=(E.BC_PERIODIC, E.BC_PERIODIC, E.BC_REFLECT, E.BC_TRANSMISSIVE), sx=(1, 1.05), sy=(1, 1.05))
+    s = E.Solver2D(9, 9, 0.0, 1.0, 0.0, 1.0, 1, p)
+    g = s.grid
+    assert g.dx[0] == g.dx[9] and g.dx[1] == g.dx[10] and g.xc[1] == g.xc[10] - 1.0
+    assert g.dy[1] == g.dy[2] and g.yc[1] == -g.yc[2]
+    with pytest.raises(ValueError):
+        E.Solver2D(9, 9, 0.0, 1.0, 0.0, 1.0, 1, params(bc=(E.BC_PERIODIC, E.BC_REFLECT, 1, 1)))
+
+
+# ------------------------------------------------------------- boundary fill --
+@needs_ref
+@pytest.mark.parametrize("bc", ALL_BCS)
+def test_boundary_fill_matches_reference_bitwise(bc):
+    s = random_smooth(11, 7, 2, params(bc=bc), seed=3)
+    np.testing.assert_array_equal(G.apply_bc(s, s.U), G.ref_apply_bc(s, s.U))
+
+
+def test_boundary_fill_closed_forms():
+    s = random_smooth(6, 5, 1, params(bc=(E.BC_REFLECT, E.BC_REFLECT, E.BC_TRANSMISSIVE, E.BC_REFLECT)), seed=1)
+    u = G.apply_bc(s, s.U).reshape(4, 9, 10)
+    q = u[:, 2:7, 2:8]
+    np.testing.assert_array_equal(u[0, 2:7, 1], q[0, :, 0])     # reflect: rho mirrored
+    np.testing.assert_array_equal(u[1, 2:7, 1], -q[1, :, 0])    # x momentum negated
+    np.testing.assert_array_equal(u[1, 2:7, 0], -q[1, :, 1])
+    np.testing.assert_array_equal(u[2, 1, 2:8], q[2, 0, :])     # transmissive at ymin
+    np.testing.assert_array_equal(u[2, 7, 2:8], -q[2, 4, :])    # reflect at ymax: y momentum negated
+    np.testing.assert_array_equal(u[3, 8, 2:8], q[3, 3, :])
+
+
+# ------------------------------------------------------------------- fluxes --
+def _states(rng, n):
+    out = []
+    for _ in range(n):
+        L = [rng.uniform(0.1, 3), rng.uniform(-3, 3), rng.uniform(-2, 2), rng.uniform(0.05, 5)]
+        R = [rng.uniform(0.1, 3), rng.uniform(-3, 3), rng.uniform(-2, 2), rng.uniform(0.05, 5)]
+        out.append((L, R))
+    return out
+
+
+@needs_ref
+@pytest.mark.parametrize("kind", [0, 1])
+def test_legacy_flux_matches_reference_within_roundoff(kind):
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for L, R in _states(rng, 400):
+        mco = rng.uniform(0.01, 1.0)
+        a, b = G.flux(kind, L, R, 1.4, mco), G.ref_flux(kind, L, R, 1.4, mco)
+        scale = max(np.max(np.abs(b)), 1.0)
+        worst = max(worst, np.max(np.abs(a - b)) / scale)
+    assert worst < 1e-13, worst
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_legacy_flux_closed_forms(kind):
+    rng = np.random.default_rng(5)
+    gm = 1.4
+    for _ in range(100):
+        r, u, v, p = rng.uniform(0.2, 2), rng.uniform(-2, 2), rng.uniform(-2, 2), rng.uniform(0.1, 3)
+        E_ = p / (gm - 1) + 0.5 * r * (u * u + v * v)
+        exact = np.array([r * u, r * u * u + p, r * u * v, u * (E_ + p)])
+        f = G.flux(kind, [r, u, v, p], [r, u, v, p], gm, 1.0)
+        np.testing.assert_allclose(f, exact, rtol=1e-13, atol=1e-13 * (abs(r * u * u) + p))
+    # supersonic to the right: the flux is the left state's exact flux
+    L, R = [1.0, 3.0, 0.4, 0.7], [0.6, 2.5, -0.3, 0.4]
+    E_ = L[3] / (gm - 1) + 0.5 * L[0] * (L[1] ** 2 + L[2] ** 2)
+    np.testing.assert_allclose(G.flux(kind, L, R, gm, 1.0),
+                               [L[0] * L[1], L[0] * L[1] ** 2 + L[3], L[0] * L[1] * L[2], L[1] * (E_ + L[3])],
+                               rtol=1e-14)
+    # mirror: swapping sides and negating normal velocities negates mass, tangential and energy flux
+    for Lw, Rw in _states(rng, 50):
+        f = G.flux(kind, Lw, Rw, gm, 0.5)
+        mL = [Rw[0], -Rw[1], Rw[2], Rw[3]]
+        mR = [Lw[0], -Lw[1], Lw[2], Lw[3]]
+        g = G.flux(kind, mL, mR, gm, 0.5)
+        np.testing.assert_allclose(g, [-f[0], f[1], -f[2], -f[3]], rtol=1e-12, atol=1e-12 * np.max(np.abs(f)))
+
+
+# ------------------------------------------------------------------ residual --
+@needs_ref
+@pytest.mark.parametrize("flux", [E.FLUX_HLLC, E.FLUX_AUSMPUP])
+@pytest.mark.parametrize("bc", ALL_BCS)
+@pytest.mark.parametrize("stretch", STRETCHES)
+def test_residual_matches_reference_within_roundoff(flux, bc, stretch):
+    s = random_smooth(13, 9, 2, params(flux=flux, bc=bc, sx=stretch[0], sy=stretch[1]), seed=flux * 7 + 1)
+    a, b = G.rhs(s, s.U), G.ref_rhs(s, s.U)
+    assert rel(interior(s, a), interior(s, b)) < 1e-12
+
+
+@pytest.mark.parametrize("flux", [E.FLUX_HLLC, E.FLUX_AUSMPUP])
+@pytest.mark.parametrize("bc", ALL_BCS)
+def test_residual_uniform_state_is_zero(flux, bc):
+    s = E.Solver2D(10, 7, 0.0, 1.0, 0.0, 2.0, 1, params(flux=flux, bc=bc, sx=(1, 1.07)))
+    if E.BC_REFLECT in bc:
+        s.set_primitive_field(0, 1.3, 0.0, 0.0, 0.9)   # a wall at rest only keeps a state at rest
+    else:
+        s.set_primitive_field(0, 1.3, 0.4, -0.2, 0.9)
+    assert np.all(G.rhs(s, s.U) == 0.0)
+
+
+def test_residual_conservation_on_periodic_box():
+    s = random_smooth(24, 16, 1, params(bc=ALL_BCS[0], sx=(1, 1.02)), seed=4)
+    r = interior(s, G.rhs(s, s.U))[0]
+    vol = s.grid.dy[2:18][:, None] * s.grid.dx[2:26][None, :]
+    for v in range(4):
+        total = float(np.sum(r[v] * vol))
+        assert abs(total) < 1e-12 * float(np.sum(np.abs(r[v] * vol))), (v, total)
+
+
+def test_residual_transpose_symmetry():
+    """A problem and its x / y transpose give transposed residuals (u and v swapped)."""
+    a = random_smooth(12, 12, 1, params(bc=(E.BC_REFLECT, E.BC_TRANSMISSIVE, E.BC_REFLECT, E.BC_TRANSMISSIVE)),
+                      seed=9)
+    b = E.Solver2D(12, 12, -0.4, 0.6, 0.0, 1.3, 1, params(bc=(E.BC_REFLECT, E.BC_TRANSMISSIVE, E.BC_REFLECT,
+                                                               E.BC_TRANSMISSIVE)))
+    qa = a.field(0)
+    qb = b.field(0)
+    qb[0] = qa[0].T
+    qb[1] = qa[2].T
+    qb[2] = qa[1].T
+    qb[3] = qa[3].T
+    ra = interior(a, G.rhs(a, a.U))[0]
+    rb = interior(b, G.rhs(b, b.U))[0]
+    np.testing.assert_allclose(rb[0], ra[0].T, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(rb[1], ra[2].T, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(rb[3], ra[3].T, rtol=1e-12, atol=1e-12)
+
+
+# ----------------------------------------------------------------------- run --
+RUNS = {
+    "vortex_hllc": lambda: vortex(24, par=params(bc=ALL_BCS[0])),
+    "vortex_ausm_stretched": lambda: vortex(20, ny=26, par=params(flux=E.FLUX_AUSMPUP, bc=ALL_BCS[0], mco=0.3,
+                                                               sx=(E.STRETCH_SINH_CENTER, 1.0))),
+    "riemann_x_cases": lambda: riemann(40, 3, "x", RIEMANN_SET[:4]),
+    "riemann_y_ausm": lambda: riemann(3, 40, "y", RIEMANN_SET[2:5], params(flux=E.FLUX_AUSMPUP, bc=ALL_BCS[4])),
+    "blast_reflect": lambda: blast(20),
+}
+
+
+@needs_ref
+@pytest.mark.parametrize("name", sorted(RUNS))
+def test_run_matches_reference_within_roundoff(name):
+    s = RUNS[name]()
+    tend = 0.6 if name.startswith("vortex") else 0.08
+    a, b = G.run(s, tend, 100000), G.ref_run(s, tend, 100000)
+    assert (a["status"], a["nsteps"], a["nreject"]) == (b["status"], b["nsteps"], b["nreject"])
+    assert a["status"] == 0
+    np.testing.assert_allclose(a["t"], b["t"], rtol=1e-13)
+    np.testing.assert_allclose(a["dt"], b["dt"], rtol=1e-9)
+    assert rel(interior(s, a["u"]), interior(s, b["u"])) < 1e-10
+
+
+@needs_ref
+def test_unstable_fixed_step_fails_after_twelve_retries_like_reference():
+    s = riemann(30, 2, "x", [RIEMANN_SET[3]], params(dt_fixed=1e6))
+    a, b = G.run(s, 1e6, 100), G.ref_run(s, 1e6, 100)
+    assert a["status"] == b["status"] == 2
+    assert a["nreject"] == b["nreject"] == 13 and a["nsteps"] == b["nsteps"] == 0
+    np.testing.assert_array_equal(interior(s, a["u"]), interior(s, s.U))
+    np.testing.assert_array_equal(a["t"], s.t)
+
+
+def test_fixed_step_with_rejections_then_acceptance():
+    """A fixed step too large for the blast is halved until admissible; the halved step is kept
+    for the accepted step (t advances by it) and the next step starts again from dt_fixed."""
+    s = riemann(30, 2, "x", [RIEMANN_SET[3]], params(dt_fixed=0.004))
+    a = G.run(s, 1.0, 1)
+    assert a["status"] == 1 and a["nsteps"] == 1 and a["nreject"] >= 1
+    assert a["dt"][0] == 0.004 / 2 ** a["nreject"] and a["t"][0] == a["dt"][0]
+
+
+def test_control_semantics():
+    s = riemann(20, 2, "x", RIEMANN_SET[:2])
+    r = G.run(s, 0.05, 0)
+    assert r["status"] == 1 and r["nsteps"] == 0
+    s.t[:] = [0.05, 0.06]
+    r = G.run(s, 0.05, 10)
+    assert r["status"] == 0 and r["nsteps"] == 0
+    s.t[:] = [0.0, 0.04]
+    full = G.run(s, 0.05, 1000)
+    assert full["status"] == 0 and full["t"][0] == 0.05 and full["t"][1] == 0.05
+    exact = G.run(s, 0.05, full["nsteps"])
+    assert exact["status"] == 1  # the last allowed step ends the call with 1 even when all are done
+    np.testing.assert_array_equal(exact["u"], full["u"])
+    # the case already at tend is carried with dt = 0: its clock stays, and its state goes through
+    # the RK combinations with a zero increment (3/4 U + 1/4 U rounds), as the reference's
+    # backend copies every case's candidate (src/euler2d_cuda.cu:682)
+    s.t[:] = [0.0, 0.05]
+    part = G.run(s, 0.05, 1000)
+    assert part["dt"][1] == 0.0 and part["t"][1] == 0.05
+    np.testing.assert_allclose(interior(s, part["u"])[1], interior(s, s.U)[1], rtol=1e-14, atol=0)
+
+
+def test_two_calls_equal_one():
+    s = vortex(16, par=params(bc=ALL_BCS[0]))
+    one = G.run(s, 0.4, 1000)
+    half = G.run(s, 0.4, 7)
+    s2 = vortex(16, par=params(bc=ALL_BCS[0]))
+    s2.U[:], s2.t[:], s2.dt[:] = half["u"], half["t"], half["dt"]
+    rest = G.run(s2, 0.4, 1000)
+    assert half["nsteps"] + rest["nsteps"] == one["nsteps"]
+    np.testing.assert_array_equal(interior(s, rest["u"]), interior(s, one["u"]))
+    np.testing.assert_array_equal(rest["t"], one["t"])
+
+
+def test_batch_equals_cases_alone_without_rejections():
+    """Each case's clock and step are its own: while every case runs (no rejection, none at
+    tend), a case in the batch is bitwise the case alone."""
+    states = RIEMANN_SET[:4]
+    batch = riemann(32, 3, "x", states)
+    both = G.run(batch, 10.0, 25)
+    assert both["status"] == 1 and len(set(both["dt"])) == len(states)
+    for c, st in enumerate(states):
+        alone = riemann(32, 3, "x", [st])
+        r = G.run(alone, 10.0, 25)
+        np.testing.assert_array_equal(interior(alone, r["u"])[0], interior(batch, both["u"])[c])
+        assert r["t"][0] == both["t"][c]
+
+
+def test_run_conserves_on_periodic_box():
+    s = vortex(20, par=params(flux=E.FLUX_AUSMPUP, bc=ALL_BCS[0], sy=(E.STRETCH_TANH_ENDS, 0.8)))
+    before = [s.integral(0, v) for v in range(4)]
+    r = G.run(s, 0.5, 1000)
+    s.U[:] = r["u"]
+    after = [s.integral(0, v) for v in range(4)]
+    for v in range(4):
+        assert abs(after[v] - before[v]) <= 1e-12 * max(abs(before[v]), 1.0), v
