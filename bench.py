@@ -153,6 +153,90 @@ def config1_workload(args) -> dict:
     }
 
 
+def legacy_workload(args) -> dict:
+    """The legacy tensor-product grid solver (include/ugrid_hip.h): --legacy-cases blast cases of
+    --legacy-n x --legacy-n cells sharing one grid on [-1, 1]^2, transmissive sides, HLLC, first
+    order, FP64, CFL 0.4; each case its own Riemann state pair (inside radius 0.3 / outside).
+    --warmup untimed steps, then --steps steps timed (one device run call each way).
+    roofline: the second RK stage kernel (reads the stage-1 candidate and the base state, writes
+    the stage-2 candidate: 96 algorithmic bytes per cell) timed with HIP events per launch.
+    cpu_baseline: the FP64 oracle of the legacy backend (one thread) on one case of 256 x 256."""
+    from oracle import pygrid as PG
+    from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import euler2d as E
+
+    states = [(1.0, 0.0, 0.0, 1.0, 0.125, 0.0, 0.0, 0.1), (1.0, 0.0, 0.0, 1000.0, 1.0, 0.0, 0.0, 0.01),
+              (1.0, 0.75, 0.0, 1.0, 0.125, 0.0, 0.0, 0.1), (1.0, 0.0, 0.3, 10.0, 1.0, 0.0, -0.2, 0.1)]
+
+    def blast_batch(n, ncase):
+        par = E.params2d_default()
+        par.limiter = E.LIM_FIRST_ORDER
+        s = E.Solver2D(n, n, -1.0, 1.0, -1.0, 1.0, ncase, par)
+        X = s.grid.xc[2:2 + n][None, :]
+        Y = s.grid.yc[2:2 + n][:, None]
+        inside = np.broadcast_to(X * X + Y * Y < 0.09, (n, n))
+        for c in range(ncase):
+            st = states[c % len(states)]
+            s.set_primitive_field(c, *[np.where(inside, st[k], st[4 + k]) for k in range(4)])
+        return s
+
+    n, ncase = args.legacy_n, args.legacy_cases
+    cells = n * n * ncase
+    s = blast_batch(n, ncase)
+    log(f"legacy grid: {ncase} cases of {n}x{n} ({cells / 1e6:.1f}M cells), FP64 HLLC")
+    with s.device_session() as d:
+        info = d.launch_info()
+        status, msg = d.run(1e9, args.warmup)
+        assert status == 1, msg
+        from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import _native as N
+        t0 = time.perf_counter()
+        status, msg = d.run(1e9, args.steps)
+        wall = time.perf_counter() - t0
+        assert status == 1, msg
+        us = np.zeros(3)
+        err = N.err_buffer()
+        assert d.lib.ugrid_profile(d.handle, 20, N.dptr(us), err, N.ERR_LEN) == 0, err.value
+        d.pull()
+    assert np.all(np.isfinite(s.U))
+    copy_rates = np.zeros(4)
+    err = N.err_buffer()
+    if N.load_library().amr_hip_copy_variants(1 << 30, 20, N.dptr(copy_rates), 4, err, N.ERR_LEN):
+        raise SystemExit(err.value.decode())
+    ms_per_step = wall / args.steps * 1e3
+    value = cells * args.steps / wall
+    stage_bytes = [64 * cells, 96 * cells, 96 * cells]
+    rates = [stage_bytes[k] / (us[k] * 1e-6) / 1e9 for k in range(3)]
+    log(f"legacy grid: {ms_per_step:.3f} ms/step, {value / 1e9:.2f} G cell-steps/s; stage us "
+        f"{us[0]:.1f} / {us[1]:.1f} / {us[2]:.1f} -> {rates[0]:.0f} / {rates[1]:.0f} / {rates[2]:.0f} GB/s; "
+        f"launch {info}")
+    # CPU baseline: the oracle on one case of 256 x 256.
+    cpu_n, cpu_steps = 256, 0
+    c = blast_batch(cpu_n, 1)
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < args.cpu_seconds:
+        r = PG.run(c, 1e9, 5)
+        c.U[:], c.t[:], c.dt[:] = r["u"], r["t"], r["dt"]
+        cpu_steps += 5
+    cpu_wall = time.perf_counter() - t0
+    cpu_rate = cpu_n * cpu_n * cpu_steps / cpu_wall
+    log(f"legacy grid cpu oracle (1 thread): {cpu_rate / 1e6:.2f}M cell-steps/s over {cpu_steps} steps")
+    return {
+        "metric": "legacy tensor-product grid solver: cell-steps per second (FP64, first order, SSP-RK3)",
+        "value": round(value), "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic: circular blasts, four Riemann state pairs",
+        "config": {"workload": f"legacy grid: {ncase} blast cases of {n}x{n}, HLLC, transmissive, CFL 0.4",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(rates[1], 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(rates[1] / 8000.0, 4), "traffic": None,
+                     "kernel": "k_stage<2> (second RK stage: 96 algorithmic bytes per cell)",
+                     "stage_us": [round(x, 2) for x in us], "stage_GBps": [round(x, 1) for x in rates],
+                     "stream_copy_variants_gbps": [round(x, 1) for x in copy_rates]},
+        "cpu_baseline": {"value": round(cpu_rate), "unit": "cell-steps/s", "cores": 1, "kind": "port",
+                         "sample": f"{cpu_steps} steps of one {cpu_n}x{cpu_n} blast case on the FP64 oracle"},
+        "launch": info,
+    }
+
+
 def serve_workload(args, n_requests: int | None = None) -> dict:
     """Continuous batching: requests (config-3 airfoils on LF1 / LF2 / level-7 meshes, 50/35/15 %)
     arrive as a Poisson stream and are admitted into an arena of slot classes sized by the HBM
@@ -261,6 +345,10 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=192, help="slots of the serving arena")
     ap.add_argument("--serve-load", type=float, default=0.8, help="arrival rate as a fraction of estimated capacity")
     ap.add_argument("--hbm-budget-gb", type=float, default=200.0, help="device memory the serving arena may use")
+    ap.add_argument("--legacy", action="store_true",
+                    help="bench the legacy tensor-product grid solver instead (its own JSON line)")
+    ap.add_argument("--legacy-n", type=int, default=1024, help="legacy grid: cells per side")
+    ap.add_argument("--legacy-cases", type=int, default=16, help="legacy grid: cases in the batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU (this build has no multi-GPU path); got --gpus {args.gpus}",
@@ -284,6 +372,9 @@ def main():
         return
     if args.serve:
         print(json.dumps(serve_workload(args)), flush=True)
+        return
+    if args.legacy:
+        print(json.dumps(legacy_workload(args)), flush=True)
         return
 
     # 0. Workload: seeded airfoils, LF2 meshes built on host threads (the mesher releases the GIL).

@@ -93,6 +93,11 @@ int ugrid_device_available(char* reason, size_t reason_len);
  * nreject, implicit, ib, cut.  Writes min(capacity, 36) values, returns 36. */
 int ugrid_solver2d_layout(int64_t* out, int32_t capacity);
 
+/* Average duration in microseconds of each RK stage kernel (stage_us[0..2]) over `trials`
+ * trials launched directly with HIP events around every kernel; the batch is advanced by
+ * those trials (towards t = infinity, no step cap).  For the bench. */
+int ugrid_profile(ugrid_session* s, int32_t trials, double* stage_us, char* err, size_t err_len);
+
 /* Rows per wave strip and the launch shape of the stage kernels (for tests and the bench). */
 typedef struct ugrid_launch_info {
   int32_t rows, strips, blocks, threads;

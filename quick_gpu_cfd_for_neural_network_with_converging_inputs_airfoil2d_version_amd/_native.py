@@ -301,6 +301,8 @@ def _bind_ugrid_api(lib):
         "ugrid_destroy": (None, [H]),
         "ugrid_get_launch": (C.c_int, [H, C.POINTER(ugrid_launch_info)]),
         "ugrid_device_available": (C.c_int, [C.c_char_p, C.c_size_t]),
+        "ugrid_profile": (C.c_int, [H, C.c_int32, c_double_p, C.c_char_p, C.c_size_t]),
+        "ugrid_solver2d_layout": (C.c_int, [C.POINTER(C.c_int64), C.c_int32]),
     }
     for name, (restype, argtypes) in sig.items():
         fn = getattr(lib, name)

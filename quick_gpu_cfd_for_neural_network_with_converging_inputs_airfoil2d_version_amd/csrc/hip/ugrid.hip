@@ -18,17 +18,17 @@
 //     normal velocity for a reflection -- bit for bit the primitives of the reference's filled
 //     ghost (negation is exact).
 //   * the RK update, the admissibility test of the candidate and, in the last stage, the
-//     candidate's wave-speed bound for the next step's dt are fused into the same pass.  The
-//     last block of the last stage (one counted atomic) commits or rejects the trial for all
-//     cases: t += dt, step counters, buffer swap (U[cur] and the other buffer alternate, so no
+//     candidate's wave-speed bound for the next step's dt are fused into the same pass.  A
+//     one-wave kernel after the last stage commits or rejects the trial for all cases: t += dt, step counters, buffer swap (U[cur] and the other buffer alternate, so no
 //     accept copy), next dt from the bound, or dt halving and the 12-retry limit.
-//   * a trial is three launches; the host replays a graph of several trials and polls the
+//   * a trial is four launches; the host replays a graph of several trials and polls the
 //     control record between replays.
 // The arithmetic of every formula follows the reference's GPU kernels operation for operation
 // and this file is compiled without FMA contraction, so results are bitwise those of the CPU
 // restatement of that backend.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -61,7 +61,7 @@ struct Control {
   int retries;     // rejections of the current step
   int cur;         // U[cur] holds the accepted state
   int ghost_from;  // download ghosts: 0 as uploaded, 1 boundary fill of U[cur], 2 of U[cur ^ 1]
-  unsigned arrivals;  // blocks of the last stage that finished this trial
+  int pad;
 };
 
 struct Grid {
@@ -267,44 +267,38 @@ __device__ __forceinline__ int logical_block() {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Commit of a trial (the last block of the last stage): src/euler2d_cuda.cu:670-693.
+// Commit of a trial (one wave, after the last stage): src/euler2d_cuda.cu:670-693.
 // ---------------------------------------------------------------------------------------------
 __device__ void commit(const Grid& g) {
   Control* c = g.ctrl;
-  __shared__ int s_flag;
-  __shared__ int s_maxed;
-  const int tid = threadIdx.x;
-  if (tid == 0) s_flag = 0;
-  __syncthreads();
+  const int lane = threadIdx.x % kLanes;
   if (__hip_atomic_load(&c->bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
     // Rejected: halve every positive step; give up after 12 retries or on an unusable step.
-    for (int k = tid; k < g.ncase; k += blockDim.x) {
+    int unusable = 0;
+    for (int k = lane; k < g.ncase; k += kLanes) {
       double d = g.dt[k];
       if (d > 0.0) d *= 0.5;
       g.dt[k] = d;
-      if (d < 0.0 || !isfinite(d)) s_flag = 1;
+      if (d < 0.0 || !isfinite(d)) unusable = 1;
       g.smax[k] = 0;
     }
-    __syncthreads();
-    if (tid == 0) {
+    unusable = __any(unusable);
+    if (lane == 0) {
       const int retries = c->retries + 1;
       c->retries = retries;
       c->nreject += 1;
-      if (retries > kMaxRetries || s_flag) c->status = 2;
+      if (retries > kMaxRetries || unusable) c->status = 2;
       c->bad = 0;
-      c->arrivals = 0;
     }
     return;
   }
   // Accepted: advance the clocks of the cases that moved, then either stop (maxsteps reached,
   // or every case at tend) or choose the next step from the candidate's wave-speed bound.
   const double tend = c->tend;
-  if (tid == 0) {
-    const long long steps = c->steps + 1;
-    c->steps = steps;
-    s_maxed = steps >= c->maxsteps;
-  }
-  for (int k = tid; k < g.ncase; k += blockDim.x) {
+  const long long steps = c->steps + 1;
+  const bool maxed = steps >= c->maxsteps;
+  int open = 0;
+  for (int k = lane; k < g.ncase; k += kLanes) {
     const double d = g.dt[k];
     g.dt_out[k] = d;
     double tk = g.t[k];
@@ -312,21 +306,20 @@ __device__ void commit(const Grid& g) {
       tk += d;
       g.t[k] = tk;
     }
-    if (tk < tend) s_flag = 1;
+    if (tk < tend) open = 1;
   }
-  __syncthreads();
-  const bool next = !s_maxed && s_flag;
-  for (int k = tid; k < g.ncase; k += blockDim.x) {
+  const bool next = !maxed && __any(open);
+  for (int k = lane; k < g.ncase; k += kLanes) {
     if (next) g.dt[k] = next_dt(g, __longlong_as_double(static_cast<long long>(load_relaxed(&g.smax[k]))), g.t[k], tend);
     g.smax[k] = 0;
   }
-  if (tid == 0) {
+  if (lane == 0) {
+    c->steps = steps;
     c->cur ^= 1;
     c->nsteps += 1;
     c->retries = 0;
     c->ghost_from = next ? 1 : 2;
-    if (!next) c->status = s_maxed ? 1 : 0;
-    c->arrivals = 0;
+    if (!next) c->status = maxed ? 1 : 0;
   }
 }
 
@@ -334,7 +327,7 @@ __device__ void commit(const Grid& g) {
 // One RK stage (MODE 1..3) or the residual alone (MODE 0), fused over a wave strip.
 //   MODE 1: U[cur^1] = U + dt L(U)                    (in U[cur])
 //   MODE 2: S2 = 3/4 U + 1/4 (U1 + dt L(U1))          (U1 in U[cur^1])
-//   MODE 3: U[cur^1] = 1/3 U + 2/3 (S2 + dt L(S2)), then the commit
+//   MODE 3: U[cur^1] = 1/3 U + 2/3 (S2 + dt L(S2)), and the candidate's wave-speed bound
 //   MODE 0: R = L(in)
 // ---------------------------------------------------------------------------------------------
 template <int MODE>
@@ -449,35 +442,42 @@ __global__ __launch_bounds__(kThreads) void k_stage(Grid g, const double* __rest
   if (MODE == 3) {
     bound = wave_max(bound);
     if (work && lane == 0 && bound) atomicMax(&g.smax[icase], bound);
-    __shared__ int s_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      __threadfence();
-      const unsigned prev = atomicAdd(&ctrl->arrivals, 1u);
-      s_last = prev == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    commit(g);
   }
 }
 
+// The commit of a trial, one wave after the last stage (the kernel boundary orders every
+// stage's flags and bounds before it; an in-kernel last-arriver commit would need device-scope
+// fences, which write back and invalidate an XCD's L2 and cost more than this launch).
+__global__ __launch_bounds__(kLanes) void k_commit(Grid g) {
+  if (g.ctrl->status != RUNNING) return;
+  commit(g);
+}
+
 // Wave-speed bound of every case at U[cur] (the first step of a run, src/euler2d_cuda.cu:223-237).
-__global__ __launch_bounds__(kThreads) void k_bound(Grid g) {
-  const int cur = g.ctrl->cur;
-  const double* u = g.U[cur];
+// Block b serves case b / per_case; each block reduces its cells to one maximum (on the bit
+// patterns, as the reference's atomicMax does) and issues one atomic.
+__global__ __launch_bounds__(kThreads) void k_bound(Grid g, int per_case) {
+  const int icase = blockIdx.x / per_case;
+  const int part = blockIdx.x - icase * per_case;
+  const double* u = g.U[g.ctrl->cur] + static_cast<size_t>(icase) * 4 * g.plane;
   const size_t inner = static_cast<size_t>(g.nx) * g.ny;
-  const size_t n = inner * g.ncase;
-  for (size_t q = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; q < n;
-       q += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    const int icase = static_cast<int>(q / inner);
-    const int ij = static_cast<int>(q - icase * inner);
-    const int i = ij % g.nx, j = ij / g.nx;
-    const size_t k = static_cast<size_t>(icase) * 4 * g.plane + static_cast<size_t>(j + kGhosts) * g.mx + i + kGhosts;
+  unsigned long long best = 0;
+  for (size_t q = part * static_cast<size_t>(blockDim.x) + threadIdx.x; q < inner;
+       q += static_cast<size_t>(per_case) * blockDim.x) {
+    const int j = static_cast<int>(q / g.nx), i = static_cast<int>(q - static_cast<size_t>(j) * g.nx);
+    const size_t k = static_cast<size_t>(j + kGhosts) * g.mx + i + kGhosts;
     const double b = wave_bound(u[k], u[k + g.plane], u[k + 2 * g.plane], u[k + 3 * g.plane], g, g.dx[i + kGhosts],
                                 g.dy[j + kGhosts]);
-    atomicMax(&g.smax[icase], static_cast<unsigned long long>(__double_as_longlong(b)));
+    const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(b));
+    best = bits > best ? bits : best;
+  }
+  best = wave_max(best);
+  __shared__ unsigned long long s_best[kWaves];
+  if (threadIdx.x % kLanes == 0) s_best[threadIdx.x / kLanes] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kWaves; ++w) best = s_best[w] > best ? s_best[w] : best;
+    if (best) atomicMax(&g.smax[icase], best);
   }
 }
 
@@ -504,8 +504,7 @@ __global__ __launch_bounds__(kThreads) void k_open(Grid g, double tend, long lon
     c->steps = 0;
     c->retries = 0;
     c->bad = 0;
-    c->arrivals = 0;
-    if (maxsteps <= 0)
+        if (maxsteps <= 0)
       c->status = 1;
     else if (!s_open)
       c->status = 0;
@@ -615,11 +614,21 @@ void fill_ghosts(const Grid& g, double* u) {
     }
 }
 
+// Blocks per case of k_bound: about 2048 blocks over the batch, at most one per 256 cells.
+void launch_bound(ugrid_session* s) {
+  const Grid& g = s->g;
+  const long long inner = static_cast<long long>(g.nx) * g.ny;
+  const long long cap = (inner + kThreads - 1) / kThreads;
+  const int per_case = static_cast<int>(std::max<long long>(1, std::min<long long>(cap, 2048 / g.ncase)));
+  hipLaunchKernelGGL(k_bound, dim3(per_case * g.ncase), dim3(kThreads), 0, s->stream, g, per_case);
+}
+
 int launch_trial(ugrid_session* s, hipStream_t stream) {
   hipLaunchKernelGGL(k_stage<1>, dim3(s->launch_blocks), dim3(kThreads), 0, stream, s->g, nullptr, nullptr);
   hipLaunchKernelGGL(k_stage<2>, dim3(s->launch_blocks), dim3(kThreads), 0, stream, s->g, nullptr, nullptr);
   hipLaunchKernelGGL(k_stage<3>, dim3(s->launch_blocks), dim3(kThreads), 0, stream, s->g, nullptr, nullptr);
-  return 3;
+  hipLaunchKernelGGL(k_commit, dim3(1), dim3(kLanes), 0, stream, s->g);
+  return 4;
 }
 
 bool build_graph(ugrid_session* s, int trials, char* err, size_t len) {
@@ -763,9 +772,7 @@ extern "C" int ugrid_run(ugrid_session* s, double tend, int64_t maxsteps, ugrid_
                          size_t err_len) {
   if (!s) return fail(err, err_len, "ugrid_run: null session");
   const Grid& g = s->g;
-  const int bound_blocks = static_cast<int>(
-      std::min<long long>((static_cast<long long>(g.nx) * g.ny * g.ncase + kThreads - 1) / kThreads, 4096));
-  hipLaunchKernelGGL(k_bound, dim3(bound_blocks), dim3(kThreads), 0, s->stream, g);
+  launch_bound(s);
   hipLaunchKernelGGL(k_open, dim3(1), dim3(kThreads), 0, s->stream, g, tend, static_cast<long long>(maxsteps));
   if (!ok(hipGetLastError(), err, err_len, "run start")) return 3;
   // Every trial of a running batch ends in an accept or a rejection, so each replay of the
@@ -855,6 +862,42 @@ extern "C" int ugrid_rhs(ugrid_session* s, const double* u, uint64_t u_len, doub
       !ok(hipMemcpyAsync(r, s->R, bytes, hipMemcpyDeviceToHost, s->stream), err, err_len, "download residual") ||
       !ok(hipStreamSynchronize(s->stream), err, err_len, "residual"))
     return 3;
+  return 0;
+}
+
+extern "C" int ugrid_profile(ugrid_session* s, int32_t trials, double* stage_us, char* err, size_t err_len) {
+  if (!s || !stage_us || trials < 1) return fail(err, err_len, "ugrid_profile: invalid argument");
+  const Grid& g = s->g;
+  launch_bound(s);
+  hipLaunchKernelGGL(k_open, dim3(1), dim3(kThreads), 0, s->stream, g, HUGE_VAL, (1LL << 62));
+  std::vector<hipEvent_t> ev(4 * static_cast<size_t>(trials), nullptr);
+  bool good = true;
+  for (auto& e : ev) good = good && hipEventCreate(&e) == hipSuccess;
+  for (int k = 0; good && k < trials; ++k) {
+    hipEvent_t* e = &ev[4 * static_cast<size_t>(k)];
+    (void)hipEventRecord(e[0], s->stream);
+    hipLaunchKernelGGL(k_stage<1>, dim3(s->launch_blocks), dim3(kThreads), 0, s->stream, g, nullptr, nullptr);
+    (void)hipEventRecord(e[1], s->stream);
+    hipLaunchKernelGGL(k_stage<2>, dim3(s->launch_blocks), dim3(kThreads), 0, s->stream, g, nullptr, nullptr);
+    (void)hipEventRecord(e[2], s->stream);
+    hipLaunchKernelGGL(k_stage<3>, dim3(s->launch_blocks), dim3(kThreads), 0, s->stream, g, nullptr, nullptr);
+    (void)hipEventRecord(e[3], s->stream);
+    hipLaunchKernelGGL(k_commit, dim3(1), dim3(kLanes), 0, s->stream, g);
+  }
+  good = good && ok(hipGetLastError(), err, err_len, "profile launch") &&
+         ok(hipStreamSynchronize(s->stream), err, err_len, "profile");
+  double sum[3] = {0, 0, 0};
+  for (int k = 0; good && k < trials; ++k)
+    for (int q = 0; q < 3; ++q) {
+      float ms = 0;
+      good = good && ok(hipEventElapsedTime(&ms, ev[4 * static_cast<size_t>(k) + q], ev[4 * static_cast<size_t>(k) + q + 1]),
+                        err, err_len, "profile events");
+      sum[q] += ms;
+    }
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (!good) return 3;
+  for (int q = 0; q < 3; ++q) stage_us[q] = 1e3 * sum[q] / trials;
   return 0;
 }
 
