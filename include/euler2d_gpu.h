@@ -12,10 +12,11 @@
 //   solver2d_run_cuda ........ replaces src/euler2d.h:211 (src/euler2d_cuda.cu:530-708)
 //   solver2d_cuda_available .. replaces src/euler2d.h:213 (src/euler2d_cuda.cu:510-528)
 // solver2d_run_cuda returns 0 (every case at tend), 1 (maxsteps steps), 2 (nonphysical
-// candidate after 12 retries), 3 (unavailable, unsupported or invalid); it reads the grid,
-// parameters, U, t, dt and mco, and writes back U (ghost cells as the reference's device array
-// ends), t, dt, nsteps and nreject.  Supported as by the reference's backend: FP64, first-order
-// limiter, SSP-RK3, regular grids; backward Euler, cut cells and the IB path return 3.
+// candidate after 12 retries), 3 (unavailable, unsupported or invalid), 4 (backward-Euler step
+// not converged); it reads the grid, parameters, U, t, dt and mco, and writes back U (ghost cells
+// as the reference's device array ends), t, dt, nsteps, nreject and the implicit statistics.
+// Supported as by the reference's backend: FP64, first-order limiter, SSP-RK3 and backward
+// Euler on regular grids; cut cells and the IB path return 3.
 #ifndef HIPAMR_EULER2D_GPU_H
 #define HIPAMR_EULER2D_GPU_H
 

@@ -12,7 +12,11 @@
  *                   per case, SSP-RK3; a candidate with a nonphysical cell in any case halves
  *                   every case's step and is retried (at most 12 times).  Returns 0 (every case
  *                   at tend), 1 (maxsteps steps taken in this call), 2 (nonphysical candidate
- *                   after 12 retries; err holds the message), 3 (error).
+ *                   after 12 retries; err holds the message), 3 (error).  With time_integrator
+ *                   1, backward Euler by pseudo-time defect relaxation (src/euler2d_cuda.cu:
+ *                   370-447): 2 also for an invalid initial state or a non-finite defect, 3 for
+ *                   invalid controls or a physical dt that cannot advance, 4 when a step does
+ *                   not converge within implicit_maxiter updates.
  *   ugrid_rhs       the residual at a state, boundary ghosts filled from its interior first
  *                   (solver2d_apply_bc + solver2d_residual, src/euler2d.cpp:249-489, as the GPU
  *                   kernels of src/euler2d_cuda.cu:153-258 compute it).
@@ -52,7 +56,7 @@ typedef struct ugrid_params {
   double ausm_mco;          /* default reference Mach number of the cases */
   int32_t bc[4];            /* xmin, xmax, ymin, ymax: 0 periodic, 1 transmissive, 2 reflect */
   double dt_fixed;          /* > 0 fixed physical dt, else CFL */
-  int32_t time_integrator;  /* 0 SSP-RK3 */
+  int32_t time_integrator;  /* 0 SSP-RK3, 1 backward Euler */
   int32_t implicit_maxiter;
   double pseudo_cfl, implicit_rtol, implicit_atol;
 } ugrid_params;
@@ -86,6 +90,17 @@ void ugrid_destroy(ugrid_session* s);
 /* 1 when a HIP device is usable (reason: its name and architecture), else 0 (reason: why);
  * the check of solver2d_cuda_available (src/euler2d_cuda.cu:510-528). */
 int ugrid_device_available(char* reason, size_t reason_len);
+
+/* Backward Euler (time_integrator 1; src/euler2d_cuda.cu:370-447): pseudo-time updates and
+ * candidate retries since create, convergence checks, the latest check's largest first norm,
+ * norm and tolerance over the cases (ImplicitStats2D, src/euler2d.h:118-121), and why the last
+ * run stopped (0 none, 1 invalid initial state, 2 candidate retries, 3 non-finite defect,
+ * 4 iteration limit, 5 physical dt). */
+typedef struct ugrid_implicit_info {
+  int64_t updates, retries, checks, reason;
+  double initial_defect, final_defect, tolerance;
+} ugrid_implicit_info;
+int ugrid_implicit_stats(const ugrid_session* s, ugrid_implicit_info* out);
 
 /* Sizes and field offsets of the Solver2D view behind cfd::solver2d_run_cuda (euler2d_gpu.h):
  * Solver2D, Grid2D, Array, Params2D, ImplicitStats2D, then grid, grid.{nx, ng, mx, my, xmin,

@@ -18,13 +18,22 @@
 //                                stages with a check after each, retry halving with the
 //                                "hdt > 0" / "hdt < 0 or not finite" tests, accept copy of the
 //                                interior, maxsteps counted per call)
+//   backward Euler ............. src/euler2d_cuda.cu:370-447 with src/implicit2d.h:16-63 and
+//                                src/implicit2d.cpp:11-69 (controls check, initial state check,
+//                                physical dt from the spectral bound with the round-off rule,
+//                                pseudo-time defect relaxation: residual, scaled defect norm per
+//                                case, tolerance max(atol, rtol * first norm), local relaxation
+//                                factor, candidate check with pseudo-CFL halving (12 retries),
+//                                interior commit of every case, time advance that lands on tend)
 // The reference's CPU solver (src/euler2d.cpp:249-698) differs from this backend only in
 // rounding (pressure and energy groupings, multiplication by 1 / dx) and in the retry test
 // for a step that underflows to zero; the tests pin this restatement against it within
 // round-off, bit for bit on the boundary fill, and on closed forms.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 #include "ugrid_hip.h"
@@ -340,9 +349,192 @@ int run(const ugrid_setup& s, double* U, double* t, double* dt_out, double tend,
   return 1;
 }
 
+// ---------------------------------------------------------------- backward Euler --
+struct Implicit {
+  int64_t updates = 0, retries = 0, checks = 0;
+  double initial_defect = 0, final_defect = 0, tolerance = 0;
+  int reason = 0;  // 1 initial state, 2 pseudo retries, 3 non-finite defect, 4 not converged, 5 dt
+};
+
+bool implicit_controls_valid(const ugrid_setup& s, const double* t, const std::vector<double>& mco, double tend,
+                             int64_t maxsteps) {
+  const ugrid_params& p = s.params;
+  if (!std::isfinite(tend) || maxsteps < 0 || !(p.gamma > 1.0) || !std::isfinite(p.gamma) || !(p.cfl > 0.0) ||
+      !std::isfinite(p.cfl) || !(p.dt_fixed >= 0.0) || !std::isfinite(p.dt_fixed) || !(p.pseudo_cfl > 0.0) ||
+      !std::isfinite(p.pseudo_cfl) || p.implicit_maxiter < 1 || !(p.implicit_rtol >= 0.0) ||
+      !std::isfinite(p.implicit_rtol) || !(p.implicit_atol > 0.0) || !std::isfinite(p.implicit_atol))
+    return false;
+  for (int c = 0; c < s.ncase; ++c)
+    if (!std::isfinite(t[c]) || !(mco[c] > 0.0) || !std::isfinite(mco[c])) return false;
+  return true;
+}
+
+// Largest bound of case c on the bit patterns (the spectral rate of a regular cell is the
+// wave-speed bound of the explicit step).
+double rate_max(const ugrid_setup& s, const double* U, int c) { return bound_max(s, U, c); }
+
+int run_implicit(const ugrid_setup& s, double* U, double* t, double* dt, double tend, int64_t maxsteps,
+                 int64_t* nsteps, Implicit* st) {
+  const Shape g(s);
+  const ugrid_params& par = s.params;
+  const double gm = par.gamma;
+  std::vector<double> mco(g.ncase, par.ausm_mco);
+  if (s.mco)
+    for (int c = 0; c < g.ncase; ++c) mco[c] = s.mco[c];
+  if (!implicit_controls_valid(s, t, mco, tend, maxsteps)) return 3;
+  if (any_bad(s, U)) {
+    st->reason = 1;
+    return 2;
+  }
+  const size_t n = static_cast<size_t>(g.ncase) * 4 * g.plane;
+  std::vector<double> Qa(n), Tb(n), R(n, 0.0), norms(g.ncase), initial(g.ncase);
+  for (int64_t step = 0;; ++step) {
+    bool done = true;
+    for (int c = 0; c < g.ncase; ++c)
+      if (t[c] < tend) done = false;
+    if (done) return 0;
+    if (step >= maxsteps) return 1;
+    for (int c = 0; c < g.ncase; ++c) {
+      const double rate = rate_max(s, U, c);
+      const double remain = tend - t[c];
+      if (remain <= 0.0) {
+        dt[c] = 0.0;
+        continue;
+      }
+      double d = par.dt_fixed > 0.0 ? par.dt_fixed : par.cfl / rate;
+      if (!(d > 0.0) || !std::isfinite(d)) {
+        st->reason = 5;
+        return 3;
+      }
+      const double roundoff = 8 * std::numeric_limits<double>::epsilon() * std::fmax(std::fabs(tend), std::fabs(t[c]));
+      if (d >= remain || remain - d <= roundoff) d = remain;
+      dt[c] = d;
+      if (!(t[c] + d > t[c])) {
+        st->reason = 5;
+        return 3;
+      }
+    }
+    std::copy(U, U + n, Qa.begin());
+    std::copy(U, U + n, Tb.begin());
+    double* Q = Qa.data();
+    double* trial = Tb.data();
+    double pseudo = par.pseudo_cfl;
+    for (int iter = 0;; ++iter) {
+      apply_bc(s, Q);
+      residual(s, mco, Q, R.data());
+      for (int c = 0; c < g.ncase; ++c) {
+        uint64_t best = 0;
+        if (dt[c] > 0.0) {
+          for (int j = 0; j < g.ny; ++j)
+            for (int i = 0; i < g.nx; ++i) {
+              double norm = 0.0;
+              for (int v = 0; v < 4; ++v) {
+                const size_t k = g.at(c, v, i, j);
+                const double defect = dt[c] * R[k] - (Q[k] - U[k]);
+                if (!std::isfinite(defect)) {
+                  norm = std::numeric_limits<double>::infinity();
+                  break;
+                }
+                norm = std::fmax(norm, std::fabs(defect) / std::fmax(1.0, std::fabs(U[k])));
+              }
+              uint64_t bits;
+              std::memcpy(&bits, &norm, 8);
+              if (bits > best) best = bits;
+            }
+        }
+        std::memcpy(&norms[c], &best, 8);
+      }
+      for (int c = 0; c < g.ncase; ++c)
+        if (!std::isfinite(norms[c])) {
+          st->reason = 3;
+          return 2;
+        }
+      // implicit2d_converged
+      bool converged = true;
+      st->initial_defect = st->final_defect = st->tolerance = 0.0;
+      for (int c = 0; c < g.ncase; ++c) {
+        if (iter == 0) initial[c] = norms[c];
+        const double tol = std::max(par.implicit_atol, par.implicit_rtol * initial[c]);
+        st->initial_defect = std::max(st->initial_defect, initial[c]);
+        st->final_defect = std::max(st->final_defect, norms[c]);
+        st->tolerance = std::max(st->tolerance, tol);
+        if (!std::isfinite(norms[c]) || !std::isfinite(tol) || norms[c] > tol) converged = false;
+      }
+      ++st->checks;
+      if (converged) break;
+      if (iter >= par.implicit_maxiter) {
+        st->reason = 4;
+        return 4;
+      }
+      int retries = 0;
+      for (;;) {
+        for (int c = 0; c < g.ncase; ++c) {
+          if (dt[c] <= 0.0) continue;
+          for (int j = 0; j < g.ny; ++j)
+            for (int i = 0; i < g.nx; ++i) {
+              const size_t k = g.at(c, 0, i, j);
+              const double r = Q[k], mu = Q[k + g.plane], mv = Q[k + 2 * g.plane], E = Q[k + 3 * g.plane];
+              const double u = mu / r, v = mv / r;
+              const double p = (gm - 1.0) * (E - 0.5 * (mu * mu + mv * mv) / r);
+              const double a = std::sqrt(gm * p / r);
+              const double rate = (std::fabs(u) + a) / s.dx[i + NG] + (std::fabs(v) + a) / s.dy[j + NG];
+              const double omega = pseudo / (pseudo + dt[c] * rate);
+              for (int w = 0; w < 4; ++w) {
+                const size_t q = k + w * g.plane;
+                trial[q] = Q[q] + omega * (dt[c] * R[q] - (Q[q] - U[q]));
+              }
+            }
+        }
+        if (!any_bad(s, trial)) break;
+        ++st->retries;
+        if (++retries > 12) {
+          st->reason = 2;
+          return 2;
+        }
+        pseudo *= 0.5;
+      }
+      std::swap(Q, trial);
+      ++st->updates;
+    }
+    for (int c = 0; c < g.ncase; ++c)
+      for (int v = 0; v < 4; ++v)
+        for (int j = 0; j < g.ny; ++j)
+          for (int i = 0; i < g.nx; ++i) U[g.at(c, v, i, j)] = Q[g.at(c, v, i, j)];
+    for (int c = 0; c < g.ncase; ++c) {
+      if (dt[c] <= 0.0) continue;
+      t[c] = dt[c] == tend - t[c] ? tend : t[c] + dt[c];
+    }
+    ++*nsteps;
+  }
+}
+
 }  // namespace grid_oracle
 
 extern "C" {
+
+// Backward Euler as the reference's GPU backend runs it.  stats: updates, retries, checks
+// (int64), then initial_defect, final_defect, tolerance (doubles), reason (int64); all in/out,
+// counters cumulative.  Returns 0, 1, 2, 3 or 4.
+int grid_oracle_run_implicit(const ugrid_setup* s, double* u, double* t, double* dt, double tend, int64_t maxsteps,
+                             int64_t* nsteps, int64_t* counts, double* defects) {
+  grid_oracle::Implicit st;
+  st.updates = counts[0];
+  st.retries = counts[1];
+  st.checks = counts[2];
+  st.initial_defect = defects[0];
+  st.final_defect = defects[1];
+  st.tolerance = defects[2];
+  const int status = grid_oracle::run_implicit(*s, u, t, dt, tend, maxsteps, nsteps, &st);
+  counts[0] = st.updates;
+  counts[1] = st.retries;
+  counts[2] = st.checks;
+  counts[3] = st.reason;
+  defects[0] = st.initial_defect;
+  defects[1] = st.final_defect;
+  defects[2] = st.tolerance;
+  return status;
+}
+
 
 void grid_oracle_apply_bc(const ugrid_setup* s, double* u) { grid_oracle::apply_bc(*s, u); }
 

@@ -29,6 +29,9 @@ def _oracle():
         lib.grid_oracle_run.restype = C.c_int
         lib.grid_oracle_run.argtypes = [S, N.c_double_p, N.c_double_p, N.c_double_p, C.c_double, C.c_int64,
                                         C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        lib.grid_oracle_run_implicit.restype = C.c_int
+        lib.grid_oracle_run_implicit.argtypes = [S, N.c_double_p, N.c_double_p, N.c_double_p, C.c_double, C.c_int64,
+                                                 C.POINTER(C.c_int64), C.POINTER(C.c_int64), N.c_double_p]
         lib.grid_oracle_flux.restype = None
         lib.grid_oracle_flux.argtypes = [C.c_int, N.c_double_p, N.c_double_p, C.c_double, C.c_double, N.c_double_p]
         _bound.add("oracle")
@@ -50,6 +53,9 @@ def _ref():
         lib.ref_grid_run.restype = C.c_int
         lib.ref_grid_run.argtypes = [C.c_int, C.c_int, D, C.c_int, P, I, D, D, D, D, D, C.c_double, C.c_int64,
                                      C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        lib.ref_grid_run_implicit.restype = C.c_int
+        lib.ref_grid_run_implicit.argtypes = [C.c_int, C.c_int, D, C.c_int, P, I, D, D, D, D, D, C.c_double,
+                                              C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64), D]
         lib.ref_flux.restype = None
         lib.ref_flux.argtypes = [C.c_int, D, D, C.c_double, C.c_double, D]
         _bound.add("ref")
@@ -86,6 +92,22 @@ def run(s, tend: float, maxsteps: int) -> dict:
     status = _oracle().grid_oracle_run(C.byref(setup), N.dptr(u), N.dptr(t), N.dptr(dt), tend, maxsteps,
                                        C.byref(nsteps), C.byref(nreject))
     return dict(u=u, t=t, dt=dt, status=status, nsteps=nsteps.value, nreject=nreject.value)
+
+
+def run_implicit(s, tend: float, maxsteps: int) -> dict:
+    """The oracle's backward-Euler run from s's current U, t, dt (s is not modified)."""
+    u = np.array(s.U, dtype=np.float64, copy=True)
+    t = np.array(s.t, dtype=np.float64, copy=True)
+    dt = np.array(s.dt, dtype=np.float64, copy=True)
+    nsteps = C.c_int64(0)
+    counts = (C.c_int64 * 4)(0, 0, 0, 0)
+    defects = np.zeros(3)
+    setup, keep = s._setup()
+    status = _oracle().grid_oracle_run_implicit(C.byref(setup), N.dptr(u), N.dptr(t), N.dptr(dt), tend, maxsteps,
+                                                C.byref(nsteps), counts, N.dptr(defects))
+    return dict(u=u, t=t, dt=dt, status=status, nsteps=nsteps.value, updates=counts[0], retries=counts[1],
+                checks=counts[2], reason=counts[3], initial_defect=defects[0], final_defect=defects[1],
+                tolerance=defects[2])
 
 
 def flux(kind: int, left, right, gamma: float = 1.4, mco: float = 1.0) -> np.ndarray:
@@ -153,6 +175,25 @@ def ref_run(s, tend: float, maxsteps: int) -> dict:
                                  N.dptr(mco), N.dptr(u), N.dptr(t), N.dptr(dt), tend, maxsteps, C.byref(nsteps),
                                  C.byref(nreject))
     return dict(u=u, t=t, dt=dt, status=status, nsteps=nsteps.value, nreject=nreject.value)
+
+
+def ref_run_implicit(s, tend: float, maxsteps: int) -> dict:
+    """The reference's CPU backward-Euler driver from s's current state (s is not modified)."""
+    u = np.array(s.U, dtype=np.float64, copy=True)
+    t = np.array(s.t, dtype=np.float64, copy=True)
+    dt = np.array(s.dt, dtype=np.float64, copy=True)
+    types, params = _stretch(s)
+    box = _box(s)
+    p = s.par.to_c()
+    mco = np.ascontiguousarray(s.mco, dtype=np.float64)
+    nsteps = C.c_int64(0)
+    counts = (C.c_int64 * 2)(0, 0)
+    defects = np.zeros(3)
+    status = _ref().ref_grid_run_implicit(s.grid.nx, s.grid.ny, N.dptr(box), s.ncase, C.byref(p), types,
+                                          N.dptr(params), N.dptr(mco), N.dptr(u), N.dptr(t), N.dptr(dt), tend,
+                                          maxsteps, C.byref(nsteps), counts, N.dptr(defects))
+    return dict(u=u, t=t, dt=dt, status=status, nsteps=nsteps.value, updates=counts[0], retries=counts[1],
+                initial_defect=defects[0], final_defect=defects[1], tolerance=defects[2])
 
 
 def ref_flux(kind: int, left, right, gamma: float = 1.4, mco: float = 1.0) -> np.ndarray:

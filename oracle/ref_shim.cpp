@@ -340,6 +340,35 @@ int ref_grid_run(int nx, int ny, const double* box, int ncase, const ugrid_param
   return status;
 }
 
+// solver2d_run_implicit (src/implicit2d.cpp:104-207), the CPU backward-Euler driver.  counts:
+// updates, retries (in/out); defects: initial, final, tolerance (out).
+int ref_grid_run_implicit(int nx, int ny, const double* box, int ncase, const ugrid_params* p, const int* stretch_type,
+                          const double* stretch_param, const double* mco, double* u, double* t, double* dt,
+                          double tend, int64_t maxsteps, int64_t* nsteps, int64_t* counts, double* defects) {
+  auto s = ref_grid_solver(nx, ny, box, ncase, p, stretch_type, stretch_param);
+  for (int c = 0; c < ncase; ++c) {
+    if (mco) solver2d_set_mco(s.get(), c, mco[c]);
+    s->t[c] = t[c];
+    s->dt[c] = dt[c];
+  }
+  std::memcpy(s->U.data(), u, s->U.size() * sizeof(double));
+  s->implicit.updates = counts[0];
+  s->implicit.retries = counts[1];
+  const int status = solver2d_run_implicit(s.get(), tend, static_cast<long>(maxsteps));
+  std::memcpy(u, s->U.data(), s->U.size() * sizeof(double));
+  for (int c = 0; c < ncase; ++c) {
+    t[c] = s->t[c];
+    dt[c] = s->dt[c];
+  }
+  *nsteps = s->nsteps;
+  counts[0] = s->implicit.updates;
+  counts[1] = s->implicit.retries;
+  defects[0] = s->implicit.initial_defect;
+  defects[1] = s->implicit.final_defect;
+  defects[2] = s->implicit.tolerance;
+  return status;
+}
+
 // The same list as ugrid_solver2d_layout (include/ugrid_hip.h), measured on the reference's types.
 int ref_solver2d_layout(int64_t* out, int32_t capacity) {
   const int64_t v[] = {

@@ -284,6 +284,11 @@ class ugrid_meta(C.Structure):
     _fields_ = [("nsteps", C.c_int64), ("nreject", C.c_int64)]
 
 
+class ugrid_implicit_info(C.Structure):
+    _fields_ = [("updates", C.c_int64), ("retries", C.c_int64), ("checks", C.c_int64), ("reason", C.c_int64),
+                ("initial_defect", C.c_double), ("final_defect", C.c_double), ("tolerance", C.c_double)]
+
+
 class ugrid_launch_info(C.Structure):
     _fields_ = [("rows", C.c_int32), ("strips", C.c_int32), ("blocks", C.c_int32), ("threads", C.c_int32)]
 
@@ -302,6 +307,7 @@ def _bind_ugrid_api(lib):
         "ugrid_get_launch": (C.c_int, [H, C.POINTER(ugrid_launch_info)]),
         "ugrid_device_available": (C.c_int, [C.c_char_p, C.c_size_t]),
         "ugrid_profile": (C.c_int, [H, C.c_int32, c_double_p, C.c_char_p, C.c_size_t]),
+        "ugrid_implicit_stats": (C.c_int, [H, C.POINTER(ugrid_implicit_info)]),
         "ugrid_solver2d_layout": (C.c_int, [C.POINTER(C.c_int64), C.c_int32]),
     }
     for name, (restype, argtypes) in sig.items():

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -49,6 +50,11 @@ constexpr int kXcds = 8;
 constexpr int kMaxRetries = 12;
 enum : int { BC_PERIODIC = 0, BC_TRANSMISSIVE = 1, BC_REFLECT = 2 };
 enum : int { RUNNING = -1 };
+// Backward-Euler phases of a physical step (each kernel of the replayed sequence acts only in its
+// phase; the one-wave decision kernels move the phase).
+enum : int { PH_BOUND = 0, PH_COPY = 1, PH_RES = 2, PH_TRIAL = 3 };
+// Why a backward-Euler run stopped with 2, 3 or 4.
+enum : int { WHY_NONE = 0, WHY_INITIAL = 1, WHY_PSEUDO = 2, WHY_DEFECT = 3, WHY_MAXITER = 4, WHY_DT = 5 };
 
 // Step control of a batch (one record, device memory).
 struct Control {
@@ -56,12 +62,17 @@ struct Control {
   long long maxsteps;  // steps allowed in the current run call
   long long steps;     // accepted in the current run call
   long long nsteps, nreject;
-  int status;      // RUNNING, or the run's result 0 / 1 / 2 (also 0 before the first run)
+  long long events;    // decisions taken (every commit, rejection, phase change): progress
+  int status;      // RUNNING, or the run's result 0 / 1 / 2 / 3 / 4 (also 0 before the first run)
   int bad;         // a stage found a nonphysical candidate cell (this trial)
   int retries;     // rejections of the current step
-  int cur;         // U[cur] holds the accepted state
-  int ghost_from;  // download ghosts: 0 as uploaded, 1 boundary fill of U[cur], 2 of U[cur ^ 1]
-  int pad;
+  int cur;         // B[cur] holds the accepted state
+  int ghost_from;  // download ghosts: 0 as uploaded, 1 boundary fill of B[cur], 2 of B[cur ^ 1]
+  // backward Euler
+  int phase, iter, pretries, qbuf, reason, pad;
+  double pseudo;   // pseudo CFL of the current physical step
+  long long updates, iretries, checks;
+  double initial_defect, final_defect, tolerance;
 };
 
 struct Grid {
@@ -76,12 +87,18 @@ struct Grid {
   const double* dx;    // mx widths (ghosts included)
   const double* dy;    // my
   const double* mco;   // ncase
-  double* U[2];
-  double* S2;          // second-stage candidate
+  // State buffers: B[0] and B[1] alternate as accepted state and RK candidate, B[2] holds the
+  // second RK stage; backward Euler rotates all three (accepted state, iterate, candidate).
+  double* B[3];
+  double* R;           // residual of the backward-Euler iterate
   double* t;
   double* dt;          // step of the running trial
   double* dt_out;      // last accepted step
   unsigned long long* smax;  // per case: bit pattern of the largest wave-speed bound (non-negative)
+  unsigned long long* norm;  // per case: bit pattern of the largest scaled defect (backward Euler)
+  double* initial;           // per case: first defect norm of the physical step
+  double pseudo0, rtol, atol;
+  int maxiter;
   Control* ctrl;
 };
 
@@ -289,6 +306,7 @@ __device__ void commit(const Grid& g) {
       c->nreject += 1;
       if (retries > kMaxRetries || unusable) c->status = 2;
       c->bad = 0;
+      c->events += 1;
     }
     return;
   }
@@ -320,30 +338,41 @@ __device__ void commit(const Grid& g) {
     c->retries = 0;
     c->ghost_from = next ? 1 : 2;
     if (!next) c->status = maxed ? 1 : 0;
+    c->events += 1;
   }
 }
 
 // ---------------------------------------------------------------------------------------------
-// One RK stage (MODE 1..3) or the residual alone (MODE 0), fused over a wave strip.
+// One RK stage (MODE 1..3), the residual alone (MODE 0) or the backward-Euler residual with the
+// defect norms (MODE 4), fused over a wave strip.
 //   MODE 1: U[cur^1] = U + dt L(U)                    (in U[cur])
 //   MODE 2: S2 = 3/4 U + 1/4 (U1 + dt L(U1))          (U1 in U[cur^1])
 //   MODE 3: U[cur^1] = 1/3 U + 2/3 (S2 + dt L(S2)), and the candidate's wave-speed bound
 //   MODE 0: R = L(in)
+//   MODE 4: R = L(Q), per case max over cells of max_v |dt R - (Q - U)| / max(1, |U|)
+//           (infinite for a non-finite defect), src/euler2d_cuda.cu:331-350
 // ---------------------------------------------------------------------------------------------
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void k_stage(Grid g, const double* __restrict__ rhs_in,
                                                      double* __restrict__ rhs_out) {
   Control* ctrl = g.ctrl;
   bool skip = false;
-  int cur = 0;
+  int cur = 0, qb = 0;
   if (MODE > 0) {
     if (__hip_atomic_load(&ctrl->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != RUNNING) return;
-    skip = __hip_atomic_load(&ctrl->bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    if (MODE == 4) {
+      const int phase = ctrl->phase;
+      if (phase != PH_COPY && phase != PH_RES) return;
+      qb = ctrl->qbuf;
+    } else {
+      skip = __hip_atomic_load(&ctrl->bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+    }
     cur = ctrl->cur;
   }
-  const double* base = MODE > 0 ? g.U[cur] : nullptr;
-  const double* in = MODE == 0 ? rhs_in : MODE == 1 ? g.U[cur] : MODE == 2 ? g.U[cur ^ 1] : g.S2;
-  double* out = MODE == 0 ? rhs_out : MODE == 2 ? g.S2 : g.U[cur ^ 1];
+  const double* base = MODE > 0 ? g.B[cur] : nullptr;
+  const double* in = MODE == 0 ? rhs_in : MODE == 1 ? g.B[cur] : MODE == 2 ? g.B[cur ^ 1] : MODE == 3 ? g.B[2]
+                                                                                                    : g.B[qb];
+  double* out = MODE == 0 ? rhs_out : MODE == 2 ? g.B[2] : MODE == 4 ? g.R : g.B[cur ^ 1];
   constexpr double A = MODE == 2 ? 0.75 : MODE == 3 ? 1.0 / 3.0 : 0.0;
   constexpr double B = MODE == 2 ? 0.25 : MODE == 3 ? 2.0 / 3.0 : 1.0;
 
@@ -390,11 +419,26 @@ __global__ __launch_bounds__(kThreads) void k_stage(Grid g, const double* __rest
           const double r2 = xres.y - (gf.y - gprev.y) / dyj;
           const double r3 = xres.e - (gf.e - gprev.e) / dyj;
           const size_t ko = case_base + static_cast<size_t>(j - 1 + kGhosts) * g.mx;
-          if (MODE == 0) {
+          if (MODE == 0 || MODE == 4) {
             out[ko] = r0;
             out[ko + plane] = r1;
             out[ko + 2 * plane] = r2;
             out[ko + 3 * plane] = r3;
+            if (MODE == 4 && dt > 0.0) {
+              const double r[4] = {r0, r1, r2, r3}, q[4] = {q0, q1, q2, q3};
+              double norm = 0.0;
+              for (int v = 0; v < 4; ++v) {
+                const double old = base[ko + v * plane];
+                const double defect = dt * r[v] - (q[v] - old);
+                if (!isfinite(defect)) {
+                  norm = HUGE_VAL;
+                  break;
+                }
+                norm = fmax(norm, fabs(defect) / fmax(1.0, fabs(old)));
+              }
+              const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(norm));
+              bound = bits > bound ? bits : bound;
+            }
           } else {
             double a0 = q0, a1 = q1, a2 = q2, a3 = q3;
             if (MODE > 1) {
@@ -438,6 +482,11 @@ __global__ __launch_bounds__(kThreads) void k_stage(Grid g, const double* __rest
     }
   }
   if (MODE == 0) return;
+  if (MODE == 4) {
+    bound = wave_max(bound);
+    if (work && lane == 0 && bound) atomicMax(&g.norm[icase], bound);
+    return;
+  }
   if (__any(wave_bad) && lane == 0) atomicOr(&ctrl->bad, 1);
   if (MODE == 3) {
     bound = wave_max(bound);
@@ -456,10 +505,12 @@ __global__ __launch_bounds__(kLanes) void k_commit(Grid g) {
 // Wave-speed bound of every case at U[cur] (the first step of a run, src/euler2d_cuda.cu:223-237).
 // Block b serves case b / per_case; each block reduces its cells to one maximum (on the bit
 // patterns, as the reference's atomicMax does) and issues one atomic.
-__global__ __launch_bounds__(kThreads) void k_bound(Grid g, int per_case) {
+// gate >= 0: only while a run is in that backward-Euler phase.
+__global__ __launch_bounds__(kThreads) void k_bound(Grid g, int per_case, int gate) {
+  if (gate >= 0 && (g.ctrl->status != RUNNING || g.ctrl->phase != gate)) return;
   const int icase = blockIdx.x / per_case;
   const int part = blockIdx.x - icase * per_case;
-  const double* u = g.U[g.ctrl->cur] + static_cast<size_t>(icase) * 4 * g.plane;
+  const double* u = g.B[g.ctrl->cur] + static_cast<size_t>(icase) * 4 * g.plane;
   const size_t inner = static_cast<size_t>(g.nx) * g.ny;
   unsigned long long best = 0;
   for (size_t q = part * static_cast<size_t>(blockDim.x) + threadIdx.x; q < inner;
@@ -515,6 +566,261 @@ __global__ __launch_bounds__(kThreads) void k_open(Grid g, double tend, long lon
   }
 }
 
+// ------------------------------------------------------------------ backward Euler --
+// src/euler2d_cuda.cu:370-447 with src/implicit2d.h:16-63 and src/implicit2d.cpp:37-69.  A replay
+// runs k_bound, k_be_open, k_be_copy, k_stage<4>, k_be_decide, k_be_trial, k_be_check; each acts
+// only in its phase, so one replay completes at least one phase.
+
+// Nonphysical interior cells of B[cur] (the run's initial check).  Grid-stride over all cases.
+__global__ __launch_bounds__(kThreads) void k_check_state(Grid g) {
+  const double* u = g.B[g.ctrl->cur];
+  const size_t inner = static_cast<size_t>(g.nx) * g.ny;
+  bool bad = false;
+  for (size_t q = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; q < inner * g.ncase;
+       q += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const size_t c = q / inner, ij = q - c * inner;
+    const int j = static_cast<int>(ij / g.nx), i = static_cast<int>(ij - static_cast<size_t>(j) * g.nx);
+    const size_t k = c * 4 * g.plane + static_cast<size_t>(j + kGhosts) * g.mx + i + kGhosts;
+    if (inadmissible(u[k], u[k + g.plane], u[k + 2 * g.plane], u[k + 3 * g.plane], g.gamma)) bad = true;
+  }
+  if (__any(bad) && threadIdx.x % kLanes == 0) atomicOr(&g.ctrl->bad, 1);
+}
+
+__global__ __launch_bounds__(kLanes) void k_be_start(Grid g, double tend, long long maxsteps) {
+  Control* c = g.ctrl;
+  if (threadIdx.x != 0) return;
+  c->tend = tend;
+  c->maxsteps = maxsteps;
+  c->steps = 0;
+  c->reason = WHY_NONE;
+  if (c->bad) {
+    c->status = 2;
+    c->reason = WHY_INITIAL;
+  } else {
+    c->status = RUNNING;
+    c->phase = PH_BOUND;
+  }
+  c->bad = 0;
+  c->events += 1;
+}
+
+// Opening of a physical step: done / maxsteps tests, then every case's dt from its spectral
+// bound (implicit2d_physical_dt: fixed or CFL step, cut to the remaining time when within
+// 8 eps of it, and it must move the clock).  Cases are processed in order: those before the
+// first failing one keep their new step.
+__global__ __launch_bounds__(kLanes) void k_be_open(Grid g) {
+  Control* c = g.ctrl;
+  if (c->status != RUNNING || c->phase != PH_BOUND) return;
+  const int lane = threadIdx.x;
+  const double tend = c->tend;
+  int open = 0;
+  for (int k = lane; k < g.ncase; k += kLanes)
+    if (g.t[k] < tend) open = 1;
+  const bool any_open = __any(open);
+  const bool maxed = c->steps >= c->maxsteps;
+  int first_fail = g.ncase;
+  if (any_open && !maxed) {
+    for (int k = lane; k < g.ncase; k += kLanes) {
+      const double t = g.t[k];
+      const double remain = tend - t;
+      if (remain <= 0.0) continue;
+      double d = g.dt_fixed > 0.0 ? g.dt_fixed : g.cfl / __longlong_as_double(static_cast<long long>(g.smax[k]));
+      if (!(d > 0.0) || !isfinite(d)) {
+        first_fail = min(first_fail, k);
+        continue;
+      }
+      const double roundoff = 8 * DBL_EPSILON * fmax(fabs(tend), fabs(t));
+      if (d >= remain || remain - d <= roundoff) d = remain;
+      if (!(t + d > t)) first_fail = min(first_fail, k);
+    }
+    for (int d = kLanes / 2; d > 0; d >>= 1) first_fail = min(first_fail, __shfl_xor(first_fail, d, kLanes));
+    for (int k = lane; k < g.ncase; k += kLanes) {
+      const double t = g.t[k];
+      const double remain = tend - t;
+      double d = 0.0;
+      bool assign = true;
+      if (remain > 0.0) {
+        d = g.dt_fixed > 0.0 ? g.dt_fixed : g.cfl / __longlong_as_double(static_cast<long long>(g.smax[k]));
+        if (!(d > 0.0) || !isfinite(d)) {
+          assign = false;
+        } else {
+          const double roundoff = 8 * DBL_EPSILON * fmax(fabs(tend), fabs(t));
+          if (d >= remain || remain - d <= roundoff) d = remain;
+        }
+      }
+      if (assign && k <= first_fail) {
+        g.dt[k] = d;
+        g.dt_out[k] = d;
+      }
+    }
+  }
+  for (int k = lane; k < g.ncase; k += kLanes) g.smax[k] = 0;
+  if (lane == 0) {
+    if (!any_open) {
+      c->status = 0;
+    } else if (maxed) {
+      c->status = 1;
+    } else if (first_fail < g.ncase) {
+      c->status = 3;
+      c->reason = WHY_DT;
+    } else {
+      c->phase = PH_COPY;
+      c->pseudo = g.pseudo0;
+      c->iter = 0;
+      c->pretries = 0;
+      c->qbuf = c->cur == 0 ? 1 : 0;
+    }
+    c->events += 1;
+  }
+}
+
+// Iterate and candidate start as copies of the accepted state (interior cells).
+__global__ __launch_bounds__(kThreads) void k_be_copy(Grid g) {
+  const Control* c = g.ctrl;
+  if (c->status != RUNNING || c->phase != PH_COPY) return;
+  const int cur = c->cur, qb = c->qbuf, tb = 3 - cur - qb;
+  const double* u = g.B[cur];
+  double* q = g.B[qb];
+  double* t = g.B[tb];
+  const size_t inner = static_cast<size_t>(g.nx) * g.ny;
+  for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < inner * 4 * g.ncase;
+       e += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const size_t cv = e / inner, ij = e - cv * inner;
+    const int j = static_cast<int>(ij / g.nx), i = static_cast<int>(ij - static_cast<size_t>(j) * g.nx);
+    const size_t k = cv * g.plane + static_cast<size_t>(j + kGhosts) * g.mx + i + kGhosts;
+    const double x = u[k];
+    q[k] = x;
+    t[k] = x;
+  }
+}
+
+// Convergence of the pseudo iteration (implicit2d_converged): tolerance max(atol, rtol * the
+// step's first norm) per case; on convergence the iterate becomes the accepted state and the
+// clocks advance (implicit2d_advance_time), else a candidate follows or the iteration limit
+// ends the run with 4.
+__global__ __launch_bounds__(kLanes) void k_be_decide(Grid g) {
+  Control* c = g.ctrl;
+  if (c->status != RUNNING || (c->phase != PH_COPY && c->phase != PH_RES)) return;
+  const int lane = threadIdx.x;
+  int nonfinite = 0;
+  for (int k = lane; k < g.ncase; k += kLanes)
+    if (!isfinite(__longlong_as_double(static_cast<long long>(g.norm[k])))) nonfinite = 1;
+  if (__any(nonfinite)) {
+    for (int k = lane; k < g.ncase; k += kLanes) g.norm[k] = 0;
+    if (lane == 0) {
+      c->status = 2;
+      c->reason = WHY_DEFECT;
+      c->events += 1;
+    }
+    return;
+  }
+  const bool first = c->iter == 0;
+  unsigned long long imax = 0, nmax = 0, tmax = 0;
+  int open = 0;
+  for (int k = lane; k < g.ncase; k += kLanes) {
+    const double norm = __longlong_as_double(static_cast<long long>(g.norm[k]));
+    if (first) g.initial[k] = norm;
+    const double initial = g.initial[k];
+    const double tol = fmax(g.atol, g.rtol * initial);
+    const unsigned long long bi = static_cast<unsigned long long>(__double_as_longlong(initial));
+    const unsigned long long bn = static_cast<unsigned long long>(__double_as_longlong(norm));
+    const unsigned long long bt = static_cast<unsigned long long>(__double_as_longlong(tol));
+    imax = bi > imax ? bi : imax;
+    nmax = bn > nmax ? bn : nmax;
+    tmax = bt > tmax ? bt : tmax;
+    if (!isfinite(tol) || norm > tol) open = 1;
+    g.norm[k] = 0;
+  }
+  imax = wave_max(imax);
+  nmax = wave_max(nmax);
+  tmax = wave_max(tmax);
+  const bool converged = !__any(open);
+  if (converged) {
+    const double tend = c->tend;
+    for (int k = lane; k < g.ncase; k += kLanes) {
+      const double d = g.dt[k];
+      if (d <= 0.0) continue;
+      const double t = g.t[k];
+      g.t[k] = d == tend - t ? tend : t + d;
+    }
+  }
+  if (lane == 0) {
+    c->initial_defect = __longlong_as_double(static_cast<long long>(imax));
+    c->final_defect = __longlong_as_double(static_cast<long long>(nmax));
+    c->tolerance = __longlong_as_double(static_cast<long long>(tmax));
+    c->checks += 1;
+    if (converged) {
+      c->cur = c->qbuf;
+      c->nsteps += 1;
+      c->steps += 1;
+      c->phase = PH_BOUND;
+    } else if (c->iter >= g.maxiter) {
+      c->status = 4;
+      c->reason = WHY_MAXITER;
+    } else {
+      c->phase = PH_TRIAL;
+      c->pretries = 0;
+    }
+    c->events += 1;
+  }
+}
+
+// Pseudo-time candidate: trial = Q + omega (dt R - (Q - U)), omega = cfl* / (cfl* + dt rate(Q))
+// for the cases that move, then the admissibility of every case's candidate cells.
+__global__ __launch_bounds__(kThreads) void k_be_trial(Grid g) {
+  const Control* c = g.ctrl;
+  if (c->status != RUNNING || c->phase != PH_TRIAL) return;
+  const int cur = c->cur, qb = c->qbuf, tb = 3 - cur - qb;
+  const double pseudo = c->pseudo;
+  const double* u = g.B[cur];
+  const double* Q = g.B[qb];
+  double* trial = g.B[tb];
+  const size_t inner = static_cast<size_t>(g.nx) * g.ny;
+  const size_t pl = g.plane;
+  bool bad = false;
+  for (size_t q = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; q < inner * g.ncase;
+       q += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const size_t cs = q / inner, ij = q - cs * inner;
+    const int j = static_cast<int>(ij / g.nx), i = static_cast<int>(ij - static_cast<size_t>(j) * g.nx);
+    const size_t k = cs * 4 * pl + static_cast<size_t>(j + kGhosts) * g.mx + i + kGhosts;
+    const double d = g.dt[cs];
+    if (d > 0.0) {
+      const double rate = wave_bound(Q[k], Q[k + pl], Q[k + 2 * pl], Q[k + 3 * pl], g, g.dx[i + kGhosts],
+                                     g.dy[j + kGhosts]);
+      const double omega = pseudo / (pseudo + d * rate);
+      for (int v = 0; v < 4; ++v) {
+        const size_t kv = k + v * pl;
+        trial[kv] = Q[kv] + omega * (d * g.R[kv] - (Q[kv] - u[kv]));
+      }
+    }
+    if (inadmissible(trial[k], trial[k + pl], trial[k + 2 * pl], trial[k + 3 * pl], g.gamma)) bad = true;
+  }
+  if (__any(bad) && threadIdx.x % kLanes == 0) atomicOr(&g.ctrl->bad, 1);
+}
+
+// Candidate accepted as the next iterate, or the pseudo CFL halved (12 retries, then 2).
+__global__ __launch_bounds__(kLanes) void k_be_check(Grid g) {
+  Control* c = g.ctrl;
+  if (threadIdx.x != 0 || c->status != RUNNING || c->phase != PH_TRIAL) return;
+  if (c->bad) {
+    c->bad = 0;
+    c->iretries += 1;
+    c->pretries += 1;
+    if (c->pretries > kMaxRetries) {
+      c->status = 2;
+      c->reason = WHY_PSEUDO;
+    } else {
+      c->pseudo *= 0.5;
+    }
+  } else {
+    c->qbuf = 3 - c->cur - c->qbuf;
+    c->updates += 1;
+    c->iter += 1;
+    c->phase = PH_RES;
+  }
+  c->events += 1;
+}
+
 __global__ void k_zero(double* p, size_t n) {
   for (size_t q = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; q < n;
        q += static_cast<size_t>(gridDim.x) * blockDim.x)
@@ -534,7 +840,9 @@ struct ugrid_session {
   ugrid_params par{};
   size_t states = 0;      // doubles per state array (ghosts included)
   int launch_blocks = 0;  // padded to a multiple of 8
-  double* R = nullptr;    // residual output of ugrid_rhs (allocated on first use)
+  double* R = nullptr;       // residual output of ugrid_rhs (allocated on first use)
+  double* rhs_in = nullptr;  // input staging of ugrid_rhs (allocated on first use)
+  std::vector<double> mco;   // host copy of the reference Mach numbers
   Control* host_ctrl = nullptr;  // pinned mirror of the control record
   hipStream_t stream = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -620,7 +928,28 @@ void launch_bound(ugrid_session* s) {
   const long long inner = static_cast<long long>(g.nx) * g.ny;
   const long long cap = (inner + kThreads - 1) / kThreads;
   const int per_case = static_cast<int>(std::max<long long>(1, std::min<long long>(cap, 2048 / g.ncase)));
-  hipLaunchKernelGGL(k_bound, dim3(per_case * g.ncase), dim3(kThreads), 0, s->stream, g, per_case);
+  hipLaunchKernelGGL(k_bound, dim3(per_case * g.ncase), dim3(kThreads), 0, s->stream, g, per_case, -1);
+}
+
+int grid_blocks(const Grid& g, size_t items) {
+  return static_cast<int>(std::min<size_t>((items + kThreads - 1) / kThreads, 8192));
+}
+
+// One backward-Euler replay: every phase kernel once, in phase order.
+int launch_be_round(ugrid_session* s, hipStream_t stream) {
+  const Grid& g = s->g;
+  const long long inner = static_cast<long long>(g.nx) * g.ny;
+  const int per_case = static_cast<int>(
+      std::max<long long>(1, std::min<long long>((inner + kThreads - 1) / kThreads, 2048 / g.ncase)));
+  const size_t cells = static_cast<size_t>(inner) * g.ncase;
+  hipLaunchKernelGGL(k_bound, dim3(per_case * g.ncase), dim3(kThreads), 0, stream, g, per_case, PH_BOUND);
+  hipLaunchKernelGGL(k_be_open, dim3(1), dim3(kLanes), 0, stream, g);
+  hipLaunchKernelGGL(k_be_copy, dim3(grid_blocks(g, 4 * cells)), dim3(kThreads), 0, stream, g);
+  hipLaunchKernelGGL(k_stage<4>, dim3(s->launch_blocks), dim3(kThreads), 0, stream, g, nullptr, nullptr);
+  hipLaunchKernelGGL(k_be_decide, dim3(1), dim3(kLanes), 0, stream, g);
+  hipLaunchKernelGGL(k_be_trial, dim3(grid_blocks(g, cells)), dim3(kThreads), 0, stream, g);
+  hipLaunchKernelGGL(k_be_check, dim3(1), dim3(kLanes), 0, stream, g);
+  return 7;
 }
 
 int launch_trial(ugrid_session* s, hipStream_t stream) {
@@ -638,7 +967,12 @@ bool build_graph(ugrid_session* s, int trials, char* err, size_t len) {
   }
   hipGraph_t graph = nullptr;
   if (!ok(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal), err, len, "begin capture")) return false;
-  for (int k = 0; k < trials; ++k) launch_trial(s, s->stream);
+  for (int k = 0; k < trials; ++k) {
+    if (s->par.time_integrator == 1)
+      launch_be_round(s, s->stream);
+    else
+      launch_trial(s, s->stream);
+  }
   const hipError_t launched = hipGetLastError();
   const hipError_t ended = hipStreamEndCapture(s->stream, &graph);
   if (!ok(launched, err, len, "trial launch") || !ok(ended, err, len, "end capture")) {
@@ -671,8 +1005,8 @@ extern "C" int ugrid_create(const ugrid_setup* setup, const double* u, uint64_t 
   const ugrid_params& p = setup->params;
   if (setup->nx < 1 || setup->ny < 1 || setup->ncase < 1)
     return fail(err, err_len, "ugrid_create: nx, ny and ncase must be positive");
-  if (p.time_integrator != 0)
-    return fail(err, err_len, "ugrid_create: only the SSP-RK3 integrator (time_integrator 0) is implemented");
+  if (p.time_integrator != 0 && p.time_integrator != 1)
+    return fail(err, err_len, "ugrid_create: time_integrator must be 0 (SSP-RK3) or 1 (backward Euler)");
   if (p.limiter != 3)
     return fail(err, err_len,
                 "ugrid_create: the device backend requires the first-order limiter (3); refusing to change the method "
@@ -701,6 +1035,10 @@ extern "C" int ugrid_create(const ugrid_setup* setup, const double* u, uint64_t 
   g.gamma = p.gamma;
   g.cfl = p.cfl;
   g.dt_fixed = p.dt_fixed;
+  g.pseudo0 = p.pseudo_cfl;
+  g.rtol = p.implicit_rtol;
+  g.atol = p.implicit_atol;
+  g.maxiter = p.implicit_maxiter;
   g.rows = choose_rows(g.nx, g.ny, g.ncase);
   g.strips_x = (g.nx + kStripCells - 1) / kStripCells;
   g.strips_y = (g.ny + g.rows - 1) / g.rows;
@@ -727,8 +1065,10 @@ extern "C" int ugrid_create(const ugrid_setup* setup, const double* u, uint64_t 
       !ok(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking), err, err_len, "hipStreamCreate") ||
       !ok(hipHostMalloc(reinterpret_cast<void**>(&s->host_ctrl), sizeof(Control), hipHostMallocDefault), err, err_len,
           "hipHostMalloc") ||
-      !device_alloc(s, &g.U[0], states, err, err_len) || !device_alloc(s, &g.U[1], states, err, err_len) ||
-      !device_alloc(s, &g.S2, states, err, err_len) || !device_alloc(s, &dxd, g.mx, err, err_len) ||
+      !device_alloc(s, &g.B[0], states, err, err_len) || !device_alloc(s, &g.B[1], states, err, err_len) ||
+      !device_alloc(s, &g.B[2], states, err, err_len) || !device_alloc(s, &dxd, g.mx, err, err_len) ||
+      (p.time_integrator == 1 && !device_alloc(s, &g.R, states, err, err_len)) ||
+      !device_alloc(s, &g.norm, g.ncase, err, err_len) || !device_alloc(s, &g.initial, g.ncase, err, err_len) ||
       !device_alloc(s, &dyd, g.my, err, err_len) || !device_alloc(s, &mcod, g.ncase, err, err_len) ||
       !device_alloc(s, &g.t, g.ncase, err, err_len) || !device_alloc(s, &g.dt, g.ncase, err, err_len) ||
       !device_alloc(s, &g.dt_out, g.ncase, err, err_len) || !device_alloc(s, &g.smax, g.ncase, err, err_len) ||
@@ -743,8 +1083,12 @@ extern "C" int ugrid_create(const ugrid_setup* setup, const double* u, uint64_t 
   c0.status = 0;
   const hipMemcpyKind h2d = hipMemcpyHostToDevice;
   const size_t bytes = states * sizeof(double);
-  if (!ok(hipMemcpyAsync(g.U[0], u, bytes, h2d, s->stream), err, err_len, "upload state") ||
-      !ok(hipMemcpyAsync(g.U[1], u, bytes, h2d, s->stream), err, err_len, "upload state") ||
+  // Every buffer that can come to hold the accepted state starts as the upload, so its ghost
+  // cells are the entry values (the device never writes ghost cells).
+  if (!ok(hipMemcpyAsync(g.B[0], u, bytes, h2d, s->stream), err, err_len, "upload state") ||
+      !ok(hipMemcpyAsync(g.B[1], u, bytes, h2d, s->stream), err, err_len, "upload state") ||
+      !ok(hipMemcpyAsync(g.B[2], u, bytes, h2d, s->stream), err, err_len, "upload state") ||
+      !ok(hipMemsetAsync(g.norm, 0, g.ncase * sizeof(unsigned long long), s->stream), err, err_len, "clear norms") ||
       !ok(hipMemcpyAsync(dxd, setup->dx, g.mx * sizeof(double), h2d, s->stream), err, err_len, "upload dx") ||
       !ok(hipMemcpyAsync(dyd, setup->dy, g.my * sizeof(double), h2d, s->stream), err, err_len, "upload dy") ||
       !ok(hipMemcpyAsync(mcod, mco.data(), g.ncase * sizeof(double), h2d, s->stream), err, err_len, "upload mco") ||
@@ -759,6 +1103,7 @@ extern "C" int ugrid_create(const ugrid_setup* setup, const double* u, uint64_t 
     return 3;
   }
   s->uploaded.assign(u, u + states);
+  s->mco = mco;
   const int trials = g.nx * static_cast<long long>(g.ny) * g.ncase >= (1 << 22) ? 4 : 16;
   if (!build_graph(s, trials, err, err_len)) {
     release(s);
@@ -772,11 +1117,33 @@ extern "C" int ugrid_run(ugrid_session* s, double tend, int64_t maxsteps, ugrid_
                          size_t err_len) {
   if (!s) return fail(err, err_len, "ugrid_run: null session");
   const Grid& g = s->g;
-  launch_bound(s);
-  hipLaunchKernelGGL(k_open, dim3(1), dim3(kThreads), 0, s->stream, g, tend, static_cast<long long>(maxsteps));
+  const bool implicit = s->par.time_integrator == 1;
+  if (implicit) {
+    // implicit2d_validate (src/implicit2d.cpp:11-35): controls, times and reference Mach numbers.
+    const ugrid_params& p = s->par;
+    std::vector<double> t(g.ncase);
+    if (!ok(hipMemcpyAsync(t.data(), g.t, g.ncase * sizeof(double), hipMemcpyDeviceToHost, s->stream), err, err_len,
+            "download t") ||
+        !ok(hipStreamSynchronize(s->stream), err, err_len, "download t"))
+      return 3;
+    bool valid = std::isfinite(tend) && maxsteps >= 0 && p.gamma > 1.0 && std::isfinite(p.gamma) && p.cfl > 0.0 &&
+                 std::isfinite(p.cfl) && p.dt_fixed >= 0.0 && std::isfinite(p.dt_fixed) && p.pseudo_cfl > 0.0 &&
+                 std::isfinite(p.pseudo_cfl) && p.implicit_maxiter >= 1 && p.implicit_rtol >= 0.0 &&
+                 std::isfinite(p.implicit_rtol) && p.implicit_atol > 0.0 && std::isfinite(p.implicit_atol);
+    for (int c = 0; c < g.ncase; ++c)
+      valid = valid && std::isfinite(t[c]) && s->mco[c] > 0.0 && std::isfinite(s->mco[c]);
+    if (!valid) return fail(err, err_len, "invalid backward-Euler controls");
+    const size_t cells = static_cast<size_t>(g.nx) * g.ny * g.ncase;
+    hipLaunchKernelGGL(k_check_state, dim3(grid_blocks(g, cells)), dim3(kThreads), 0, s->stream, g);
+    hipLaunchKernelGGL(k_be_start, dim3(1), dim3(kLanes), 0, s->stream, g, tend, static_cast<long long>(maxsteps));
+  } else {
+    launch_bound(s);
+    hipLaunchKernelGGL(k_open, dim3(1), dim3(kThreads), 0, s->stream, g, tend, static_cast<long long>(maxsteps));
+  }
   if (!ok(hipGetLastError(), err, err_len, "run start")) return 3;
-  // Every trial of a running batch ends in an accept or a rejection, so each replay of the
-  // trial graph must move nsteps + nreject; a replay that does not is reported, not waited on.
+  // Every replay of a running batch takes at least one decision (a commit, a rejection, a
+  // phase change), so it must move the event count; a replay that does not is reported, not
+  // waited on.
   long long progress = -1;
   for (;;) {
     if (!ok(hipMemcpyAsync(s->host_ctrl, g.ctrl, sizeof(Control), hipMemcpyDeviceToHost, s->stream), err, err_len,
@@ -784,7 +1151,7 @@ extern "C" int ugrid_run(ugrid_session* s, double tend, int64_t maxsteps, ugrid_
         !ok(hipStreamSynchronize(s->stream), err, err_len, "trial"))
       return 3;
     if (s->host_ctrl->status != RUNNING) break;
-    const long long now = s->host_ctrl->nsteps + s->host_ctrl->nreject;
+    const long long now = s->host_ctrl->events;
     if (now == progress) return fail(err, err_len, "ugrid_run: device step control made no progress");
     progress = now;
     if (!ok(hipGraphLaunch(s->exec, s->stream), err, err_len, "trial graph")) return 3;
@@ -794,12 +1161,27 @@ extern "C" int ugrid_run(ugrid_session* s, double tend, int64_t maxsteps, ugrid_
     meta->nsteps = c.nsteps;
     meta->nreject = c.nreject;
   }
-  if (c.status == 2) {
-    char line[160];
+  char line[256];
+  line[0] = 0;
+  if (implicit) {
+    if (c.reason == WHY_INITIAL)
+      std::snprintf(line, sizeof(line), "backward Euler: invalid initial fluid state");
+    else if (c.reason == WHY_PSEUDO)
+      std::snprintf(line, sizeof(line), "backward Euler: nonphysical pseudo candidate; physical step not accepted");
+    else if (c.reason == WHY_DEFECT)
+      std::snprintf(line, sizeof(line), "backward Euler: non-finite defect; physical step not accepted");
+    else if (c.reason == WHY_MAXITER)
+      std::snprintf(line, sizeof(line),
+                    "backward Euler not converged: step=%lld updates=%d initial=%.6e defect=%.6e tolerance=%.6e; "
+                    "physical step not accepted",
+                    c.nsteps, c.iter, c.initial_defect, c.final_defect, c.tolerance);
+    else if (c.reason == WHY_DT)
+      std::snprintf(line, sizeof(line), "backward Euler: physical dt is invalid or cannot advance time");
+  } else if (c.status == 2) {
     std::snprintf(line, sizeof(line), "non-physical RK candidate after %d retries at accepted step %lld", c.retries,
                   c.nsteps);
-    fail(err, err_len, line);
   }
+  if (line[0]) fail(err, err_len, line);
   return c.status;
 }
 
@@ -814,7 +1196,7 @@ extern "C" int ugrid_download(ugrid_session* s, double* u, uint64_t u_len, doubl
     return 3;
   const size_t bytes = s->states * sizeof(double);
   if (u) {
-    if (!ok(hipMemcpyAsync(u, g.U[c.cur], bytes, hipMemcpyDeviceToHost, s->stream), err, err_len, "download") ||
+    if (!ok(hipMemcpyAsync(u, g.B[c.cur], bytes, hipMemcpyDeviceToHost, s->stream), err, err_len, "download") ||
         !ok(hipStreamSynchronize(s->stream), err, err_len, "download"))
       return 3;
     std::vector<double> other;
@@ -824,7 +1206,7 @@ extern "C" int ugrid_download(ugrid_session* s, double* u, uint64_t u_len, doubl
       ghosts = nullptr;
     } else if (c.ghost_from == 2) {
       other.resize(s->states);
-      if (!ok(hipMemcpyAsync(other.data(), g.U[c.cur ^ 1], bytes, hipMemcpyDeviceToHost, s->stream), err, err_len,
+      if (!ok(hipMemcpyAsync(other.data(), g.B[c.cur ^ 1], bytes, hipMemcpyDeviceToHost, s->stream), err, err_len,
               "download") ||
           !ok(hipStreamSynchronize(s->stream), err, err_len, "download"))
         return 3;
@@ -854,10 +1236,11 @@ extern "C" int ugrid_rhs(ugrid_session* s, const double* u, uint64_t u_len, doub
   if (u_len != s->states) return fail(err, err_len, "ugrid_rhs: u length mismatch");
   const size_t bytes = s->states * sizeof(double);
   if (!s->R && !device_alloc(s, &s->R, s->states, err, err_len)) return 3;
-  // The second-stage buffer carries the input: it holds nothing between trials.
+  if (!s->rhs_in && !device_alloc(s, &s->rhs_in, s->states, err, err_len)) return 3;
   hipLaunchKernelGGL(k_zero, dim3(1024), dim3(kThreads), 0, s->stream, s->R, s->states);
-  if (!ok(hipMemcpyAsync(s->g.S2, u, bytes, hipMemcpyHostToDevice, s->stream), err, err_len, "upload state")) return 3;
-  hipLaunchKernelGGL(k_stage<0>, dim3(s->launch_blocks), dim3(kThreads), 0, s->stream, s->g, s->g.S2, s->R);
+  if (!ok(hipMemcpyAsync(s->rhs_in, u, bytes, hipMemcpyHostToDevice, s->stream), err, err_len, "upload state"))
+    return 3;
+  hipLaunchKernelGGL(k_stage<0>, dim3(s->launch_blocks), dim3(kThreads), 0, s->stream, s->g, s->rhs_in, s->R);
   if (!ok(hipGetLastError(), err, err_len, "residual launch") ||
       !ok(hipMemcpyAsync(r, s->R, bytes, hipMemcpyDeviceToHost, s->stream), err, err_len, "download residual") ||
       !ok(hipStreamSynchronize(s->stream), err, err_len, "residual"))
@@ -865,8 +1248,25 @@ extern "C" int ugrid_rhs(ugrid_session* s, const double* u, uint64_t u_len, doub
   return 0;
 }
 
+extern "C" int ugrid_implicit_stats(const ugrid_session* s, ugrid_implicit_info* out) {
+  if (!s || !out) return 1;
+  Control c{};
+  if (hipMemcpyAsync(&c, s->g.ctrl, sizeof(Control), hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+      hipStreamSynchronize(s->stream) != hipSuccess)
+    return 1;
+  out->updates = c.updates;
+  out->retries = c.iretries;
+  out->checks = c.checks;
+  out->reason = c.reason;
+  out->initial_defect = c.initial_defect;
+  out->final_defect = c.final_defect;
+  out->tolerance = c.tolerance;
+  return 0;
+}
+
 extern "C" int ugrid_profile(ugrid_session* s, int32_t trials, double* stage_us, char* err, size_t err_len) {
   if (!s || !stage_us || trials < 1) return fail(err, err_len, "ugrid_profile: invalid argument");
+  if (s->par.time_integrator != 0) return fail(err, err_len, "ugrid_profile: SSP-RK3 sessions only");
   const Grid& g = s->g;
   launch_bound(s);
   hipLaunchKernelGGL(k_open, dim3(1), dim3(kThreads), 0, s->stream, g, HUGE_VAL, (1LL << 62));

@@ -61,11 +61,12 @@ bool cfd::solver2d_cuda_available(char* reason, size_t reason_size) {
 
 int cfd::solver2d_run_cuda(Solver2D* solver, double tend, long maxsteps) {
   SolverView& s = *reinterpret_cast<SolverView*>(solver);
-  if (s.par.time_integrator == TIME_BACKWARD_EULER) {
-    std::fprintf(stderr, "HIP backend: backward Euler is not implemented on the device; use the SSP-RK3 integrator\n");
+  const bool implicit = s.par.time_integrator == TIME_BACKWARD_EULER;
+  if (!implicit && s.par.time_integrator != TIME_SSP_RK3) return 3;
+  if (implicit && (s.par.limiter != LIM_FIRST_ORDER || s.ib != nullptr)) {
+    std::fprintf(stderr, "backward Euler currently requires first-order regular/cut-cell grids (no legacy IB)\n");
     return 3;
   }
-  if (s.par.time_integrator != TIME_SSP_RK3) return 3;
   char available[256];
   if (!solver2d_cuda_available(available, sizeof(available))) {
     std::fprintf(stderr, "CUDA backend unavailable: %s\n", available);
@@ -124,8 +125,14 @@ int cfd::solver2d_run_cuda(Solver2D* solver, double tend, long maxsteps) {
   ugrid_meta meta{};
   int status = ugrid_run(session, tend, maxsteps, &meta, err, sizeof(err));
   const auto t2 = std::chrono::steady_clock::now();
-  if (status == 2) std::fprintf(stderr, "CUDA solver: %s\n", err);
-  if (status == 0 || status == 1 || status == 2) {
+  ugrid_implicit_info info{};
+  const bool have_info = ugrid_implicit_stats(session, &info) == 0;
+  if (status == 2 && !implicit) std::fprintf(stderr, "CUDA solver: %s\n", err);
+  if ((status == 2 || status == 4) && implicit) std::fprintf(stderr, "CUDA %s\n", err);
+  // State, clocks and steps come back after every device run (a backward-Euler step that
+  // cannot advance included); refused controls stop before the device (src/euler2d_cuda.cu:694-707).
+  const bool dt_stop = implicit && status == 3 && have_info && info.reason == 5;
+  if (status == 0 || status == 1 || status == 2 || status == 4 || dt_stop) {
     char derr[256];
     if (ugrid_download(session, s.U.p, states, s.t.p, s.dt.p, derr, sizeof(derr))) {
       std::fprintf(stderr, "HIP backend: %s\n", derr);
@@ -133,6 +140,15 @@ int cfd::solver2d_run_cuda(Solver2D* solver, double tend, long maxsteps) {
     } else {
       s.nsteps += static_cast<long>(meta.nsteps);
       s.nreject += static_cast<long>(meta.nreject);
+      if (have_info) {
+        s.implicit.updates += static_cast<long>(info.updates);
+        s.implicit.retries += static_cast<long>(info.retries);
+        if (info.checks > 0) {
+          s.implicit.initial_defect = info.initial_defect;
+          s.implicit.final_defect = info.final_defect;
+          s.implicit.tolerance = info.tolerance;
+        }
+      }
     }
   } else {
     std::fprintf(stderr, "HIP backend: %s\n", err);

@@ -4,10 +4,11 @@ src/euler2d.h:1-225) over the HIP backend of include/ugrid_hip.h.
 Mirrors the reference interface: `params2d_default`, `grid2d_stretch_faces`, `Solver2D`
 (`solver2d_init`'s grid and arrays, `set_primitive`, `get_primitive`, `set_mco`, `integral`,
 `u2idx`), `solver2d_run_cuda` / `solver2d_run_backend` / `solver2d_cuda_available` with the
-reference's return codes (0 every case at tend, 1 maxsteps, 2 nonphysical after 12 retries,
-3 unsupported or unavailable).  The device run covers what the reference's GPU backend covers
-on a regular grid: FP64, first order (limiter 3), HLLC / AUSM+-up, SSP-RK3, per-case time,
-step and AUSM+-up reference Mach number.  There is no CPU solver here: backend 0 is the
+reference's return codes (0 every case at tend, 1 maxsteps, 2 nonphysical, 3 unsupported or
+unavailable, 4 backward-Euler step not converged).  The device run covers what the reference's
+GPU backend covers on a regular grid: FP64, first order (limiter 3), HLLC / AUSM+-up, SSP-RK3
+or backward Euler by pseudo-time defect relaxation, per-case time, step and AUSM+-up reference
+Mach number.  There is no CPU solver here: backend 0 is the
 reference's own CPU code and is refused.
 """
 from __future__ import annotations
@@ -123,6 +124,9 @@ class Solver2D:
         self.mco = np.full(ncase, par.ausm_mco, dtype=np.float64)
         self.nsteps = 0
         self.nreject = 0
+        # ImplicitStats2D (src/euler2d.h:118-121): backward-Euler updates / retries (cumulative)
+        # and the latest convergence check's largest first norm, norm and tolerance
+        self.implicit = dict(updates=0, retries=0, initial_defect=0.0, final_defect=0.0, tolerance=0.0)
         self.last_error = ""
 
     # -- field access (src/euler2d.h:137-140, src/euler2d.cpp:158-191) --
@@ -213,7 +217,7 @@ class DeviceSession:
         if status:
             raise RuntimeError(err.value.decode())
         self.meta = N.ugrid_meta(0, 0)
-        self._counted = (0, 0)
+        self._counted = (0, 0, 0, 0)
 
     def run(self, tend: float, maxsteps: int) -> tuple[int, str]:
         err = N.err_buffer()
@@ -233,7 +237,19 @@ class DeviceSession:
         s.dt[:] = dt
         s.nsteps += self.meta.nsteps - self._counted[0]
         s.nreject += self.meta.nreject - self._counted[1]
-        self._counted = (self.meta.nsteps, self.meta.nreject)
+        imp = self.implicit_stats()
+        s.implicit["updates"] += imp["updates"] - self._counted[2]
+        s.implicit["retries"] += imp["retries"] - self._counted[3]
+        if imp["checks"]:
+            for key in ("initial_defect", "final_defect", "tolerance"):
+                s.implicit[key] = imp[key]
+        self._counted = (self.meta.nsteps, self.meta.nreject, imp["updates"], imp["retries"])
+
+    def implicit_stats(self) -> dict:
+        info = N.ugrid_implicit_info()
+        if self.lib.ugrid_implicit_stats(self.handle, C.byref(info)):
+            raise RuntimeError("ugrid_implicit_stats failed")
+        return {name: getattr(info, name) for name, _ in info._fields_}
 
     def rhs(self, u: np.ndarray) -> np.ndarray:
         u = np.ascontiguousarray(u, dtype=np.float64)
@@ -282,8 +298,8 @@ def solver2d_run_cuda(s: Solver2D, tend: float, maxsteps: int) -> int:
     runs, writes back U, t, dt, nsteps and nreject.  Refusals print the reference's message and
     return 3."""
     par = s.par
-    if par.time_integrator != TIME_SSP_RK3:
-        s.last_error = "the device backend implements the SSP-RK3 integrator only"
+    if par.time_integrator not in (TIME_SSP_RK3, TIME_BACKWARD_EULER):
+        s.last_error = "unknown time integrator"
     elif par.limiter != LIM_FIRST_ORDER:
         s.last_error = ("CUDA backend currently requires --limiter first; refusing to change the method "
                         "implicitly")
@@ -295,12 +311,14 @@ def solver2d_run_cuda(s: Solver2D, tend: float, maxsteps: int) -> int:
         return 3
     with DeviceSession(s) as session:
         status, message = session.run(tend, maxsteps)
-        if status in (0, 1, 2):
+        # The reference writes the state, clocks and steps back after every device run, a step
+        # that cannot advance included (src/euler2d_cuda.cu:694-707); refused controls stop first.
+        if status in (0, 1, 2, 4) or (status == 3 and session.implicit_stats()["reason"] == 5):
             session.pull()
     s.last_error = message
-    if status == 2:
+    if status == 2 and par.time_integrator == TIME_SSP_RK3:
         print(f"CUDA solver: {message}", file=sys.stderr)
-    elif status not in (0, 1):
+    elif status != 0 and status != 1 and message:
         print(message, file=sys.stderr)
     return status
 

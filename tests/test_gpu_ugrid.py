@@ -183,3 +183,100 @@ def test_large_grid_properties():
         np.testing.assert_allclose(q[0], q[0][::-1, :], **tol)
         np.testing.assert_allclose(q[1], -q[1][:, ::-1], **tol)
         np.testing.assert_allclose(q[3], q[3].T, **tol)
+
+
+# ------------------------------------------------------------------ backward Euler --
+def implicit_params(**kw):
+    p = params(**kw)
+    p.time_integrator = E.TIME_BACKWARD_EULER
+    p.cfl = kw.get("cfl", 2.0)
+    return p
+
+
+def device_implicit(s, tend, maxsteps):
+    with s.device_session() as d:
+        status, msg = d.run(tend, maxsteps)
+        u = np.empty_like(s.U)
+        t = np.empty(s.ncase)
+        dt = np.empty(s.ncase)
+        err = N.err_buffer()
+        assert d.lib.ugrid_download(d.handle, N.dptr(u), u.size, N.dptr(t), N.dptr(dt), err, N.ERR_LEN) == 0
+        return dict(u=u, t=t, dt=dt, status=status, message=msg, nsteps=d.meta.nsteps, **d.implicit_stats())
+
+
+def assert_same_implicit(dev, ora):
+    assert dev["status"] == ora["status"]
+    for key in ("nsteps", "updates", "retries", "checks", "reason"):
+        assert dev[key] == ora[key], key
+    for key in ("initial_defect", "final_defect", "tolerance"):
+        assert dev[key] == ora[key], key
+    np.testing.assert_array_equal(dev["t"], ora["t"])
+    np.testing.assert_array_equal(dev["dt"], ora["dt"])
+    np.testing.assert_array_equal(dev["u"], ora["u"])
+
+
+BE_RUNS = {
+    "sod_cases": lambda: (riemann(130, 6, "x", RIEMANN_SET[:3], implicit_params()), 0.05),
+    "vortex_periodic_ausm": lambda: (vortex(70, ny=40, par=implicit_params(flux=E.FLUX_AUSMPUP, bc=ALL_BCS[0],
+                                                                           mco=0.3)), 0.2),
+    "blast_reflect_stretched": lambda: (blast(48, par=implicit_params(bc=ALL_BCS[2], cfl=1.5,
+                                                                      sx=(E.STRETCH_TANH_ENDS, 1.0))), 0.1),
+    # candidate retries (pseudo CFL halved 14 times over the run) that still converge
+    "strong_blast_retries": lambda: (riemann(64, 3, "x", RIEMANN_SET[3:5], _with(implicit_params(cfl=3.0),
+                                                                                  pseudo_cfl=100.0)), 0.002),
+}
+
+
+def _with(p, **kw):
+    for key, value in kw.items():
+        setattr(p, key, value)
+    return p
+
+
+@pytest.mark.parametrize("name", sorted(BE_RUNS))
+def test_backward_euler_bitwise(name):
+    s, tend = BE_RUNS[name]()
+    print(f"backward Euler {name} {s.grid.nx}x{s.grid.ny}x{s.ncase}", flush=True)
+    ora = G.run_implicit(s, tend, 500)
+    assert ora["status"] == 0 and ora["updates"] > ora["nsteps"] > 1
+    assert_same_implicit(device_implicit(s, tend, 500), ora)
+
+
+def test_backward_euler_stops():
+    print("backward Euler stops", flush=True)
+    # iteration limit (4), pseudo-candidate retries (2), a clock that cannot move (3), maxsteps (1)
+    s = riemann(64, 3, "x", [RIEMANN_SET[3]], implicit_params(cfl=4.0))
+    s.par.implicit_maxiter = 2
+    assert_same_implicit(device_implicit(s, 1.0, 10), G.run_implicit(s, 1.0, 10))
+    s = riemann(64, 3, "x", [RIEMANN_SET[3]], implicit_params(cfl=50.0))
+    s.par.pseudo_cfl = 1e4
+    ora = G.run_implicit(s, 1.0, 10)
+    assert_same_implicit(device_implicit(s, 1.0, 10), ora)
+    s = riemann(20, 2, "x", RIEMANN_SET[:3], implicit_params())
+    s.t[:] = [0.0, 1e20, 0.0]
+    s.dt[:] = [-1.0, -1.0, -1.0]
+    ora = G.run_implicit(s, 2e20, 10)
+    assert ora["status"] == 3 and ora["reason"] == 5
+    assert_same_implicit(device_implicit(s, 2e20, 10), ora)
+    s = riemann(40, 3, "x", RIEMANN_SET[:2], implicit_params())
+    ora = G.run_implicit(s, 1.0, 2)
+    assert ora["status"] == 1
+    assert_same_implicit(device_implicit(s, 1.0, 2), ora)
+    # an invalid initial state (2) and invalid controls (3, nothing runs)
+    bad = riemann(16, 2, "x", [RIEMANN_SET[0]], implicit_params())
+    bad.field(0)[3, 3, 4] = -1.0
+    dev = device_implicit(bad, 1.0, 10)
+    assert dev["status"] == 2 and dev["reason"] == 1 and "initial" in dev["message"]
+    bad = riemann(16, 2, "x", [RIEMANN_SET[0]], implicit_params())
+    bad.par.pseudo_cfl = 0.0
+    assert device_implicit(bad, 1.0, 10)["status"] == 3
+
+
+def test_backward_euler_python_mirror():
+    s = vortex(40, par=implicit_params(bc=ALL_BCS[0]), ncase=2)
+    ora = G.run_implicit(s, 0.2, 100)
+    assert E.solver2d_run_cuda(s, 0.2, 100) == 0
+    np.testing.assert_array_equal(s.U, ora["u"])
+    np.testing.assert_array_equal(s.t, ora["t"])
+    assert s.nsteps == ora["nsteps"]
+    assert s.implicit["updates"] == ora["updates"] and s.implicit["final_defect"] == ora["final_defect"]

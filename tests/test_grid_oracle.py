@@ -288,3 +288,81 @@ def test_run_conserves_on_periodic_box():
     after = [s.integral(0, v) for v in range(4)]
     for v in range(4):
         assert abs(after[v] - before[v]) <= 1e-12 * max(abs(before[v]), 1.0), v
+
+
+# ------------------------------------------------------------------ backward Euler --
+def implicit_params(**kw):
+    p = params(**kw)
+    p.time_integrator = E.TIME_BACKWARD_EULER
+    p.cfl = kw.get("cfl", 2.0)
+    return p
+
+
+BE_RUNS = {
+    "sod_two_cases": lambda: (riemann(40, 3, "x", RIEMANN_SET[:2], implicit_params()), 0.05),
+    "vortex_periodic": lambda: (vortex(16, par=implicit_params(bc=ALL_BCS[0])), 0.3),
+    "ausm_reflect_stretched": lambda: (riemann(30, 4, "x", RIEMANN_SET[2:4],
+                                               implicit_params(flux=E.FLUX_AUSMPUP, bc=ALL_BCS[4], cfl=1.5,
+                                                               sx=(E.STRETCH_GEOMETRIC, 1.02))), 0.02),
+}
+
+
+@needs_ref
+@pytest.mark.parametrize("name", sorted(BE_RUNS))
+def test_backward_euler_matches_reference_within_roundoff(name):
+    s, tend = BE_RUNS[name]()
+    a, b = G.run_implicit(s, tend, 1000), G.ref_run_implicit(s, tend, 1000)
+    assert a["status"] == b["status"] == 0
+    assert (a["nsteps"], a["updates"], a["retries"]) == (b["nsteps"], b["updates"], b["retries"])
+    assert a["updates"] > a["nsteps"]
+    np.testing.assert_array_equal(a["t"], b["t"])
+    np.testing.assert_allclose(a["dt"], b["dt"], rtol=1e-12)
+    assert rel(interior(s, a["u"]), interior(s, b["u"])) < 1e-12
+    np.testing.assert_allclose([a["initial_defect"], a["final_defect"], a["tolerance"]],
+                               [b["initial_defect"], b["final_defect"], b["tolerance"]], rtol=1e-6)
+
+
+def test_backward_euler_closed_forms():
+    # a uniform state is a steady solution: every step converges at the first check (no update)
+    s = E.Solver2D(12, 5, 0.0, 1.0, 0.0, 1.0, 2, implicit_params(bc=ALL_BCS[1]))
+    for c in range(2):
+        s.set_primitive_field(c, 1.2, 0.3 * c, -0.1, 0.8)
+    r = G.run_implicit(s, 0.1, 100)
+    assert r["status"] == 0 and r["updates"] == 0 and r["nsteps"] >= 1
+    np.testing.assert_array_equal(interior(s, r["u"]), interior(s, s.U))
+    assert np.all(r["t"] == 0.1)
+    # the converged step satisfies the backward-Euler equation to the tolerance: U1 - U0 = dt L(U1)
+    s = riemann(24, 2, "x", [RIEMANN_SET[0]], implicit_params(cfl=1.0))
+    s.par.implicit_rtol = 0.0
+    s.par.implicit_atol = 1e-12
+    r = G.run_implicit(s, 10.0, 1)
+    assert r["status"] == 1 and r["nsteps"] == 1
+    s1 = riemann(24, 2, "x", [RIEMANN_SET[0]], implicit_params(cfl=1.0))
+    s1.U[:] = r["u"]
+    R = G.rhs(s1, r["u"])
+    defect = r["dt"][0] * interior(s1, R) - (interior(s1, r["u"]) - interior(s, s.U))
+    assert np.max(np.abs(defect) / np.maximum(1.0, np.abs(interior(s, s.U)))) <= 1e-12
+    # the iteration limit ends the run with 4 and leaves state and clock at the last step
+    s = riemann(24, 2, "x", [RIEMANN_SET[3]], implicit_params(cfl=4.0))
+    s.par.implicit_maxiter = 2
+    r = G.run_implicit(s, 1.0, 10)
+    assert r["status"] == 4 and r["reason"] == 4 and r["nsteps"] == 0 and r["updates"] == 2
+    np.testing.assert_array_equal(interior(s, r["u"]), interior(s, s.U))
+    assert r["t"][0] == 0.0 and r["dt"][0] > 0.0
+    # invalid controls and an invalid initial state
+    s = riemann(8, 2, "x", [RIEMANN_SET[0]], implicit_params())
+    s.par.pseudo_cfl = 0.0
+    assert G.run_implicit(s, 1.0, 10)["status"] == 3
+    s = riemann(8, 2, "x", [RIEMANN_SET[0]], implicit_params())
+    s.field(0)[3, 3, 4] = -1.0
+    r = G.run_implicit(s, 1.0, 10)
+    assert r["status"] == 2 and r["reason"] == 1
+    # a step that cannot move the clock is refused with 3; the failing case's step is stored
+    # before the test (src/implicit2d.cpp:60-61), later cases keep theirs
+    s = riemann(8, 2, "x", RIEMANN_SET[:3], implicit_params())
+    s.t[:] = [0.0, 1e20, 0.0]
+    s.dt[:] = [-1.0, -1.0, -1.0]
+    r = G.run_implicit(s, 2e20, 10)
+    assert r["status"] == 3 and r["reason"] == 5
+    assert r["dt"][0] > 0.0 and 0.0 < r["dt"][1] < 1.0 and r["dt"][2] == -1.0
+    np.testing.assert_array_equal(r["t"], s.t)
