@@ -7,7 +7,8 @@
 //   Array .................... src/array.h:12-58
 //   Grid2D ................... src/euler2d.h:54-60
 //   ImplicitStats2D .......... src/euler2d.h:118-121
-//   Solver2D ................. src/euler2d.h:123-135 (IB2D / Cut2D only by pointer)
+//   WallCell2D, Cut2D ........ src/euler2d.h:92-116
+//   Solver2D ................. src/euler2d.h:123-135 (IB2D only by pointer)
 //   Params2D, Stretch, enums . amr2d_types.h (src/euler2d.h:23-52, src/types.h:7-13)
 //   solver2d_run_cuda ........ replaces src/euler2d.h:211 (src/euler2d_cuda.cu:530-708)
 //   solver2d_cuda_available .. replaces src/euler2d.h:213 (src/euler2d_cuda.cu:510-528)
@@ -16,13 +17,14 @@
 // not converged); it reads the grid, parameters, U, t, dt and mco, and writes back U (ghost cells
 // as the reference's device array ends), t, dt, nsteps, nreject and the implicit statistics.
 // Supported as by the reference's backend: FP64, first-order limiter, SSP-RK3 and backward
-// Euler on regular grids; cut cells and the IB path return 3.
+// Euler on regular and cut-cell grids (SRD after each stage); the IB path returns 3.
 #ifndef HIPAMR_EULER2D_GPU_H
 #define HIPAMR_EULER2D_GPU_H
 
 #include <cstddef>
 #include <cstdio>
 #include <cstdlib>
+#include <vector>
 
 #include "amr2d_types.h"
 
@@ -89,7 +91,34 @@ struct Grid2D {
 };
 
 struct IB2D;
-struct Cut2D;
+
+struct WallCell2D {
+  size_t owner_cell, begin, count;
+};
+
+// Cut-cell geometry and SRD neighbourhoods (built by the reference's cut2d_build_polygon, or by
+// ugrid_cut_build of ugrid_hip.h).  Cell arrays have mx * my entries, x-face arrays
+// (nx + 1) * ny (index j * (nx + 1) + i), y-face arrays nx * (ny + 1) (index j * nx + i).
+struct Cut2D {
+  std::vector<WallFragment2D> wall_fragments;
+  std::vector<WallCell2D> wall_cells;
+  int nsmall;
+  int nbad_nbhd;
+  int nsolid, ncut;
+  Array vol;
+  Array cx, cy;
+  Array flag;
+  Array axl, axy;
+  Array ayl, ayx;
+  Array wax, way;
+  Array wcx, wcy;
+  Array nbrcount;
+  Array smallpos;
+  Array sidx;
+  Array scnt;
+  Array smem;
+  Array svhat, sxhat, syhat;
+};
 
 struct ImplicitStats2D {
   long updates, retries;

@@ -37,6 +37,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "amr_hip.h" /* amr_wall: the WallFragment2D record */
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -61,6 +63,22 @@ typedef struct ugrid_params {
   double pseudo_cfl, implicit_rtol, implicit_atol;
 } ugrid_params;
 
+/* Cut-cell geometry of a grid (the arrays of Cut2D, src/euler2d.h:97-116, same layouts and
+ * meanings; mx * my cell arrays include the ghosts).  The device backend reads nsmall,
+ * nbad_nbhd, vol, flag, axl, ayl, wax, way, nbrcount, smallpos, scnt, smem and svhat. */
+typedef struct ugrid_cut_view {
+  int32_t nsmall, nbad_nbhd, nsolid, ncut;
+  uint64_t n_walls;
+  const double *vol, *cx, *cy, *flag;   /* mx * my: fluid area (0 solid), centroid, 0 full / 1 cut / 2 solid */
+  const double *axl, *axy;              /* (nx + 1) * ny x-face apertures and their y centroids, j * (nx + 1) + i */
+  const double *ayl, *ayx;              /* nx * (ny + 1) y-face apertures and their x centroids, j * nx + i */
+  const double *wax, *way, *wcx, *wcy;  /* mx * my wall vector (into the solid, times length), wall centroid */
+  const double *nbrcount, *smallpos;    /* mx * my SRD neighbourhood count, position in the small-cell list or -1 */
+  const double *sidx, *scnt, *smem;     /* nsmall cell indices, member counts, 25 * nsmall members */
+  const double *svhat, *sxhat, *syhat;  /* nsmall weighted volumes and centroids */
+  const amr_wall* walls;                /* n_walls wall fragments (normal fluid -> solid) */
+} ugrid_cut_view;
+
 typedef struct ugrid_setup {
   int32_t nx, ny, ncase;
   int32_t pad;
@@ -68,7 +86,24 @@ typedef struct ugrid_setup {
   const double* dy;     /* ny + 4 */
   ugrid_params params;
   const double* mco;    /* ncase AUSM+-up reference Mach numbers, or NULL for params.ausm_mco */
+  const ugrid_cut_view* cut;  /* NULL: regular grid */
 } ugrid_setup;
+
+/* Cut-cell geometry and SRD neighbourhoods of a closed solid polygon on the grid
+ * (cut2d_build_polygon, src/cutcell2d.cpp:247-503; bit-identical arrays).  axes: nx, ny and the
+ * ugrid_axis outputs of both axes.  patches: one patch id per edge or NULL.  Returns 0, 1 (an
+ * invalid polygon), 2 (the body within three cells of the domain boundary) or 3 (an edge on a
+ * grid line), as the reference does. */
+typedef struct ugrid_axes {
+  int32_t nx, ny;
+  const double *xf, *xc, *dx;  /* nx + 1, nx + 4, nx + 4 */
+  const double *yf, *yc, *dy;  /* ny + 1, ny + 4, ny + 4 */
+} ugrid_axes;
+typedef struct ugrid_cut ugrid_cut;
+int ugrid_cut_build(const ugrid_axes* axes, const double* px, const double* py, int32_t n, const int32_t* patches,
+                    ugrid_cut** out, char* err, size_t err_len);
+int ugrid_cut_get(const ugrid_cut* cut, ugrid_cut_view* view);
+void ugrid_cut_destroy(ugrid_cut* cut);
 
 typedef struct ugrid_session ugrid_session;
 
@@ -105,7 +140,9 @@ int ugrid_implicit_stats(const ugrid_session* s, ugrid_implicit_info* out);
 /* Sizes and field offsets of the Solver2D view behind cfd::solver2d_run_cuda (euler2d_gpu.h):
  * Solver2D, Grid2D, Array, Params2D, ImplicitStats2D, then grid, grid.{nx, ng, mx, my, xmin,
  * xf, yf, xc, dx, yc, dy}, par, ncase, U, U1, U2, R, W, SX, SY, FX, FY, t, dt, mco, nsteps,
- * nreject, implicit, ib, cut.  Writes min(capacity, 36) values, returns 36. */
+ * nreject, implicit, ib, cut; then the size of Cut2D and the offsets of nsmall, nbad_nbhd, vol,
+ * flag, axl, ayl, wax, way, nbrcount, smallpos, scnt, smem, svhat.  Writes min(capacity, 50)
+ * values, returns 50. */
 int ugrid_solver2d_layout(int64_t* out, int32_t capacity);
 
 /* Average duration in microseconds of each RK stage kernel (stage_us[0..2]) over `trials`

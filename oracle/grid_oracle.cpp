@@ -14,6 +14,18 @@
 //   wave-speed bound ........... src/euler2d_cuda.cu:271-287 (maximum taken on the bit
 //                                patterns, as atomicMax on the bits does: a NaN bound wins)
 //   RK combine, admissibility .. src/euler2d_cuda.cu:289-314
+//   cut cells .................. src/euler2d_cuda.cu:182-258, 271-314 with a Cut2D: solid cells
+//                                (area 0) get primitives (1, 0, 0, 1), residual 0 and are
+//                                skipped by the bound and the admissibility test; closed faces
+//                                carry no flux; a cut cell gathers its apertures' fluxes and the
+//                                wall pressure term with its own pressure
+//   SRD ........................ src/euler2d_cuda.cu:316-368, 461-468 (group averages over the
+//                                members, regular cells keep their value, every member adds each
+//                                neighbourhood's average -- here in ascending neighbourhood
+//                                order, the CPU solver's order; the reference's device adds them
+//                                with atomics in no fixed order -- then divides by its count),
+//                                after each stage whose candidate is admissible, then the
+//                                admissibility test again
 //   run loop ................... src/euler2d_cuda.cu:605-693 (dt choice and clipping, three
 //                                stages with a check after each, retry halving with the
 //                                "hdt > 0" / "hdt < 0 or not finite" tests, accept copy of the
@@ -185,9 +197,17 @@ void residual(const ugrid_setup& s, const std::vector<double>& mco, const double
   const Shape g(s);
   const double gm = s.params.gamma;
   std::vector<double> W(static_cast<size_t>(g.ncase) * 4 * g.plane);
+  const ugrid_cut_view* cut = s.cut;
   for (int c = 0; c < g.ncase; ++c)
     for (size_t k = 0; k < g.plane; ++k) {
       const size_t b = static_cast<size_t>(c) * 4 * g.plane + k;
+      if (cut && cut->vol[k] <= 0.0) {
+        W[b] = 1.0;
+        W[b + g.plane] = 0.0;
+        W[b + 2 * g.plane] = 0.0;
+        W[b + 3 * g.plane] = 1.0;
+        continue;
+      }
       const double r = U[b], mu = U[b + g.plane], mv = U[b + 2 * g.plane], E = U[b + 3 * g.plane];
       W[b] = r;
       W[b + g.plane] = mu / r;
@@ -207,18 +227,24 @@ void residual(const ugrid_setup& s, const std::vector<double>& mco, const double
     double L[4], Rt[4], f[4];
     for (int j = 0; j < g.ny; ++j)
       for (int i = 0; i <= g.nx; ++i) {
-        prim(i - 1, j, true, L);
-        prim(i, j, true, Rt);
-        face(s.params, mco[c], L, Rt, f);
         const size_t q = static_cast<size_t>(j) * (g.nx + 1) + i;
+        f[0] = f[1] = f[2] = f[3] = 0.0;
+        if (!cut || cut->axl[q] > 0.0) {
+          prim(i - 1, j, true, L);
+          prim(i, j, true, Rt);
+          face(s.params, mco[c], L, Rt, f);
+        }
         for (int v = 0; v < 4; ++v) FX[v * nfx + q] = f[v];
       }
     for (int j = 0; j <= g.ny; ++j)
       for (int i = 0; i < g.nx; ++i) {
-        prim(i, j - 1, false, L);
-        prim(i, j, false, Rt);
-        face(s.params, mco[c], L, Rt, f);
         const size_t q = static_cast<size_t>(j) * g.nx + i;
+        f[0] = f[1] = f[2] = f[3] = 0.0;
+        if (!cut || cut->ayl[q] > 0.0) {
+          prim(i, j - 1, false, L);
+          prim(i, j, false, Rt);
+          face(s.params, mco[c], L, Rt, f);
+        }
         FY[q] = f[0];
         FY[nfy + q] = f[2];
         FY[2 * nfy + q] = f[1];
@@ -227,9 +253,25 @@ void residual(const ugrid_setup& s, const std::vector<double>& mco, const double
     for (int j = 0; j < g.ny; ++j)
       for (int i = 0; i < g.nx; ++i) {
         const size_t e = static_cast<size_t>(j) * (g.nx + 1) + i, n = static_cast<size_t>(j) * g.nx + i;
-        for (int v = 0; v < 4; ++v)
-          R[g.at(c, v, i, j)] = -(FX[v * nfx + e + 1] - FX[v * nfx + e]) / s.dx[i + NG] -
-                                (FY[v * nfy + n + g.nx] - FY[v * nfy + n]) / s.dy[j + NG];
+        const size_t k = static_cast<size_t>(j + NG) * g.mx + (i + NG);
+        if (cut && cut->vol[k] <= 0.0) {
+          for (int v = 0; v < 4; ++v) R[g.at(c, v, i, j)] = 0.0;
+          continue;
+        }
+        if (!cut || cut->flag[k] == 0.0) {
+          for (int v = 0; v < 4; ++v)
+            R[g.at(c, v, i, j)] = -(FX[v * nfx + e + 1] - FX[v * nfx + e]) / s.dx[i + NG] -
+                                  (FY[v * nfy + n + g.nx] - FY[v * nfy + n]) / s.dy[j + NG];
+          continue;
+        }
+        const double iv = 1.0 / cut->vol[k], pw = W[g.at(c, 3, i, j)];
+        for (int v = 0; v < 4; ++v) {
+          double r = -iv * (cut->axl[e + 1] * FX[v * nfx + e + 1] - cut->axl[e] * FX[v * nfx + e] +
+                            cut->ayl[n + g.nx] * FY[v * nfy + n + g.nx] - cut->ayl[n] * FY[v * nfy + n]);
+          if (v == 1) r -= iv * pw * cut->wax[k];
+          if (v == 2) r -= iv * pw * cut->way[k];
+          R[g.at(c, v, i, j)] = r;
+        }
       }
   }
 }
@@ -241,6 +283,7 @@ double bound_max(const ugrid_setup& s, const double* U, int c) {
   uint64_t best = 0;
   for (int j = 0; j < g.ny; ++j)
     for (int i = 0; i < g.nx; ++i) {
+      if (s.cut && s.cut->vol[static_cast<size_t>(j + NG) * g.mx + (i + NG)] <= 0.0) continue;
       const size_t k = g.at(c, 0, i, j);
       const double r = U[k], mu = U[k + g.plane], mv = U[k + 2 * g.plane], E = U[k + 3 * g.plane];
       const double u = mu / r, v = mv / r;
@@ -262,12 +305,51 @@ bool any_bad(const ugrid_setup& s, const double* U) {
   for (int c = 0; c < g.ncase; ++c)
     for (int j = 0; j < g.ny; ++j)
       for (int i = 0; i < g.nx; ++i) {
+        if (s.cut && s.cut->vol[static_cast<size_t>(j + NG) * g.mx + (i + NG)] <= 0.0) continue;
         const size_t k = g.at(c, 0, i, j);
         const double r = U[k], mu = U[k + g.plane], mv = U[k + 2 * g.plane], E = U[k + 3 * g.plane];
         const double p = (gm - 1.0) * (E - 0.5 * (mu * mu + mv * mv) / r);
         if (!(r > 0.0) || !(p > 0.0) || !std::isfinite(r) || !std::isfinite(p)) return true;
       }
   return false;
+}
+
+// State redistribution of every case in place (first order: the neighbourhood average).
+void srd(const ugrid_setup& s, double* U) {
+  const ugrid_cut_view* cut = s.cut;
+  if (!cut || cut->nsmall == 0) return;
+  const Shape g(s);
+  const int ns = cut->nsmall;
+  std::vector<double> Qh(static_cast<size_t>(ns) * 4), acc(4 * g.plane);
+  for (int c = 0; c < g.ncase; ++c) {
+    double* Uc = U + static_cast<size_t>(c) * 4 * g.plane;
+    for (int q = 0; q < ns; ++q)
+      for (int v = 0; v < 4; ++v) {
+        double sum = 0.0;
+        for (int m = 0; m < static_cast<int>(cut->scnt[q]); ++m) {
+          const size_t r = static_cast<size_t>(cut->smem[25 * static_cast<size_t>(q) + m]);
+          sum += cut->vol[r] / cut->nbrcount[r] * Uc[v * g.plane + r];
+        }
+        Qh[static_cast<size_t>(q) * 4 + v] = sum / cut->svhat[q];
+      }
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        const size_t k = static_cast<size_t>(j + NG) * g.mx + (i + NG);
+        for (int v = 0; v < 4; ++v)
+          acc[v * g.plane + k] = (cut->vol[k] > 0.0 && cut->smallpos[k] < 0.0) ? Uc[v * g.plane + k] : 0.0;
+      }
+    for (int q = 0; q < ns; ++q)
+      for (int m = 0; m < static_cast<int>(cut->scnt[q]); ++m) {
+        const size_t r = static_cast<size_t>(cut->smem[25 * static_cast<size_t>(q) + m]);
+        for (int v = 0; v < 4; ++v) acc[v * g.plane + r] += Qh[static_cast<size_t>(q) * 4 + v];
+      }
+    for (int j = 0; j < g.ny; ++j)
+      for (int i = 0; i < g.nx; ++i) {
+        const size_t k = static_cast<size_t>(j + NG) * g.mx + (i + NG);
+        if (cut->vol[k] <= 0.0 || (cut->smallpos[k] < 0.0 && cut->nbrcount[k] == 1.0)) continue;
+        for (int v = 0; v < 4; ++v) Uc[v * g.plane + k] = acc[v * g.plane + k] / cut->nbrcount[k];
+      }
+  }
 }
 
 // out = a A + b (B + dt R) on interior cells, dt per case.
@@ -314,17 +396,29 @@ int run(const ugrid_setup& s, double* U, double* t, double* dt_out, double tend,
       residual(s, mco, U, R.data());
       combine(s, 0.0, U, 1.0, U, R.data(), dt, U1.data());
       bool bad = any_bad(s, U1.data());
+      if (!bad && s.cut) {
+        srd(s, U1.data());
+        bad = any_bad(s, U1.data());
+      }
       if (!bad) {
         apply_bc(s, U1.data());
         residual(s, mco, U1.data(), R.data());
         combine(s, 0.75, U, 0.25, U1.data(), R.data(), dt, U2.data());
         bad = any_bad(s, U2.data());
+        if (!bad && s.cut) {
+          srd(s, U2.data());
+          bad = any_bad(s, U2.data());
+        }
       }
       if (!bad) {
         apply_bc(s, U2.data());
         residual(s, mco, U2.data(), R.data());
         combine(s, 1.0 / 3.0, U, 2.0 / 3.0, U2.data(), R.data(), dt, U1.data());
         bad = any_bad(s, U1.data());
+        if (!bad && s.cut) {
+          srd(s, U1.data());
+          bad = any_bad(s, U1.data());
+        }
       }
       if (!bad) break;
       ++*nreject;
@@ -427,6 +521,7 @@ int run_implicit(const ugrid_setup& s, double* U, double* t, double* dt, double 
         if (dt[c] > 0.0) {
           for (int j = 0; j < g.ny; ++j)
             for (int i = 0; i < g.nx; ++i) {
+              if (s.cut && s.cut->vol[static_cast<size_t>(j + NG) * g.mx + (i + NG)] <= 0.0) continue;
               double norm = 0.0;
               for (int v = 0; v < 4; ++v) {
                 const size_t k = g.at(c, v, i, j);
@@ -472,12 +567,26 @@ int run_implicit(const ugrid_setup& s, double* U, double* t, double* dt, double 
           if (dt[c] <= 0.0) continue;
           for (int j = 0; j < g.ny; ++j)
             for (int i = 0; i < g.nx; ++i) {
+              const size_t kc = static_cast<size_t>(j + NG) * g.mx + (i + NG);
+              if (s.cut && s.cut->vol[kc] <= 0.0) continue;
               const size_t k = g.at(c, 0, i, j);
               const double r = Q[k], mu = Q[k + g.plane], mv = Q[k + 2 * g.plane], E = Q[k + 3 * g.plane];
               const double u = mu / r, v = mv / r;
               const double p = (gm - 1.0) * (E - 0.5 * (mu * mu + mv * mv) / r);
               const double a = std::sqrt(gm * p / r);
-              const double rate = (std::fabs(u) + a) / s.dx[i + NG] + (std::fabs(v) + a) / s.dy[j + NG];
+              double rate;
+              if (!s.cut) {
+                rate = (std::fabs(u) + a) / s.dx[i + NG] + (std::fabs(v) + a) / s.dy[j + NG];
+              } else {
+                // implicit_spectral_rate of a cell of a cut grid (src/implicit2d.h:33-43)
+                const ugrid_cut_view* ct = s.cut;
+                const size_t fe = static_cast<size_t>(j) * (g.nx + 1) + i, fn = static_cast<size_t>(j) * g.nx + i;
+                const double wall = std::sqrt(ct->wax[kc] * ct->wax[kc] + ct->way[kc] * ct->way[kc]);
+                rate = ((std::fabs(u) + a) * (ct->axl[fe + 1] + ct->axl[fe]) +
+                        (std::fabs(v) + a) * (ct->ayl[fn + g.nx] + ct->ayl[fn]) +
+                        std::fabs(u * ct->wax[kc] + v * ct->way[kc]) + a * wall) /
+                       (2.0 * ct->vol[kc]);
+              }
               const double omega = pseudo / (pseudo + dt[c] * rate);
               for (int w = 0; w < 4; ++w) {
                 const size_t q = k + w * g.plane;

@@ -56,6 +56,16 @@ def _ref():
         lib.ref_grid_run_implicit.restype = C.c_int
         lib.ref_grid_run_implicit.argtypes = [C.c_int, C.c_int, D, C.c_int, P, I, D, D, D, D, D, C.c_double,
                                               C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64), D]
+        lib.ref_grid_cut_build.restype = C.c_int
+        lib.ref_grid_cut_build.argtypes = [C.c_int, C.c_int, D, P, I, D, D, D, C.c_int, C.POINTER(C.c_int),
+                                           C.POINTER(D), C.POINTER(D), C.POINTER(D), C.POINTER(D), D, C.c_int,
+                                           C.POINTER(C.c_int64), C.POINTER(N.amr_wall), C.c_int]
+        lib.ref_grid_cut_rhs.restype = C.c_int
+        lib.ref_grid_cut_rhs.argtypes = [C.c_int, C.c_int, D, C.c_int, P, I, D, D, D, D, C.c_int, D, D]
+        lib.ref_grid_cut_run.restype = C.c_int
+        lib.ref_grid_cut_run.argtypes = [C.c_int, C.c_int, D, C.c_int, P, I, D, D, D, D, C.c_int, D, D, D,
+                                         C.c_double, C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                         C.POINTER(C.c_int64), D]
         lib.ref_flux.restype = None
         lib.ref_flux.argtypes = [C.c_int, D, D, C.c_double, C.c_double, D]
         _bound.add("ref")
@@ -202,3 +212,72 @@ def ref_flux(kind: int, left, right, gamma: float = 1.4, mco: float = 1.0) -> np
     f = np.zeros(4)
     _ref().ref_flux(kind, N.dptr(a), N.dptr(b), gamma, mco, N.dptr(f))
     return f
+
+
+def ref_cut_build(s, px, py, patches=None, capacity: int = 4096, capacity_walls: int = 8192) -> dict:
+    """The reference's cut2d_build_polygon on s's grid: every array (the counts under 'counts')."""
+    g = s.grid
+    cells, nfx, nfy = g.mx * g.my, (g.nx + 1) * g.ny, g.nx * (g.ny + 1)
+    cell = [np.zeros(cells) for _ in range(10)]
+    xface = [np.zeros(nfx) for _ in range(2)]
+    yface = [np.zeros(nfy) for _ in range(2)]
+    small = [np.zeros(capacity) for _ in range(5)]
+    smem = np.zeros(25 * capacity)
+    counts = (C.c_int64 * 5)()
+    walls = (N.amr_wall * capacity_walls)()
+    ptrs = lambda arrays: (N.c_double_p * len(arrays))(*[N.dptr(a) for a in arrays])
+    types, params = _stretch(s)
+    box = _box(s)
+    p = s.par.to_c()
+    px = np.ascontiguousarray(px, dtype=np.float64)
+    py = np.ascontiguousarray(py, dtype=np.float64)
+    pat = None if patches is None else (C.c_int * len(patches))(*patches)
+    status = _ref().ref_grid_cut_build(g.nx, g.ny, N.dptr(box), C.byref(p), types, N.dptr(params), N.dptr(px),
+                                       N.dptr(py), len(px), pat, ptrs(cell), ptrs(xface), ptrs(yface), ptrs(small),
+                                       N.dptr(smem), capacity, counts, walls, capacity_walls)
+    out = dict(status=status)
+    if status:
+        return out
+    ns = counts[0]
+    out.update(zip(("vol", "cx", "cy", "flag", "wax", "way", "wcx", "wcy", "nbrcount", "smallpos"), cell))
+    out.update(axl=xface[0], axy=xface[1], ayl=yface[0], ayx=yface[1], smem=smem[:25 * ns])
+    out.update(zip(("sidx", "scnt", "svhat", "sxhat", "syhat"), [a[:ns] for a in small]))
+    out.update(nsmall=ns, nbad_nbhd=counts[1], nsolid=counts[2], ncut=counts[3])
+    out["walls"] = N.as_array(C.cast(walls, C.POINTER(N.amr_wall)), counts[4], N.WALL_DTYPE)
+    return out
+
+
+def ref_cut_rhs(s, px, py, u: np.ndarray) -> np.ndarray:
+    """The reference's residual with the cut geometry of the polygon (built by the reference)."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    r = np.zeros_like(u)
+    types, params = _stretch(s)
+    box = _box(s)
+    p = s.par.to_c()
+    mco = np.ascontiguousarray(s.mco, dtype=np.float64)
+    px = np.ascontiguousarray(px, dtype=np.float64)
+    py = np.ascontiguousarray(py, dtype=np.float64)
+    assert _ref().ref_grid_cut_rhs(s.grid.nx, s.grid.ny, N.dptr(box), s.ncase, C.byref(p), types, N.dptr(params),
+                                   N.dptr(mco), N.dptr(px), N.dptr(py), len(px), N.dptr(u), N.dptr(r)) == 0
+    return r
+
+
+def ref_cut_run(s, px, py, tend: float, maxsteps: int) -> dict:
+    """The reference's CPU solver2d_run (RK or backward Euler) with the polygon's cut geometry."""
+    u = np.array(s.U, dtype=np.float64, copy=True)
+    t = np.array(s.t, dtype=np.float64, copy=True)
+    dt = np.array(s.dt, dtype=np.float64, copy=True)
+    types, params = _stretch(s)
+    box = _box(s)
+    p = s.par.to_c()
+    mco = np.ascontiguousarray(s.mco, dtype=np.float64)
+    px = np.ascontiguousarray(px, dtype=np.float64)
+    py = np.ascontiguousarray(py, dtype=np.float64)
+    nsteps, nreject = C.c_int64(0), C.c_int64(0)
+    counts = (C.c_int64 * 2)(0, 0)
+    defects = np.zeros(3)
+    status = _ref().ref_grid_cut_run(s.grid.nx, s.grid.ny, N.dptr(box), s.ncase, C.byref(p), types, N.dptr(params),
+                                     N.dptr(mco), N.dptr(px), N.dptr(py), len(px), N.dptr(u), N.dptr(t), N.dptr(dt),
+                                     tend, maxsteps, C.byref(nsteps), C.byref(nreject), counts, N.dptr(defects))
+    return dict(u=u, t=t, dt=dt, status=status, nsteps=nsteps.value, nreject=nreject.value, updates=counts[0],
+                retries=counts[1], initial_defect=defects[0], final_defect=defects[1], tolerance=defects[2])

@@ -369,6 +369,89 @@ int ref_grid_run_implicit(int nx, int ny, const double* box, int ncase, const ug
   return status;
 }
 
+// cut2d_build_polygon (src/cutcell2d.cpp:247-503) on the grid of solver2d_init.  cell: 14 arrays of
+// mx * my (vol, cx, cy, flag, wax, way, wcx, wcy, nbrcount, smallpos, then 4 unused); xface: axl,
+// axy; yface: ayl, ayx; small: sidx, scnt, svhat, sxhat, syhat (capacity each) and smem (25 x
+// capacity); counts: nsmall, nbad_nbhd, nsolid, ncut, walls.  walls: capacity_walls records.
+int ref_grid_cut_build(int nx, int ny, const double* box, const ugrid_params* p, const int* stretch_type,
+                       const double* stretch_param, const double* px, const double* py, int n, const int* patches,
+                       double* const* cell, double* const* xface, double* const* yface, double* const* small,
+                       double* smem, int capacity, int64_t* counts, amr_wall* walls, int capacity_walls) {
+  auto s = ref_grid_solver(nx, ny, box, 1, p, stretch_type, stretch_param);
+  Cut2D cut;
+  const int status = cut2d_build_polygon(s.get(), px, py, n, &cut, patches);
+  if (status) return status;
+  const Array* cells[] = {&cut.vol, &cut.cx, &cut.cy, &cut.flag, &cut.wax, &cut.way, &cut.wcx, &cut.wcy,
+                          &cut.nbrcount, &cut.smallpos};
+  for (int k = 0; k < 10; ++k) std::memcpy(cell[k], cells[k]->data(), cells[k]->size() * sizeof(double));
+  std::memcpy(xface[0], cut.axl.data(), cut.axl.size() * sizeof(double));
+  std::memcpy(xface[1], cut.axy.data(), cut.axy.size() * sizeof(double));
+  std::memcpy(yface[0], cut.ayl.data(), cut.ayl.size() * sizeof(double));
+  std::memcpy(yface[1], cut.ayx.data(), cut.ayx.size() * sizeof(double));
+  if (cut.nsmall > capacity || static_cast<int>(cut.wall_fragments.size()) > capacity_walls) return 9;
+  const Array* smalls[] = {&cut.sidx, &cut.scnt, &cut.svhat, &cut.sxhat, &cut.syhat};
+  for (int k = 0; k < 5; ++k) std::memcpy(small[k], smalls[k]->data(), smalls[k]->size() * sizeof(double));
+  std::memcpy(smem, cut.smem.data(), cut.smem.size() * sizeof(double));
+  static_assert(sizeof(amr_wall) == sizeof(WallFragment2D), "wall record layout");
+  std::memcpy(walls, cut.wall_fragments.data(), cut.wall_fragments.size() * sizeof(WallFragment2D));
+  counts[0] = cut.nsmall;
+  counts[1] = cut.nbad_nbhd;
+  counts[2] = cut.nsolid;
+  counts[3] = cut.ncut;
+  counts[4] = static_cast<int64_t>(cut.wall_fragments.size());
+  return 0;
+}
+
+// The residual and solver2d_run (SSP-RK3 or backward Euler, by params->time_integrator) on the
+// grid with the cut geometry of the polygon (cut2d_build_polygon).  counts: updates, retries
+// (in/out); defects: initial, final, tolerance.
+int ref_grid_cut_rhs(int nx, int ny, const double* box, int ncase, const ugrid_params* p, const int* stretch_type,
+                     const double* stretch_param, const double* mco, const double* px, const double* py, int n,
+                     const double* u, double* r) {
+  auto s = ref_grid_solver(nx, ny, box, ncase, p, stretch_type, stretch_param);
+  Cut2D cut;
+  if (cut2d_build_polygon(s.get(), px, py, n, &cut, nullptr)) return 1;
+  solver2d_set_cut(s.get(), &cut);
+  if (mco)
+    for (int c = 0; c < ncase; ++c) solver2d_set_mco(s.get(), c, mco[c]);
+  std::memcpy(s->U.data(), u, s->U.size() * sizeof(double));
+  solver2d_apply_bc(s.get(), s->U.data());
+  solver2d_residual(s.get(), s->U.data(), s->R.data(), s->W.data(), s->SX.data(), s->SY.data(), s->FX.data(),
+                    s->FY.data());
+  std::memcpy(r, s->R.data(), s->R.size() * sizeof(double));
+  return 0;
+}
+
+int ref_grid_cut_run(int nx, int ny, const double* box, int ncase, const ugrid_params* p, const int* stretch_type,
+                     const double* stretch_param, const double* mco, const double* px, const double* py, int n,
+                     double* u, double* t, double* dt, double tend, int64_t maxsteps, int64_t* nsteps,
+                     int64_t* nreject, int64_t* counts, double* defects) {
+  auto s = ref_grid_solver(nx, ny, box, ncase, p, stretch_type, stretch_param);
+  Cut2D cut;
+  if (cut2d_build_polygon(s.get(), px, py, n, &cut, nullptr)) return 9;
+  solver2d_set_cut(s.get(), &cut);
+  for (int c = 0; c < ncase; ++c) {
+    if (mco) solver2d_set_mco(s.get(), c, mco[c]);
+    s->t[c] = t[c];
+    s->dt[c] = dt[c];
+  }
+  std::memcpy(s->U.data(), u, s->U.size() * sizeof(double));
+  const int status = solver2d_run(s.get(), tend, static_cast<long>(maxsteps));
+  std::memcpy(u, s->U.data(), s->U.size() * sizeof(double));
+  for (int c = 0; c < ncase; ++c) {
+    t[c] = s->t[c];
+    dt[c] = s->dt[c];
+  }
+  *nsteps = s->nsteps;
+  *nreject = s->nreject;
+  counts[0] = s->implicit.updates;
+  counts[1] = s->implicit.retries;
+  defects[0] = s->implicit.initial_defect;
+  defects[1] = s->implicit.final_defect;
+  defects[2] = s->implicit.tolerance;
+  return status;
+}
+
 // The same list as ugrid_solver2d_layout (include/ugrid_hip.h), measured on the reference's types.
 int ref_solver2d_layout(int64_t* out, int32_t capacity) {
   const int64_t v[] = {
@@ -380,7 +463,10 @@ int ref_solver2d_layout(int64_t* out, int32_t capacity) {
       offsetof(Solver2D, W), offsetof(Solver2D, SX), offsetof(Solver2D, SY), offsetof(Solver2D, FX),
       offsetof(Solver2D, FY), offsetof(Solver2D, t), offsetof(Solver2D, dt), offsetof(Solver2D, mco),
       offsetof(Solver2D, nsteps), offsetof(Solver2D, nreject), offsetof(Solver2D, implicit), offsetof(Solver2D, ib),
-      offsetof(Solver2D, cut)};
+      offsetof(Solver2D, cut), sizeof(Cut2D), offsetof(Cut2D, nsmall), offsetof(Cut2D, nbad_nbhd),
+      offsetof(Cut2D, vol), offsetof(Cut2D, flag), offsetof(Cut2D, axl), offsetof(Cut2D, ayl), offsetof(Cut2D, wax),
+      offsetof(Cut2D, way), offsetof(Cut2D, nbrcount), offsetof(Cut2D, smallpos), offsetof(Cut2D, scnt),
+      offsetof(Cut2D, smem), offsetof(Cut2D, svhat)};
   const int32_t n = static_cast<int32_t>(sizeof(v) / sizeof(v[0]));
   for (int32_t k = 0; k < n && k < capacity; ++k) out[k] = v[k];
   return n;

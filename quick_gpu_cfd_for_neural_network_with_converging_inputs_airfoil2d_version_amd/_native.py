@@ -275,9 +275,22 @@ class ugrid_params(C.Structure):
                 ("implicit_rtol", C.c_double), ("implicit_atol", C.c_double)]
 
 
+class ugrid_cut_view(C.Structure):
+    _fields_ = [("nsmall", C.c_int32), ("nbad_nbhd", C.c_int32), ("nsolid", C.c_int32), ("ncut", C.c_int32),
+                ("n_walls", C.c_uint64)] + [(name, c_double_p) for name in (
+                    "vol", "cx", "cy", "flag", "axl", "axy", "ayl", "ayx", "wax", "way", "wcx", "wcy", "nbrcount",
+                    "smallpos", "sidx", "scnt", "smem", "svhat", "sxhat", "syhat")] + [("walls", C.POINTER(amr_wall))]
+
+
 class ugrid_setup(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("ncase", C.c_int32), ("pad", C.c_int32),
-                ("dx", c_double_p), ("dy", c_double_p), ("params", ugrid_params), ("mco", c_double_p)]
+                ("dx", c_double_p), ("dy", c_double_p), ("params", ugrid_params), ("mco", c_double_p),
+                ("cut", C.POINTER(ugrid_cut_view))]
+
+
+class ugrid_axes(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("xf", c_double_p), ("xc", c_double_p), ("dx", c_double_p),
+                ("yf", c_double_p), ("yc", c_double_p), ("dy", c_double_p)]
 
 
 class ugrid_meta(C.Structure):
@@ -308,6 +321,10 @@ def _bind_ugrid_api(lib):
         "ugrid_device_available": (C.c_int, [C.c_char_p, C.c_size_t]),
         "ugrid_profile": (C.c_int, [H, C.c_int32, c_double_p, C.c_char_p, C.c_size_t]),
         "ugrid_implicit_stats": (C.c_int, [H, C.POINTER(ugrid_implicit_info)]),
+        "ugrid_cut_build": (C.c_int, [C.POINTER(ugrid_axes), c_double_p, c_double_p, C.c_int32, c_i32_p,
+                                      C.POINTER(H), C.c_char_p, C.c_size_t]),
+        "ugrid_cut_get": (C.c_int, [H, C.POINTER(ugrid_cut_view)]),
+        "ugrid_cut_destroy": (None, [H]),
         "ugrid_solver2d_layout": (C.c_int, [C.POINTER(C.c_int64), C.c_int32]),
     }
     for name, (restype, argtypes) in sig.items():

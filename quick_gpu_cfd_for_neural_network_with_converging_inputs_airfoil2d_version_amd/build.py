@@ -28,7 +28,7 @@ OBJ = ROOT / "_build" / "obj"
 
 HOST_SOURCES = ["host/cutcell.cpp", "host/mesher.cpp", "host/viscous_prep.cpp", "host/mesh_capi.cpp",
                 "host/cfd_api.cpp", "host/steady.cpp", "host/batch_steady.cpp", "host/ugrid_axis.cpp",
-                "host/euler2d_api.cpp"]
+                "host/euler2d_api.cpp", "host/ugrid_cut.cpp"]
 HIP_SOURCES = ["hip/amr_kernels_f64.hip", "hip/amr_kernels_f32.hip", "hip/amr_device.hip", "hip/amr_session.hip",
                "hip/amr_batch.hip", "hip/ugrid.hip"]
 # Per-source extra flags: FP32 quotients and square roots of the FP32 kernels are not correctly

@@ -99,6 +99,25 @@ struct Grid {
   double* initial;           // per case: first defect norm of the physical step
   double pseudo0, rtol, atol;
   int maxiter;
+  // Cut cells (has_cut): areas, 0 / 1 / 2 flags, apertures, wall vectors, SRD counts and
+  // positions; SRD neighbourhoods as member lists, and the cells SRD rewrites with the
+  // neighbourhoods each belongs to (ascending), so the sums run in one fixed order.
+  int has_cut, nsmall, n_srd;
+  const double* vol;
+  const double* flag;
+  const double* axl;
+  const double* ayl;
+  const double* wax;
+  const double* way;
+  const double* nbrcount;
+  const double* smallpos;
+  const double* svhat;
+  const int* mem_off;
+  const int* mem_idx;
+  const int* srd_cell;
+  const int* grp_off;
+  const int* grp_idx;
+  double* Qh;  // ncase x nsmall x 4 neighbourhood averages
   Control* ctrl;
 };
 
@@ -352,7 +371,7 @@ __device__ void commit(const Grid& g) {
 //   MODE 4: R = L(Q), per case max over cells of max_v |dt R - (Q - U)| / max(1, |U|)
 //           (infinite for a non-finite defect), src/euler2d_cuda.cu:331-350
 // ---------------------------------------------------------------------------------------------
-template <int MODE>
+template <int MODE, bool CUT>
 __global__ __launch_bounds__(kThreads) void k_stage(Grid g, const double* __restrict__ rhs_in,
                                                      double* __restrict__ rhs_out) {
   Control* ctrl = g.ctrl;
@@ -399,32 +418,59 @@ __global__ __launch_bounds__(kThreads) void k_stage(Grid g, const double* __rest
     const double dt = MODE > 0 ? g.dt[icase] : 0.0;
 
     Prim wprev{};
-    Flux gprev{}, xres{};
+    Flux gprev{}, xres{};  // xres: x residual of a full cell, or ae FX_e - aw FX_w of a cut cell
+    double ay_prev = 1.0;  // aperture of the previous y face (cut grids)
     double q0 = 0, q1 = 0, q2 = 0, q3 = 0;  // conserved state of the owned cell of the previous row
     for (int j = j0 - 1; j <= jl + 1; ++j) {
       bool flip_v = false;
       const int jc = map_axis(j, g.ny, g.bc[2], g.bc[3], &flip_v);
       const size_t k = case_base + static_cast<size_t>(jc + kGhosts) * g.mx;
       const double c0 = in[k], c1 = in[k + plane], c2 = in[k + 2 * plane], c3 = in[k + 3 * plane];
-      Prim w = primitive(c0, c1, c2, c3, g.gamma);
-      if (flip_u) w.u = -w.u;
-      if (flip_v) w.v = -w.v;
+      Prim w;
+      if (CUT && g.vol[static_cast<size_t>(jc + kGhosts) * g.mx + ic + kGhosts] <= 0.0) {
+        w = Prim{1.0, 0.0, 0.0, 1.0};  // solid cell (never a ghost's source: the body keeps clear of the sides)
+      } else {
+        w = primitive(c0, c1, c2, c3, g.gamma);
+        if (flip_u) w.u = -w.u;
+        if (flip_v) w.v = -w.v;
+      }
       if (j >= j0) {
-        const Flux gf = face_flux<false>(g, mco, wprev, w);
+        // y face between rows j - 1 and j (closed faces carry no flux)
+        double ay = 1.0;
+        if (CUT && i >= 0 && i < g.nx) ay = g.ayl[static_cast<size_t>(j) * g.nx + i];
+        Flux gf{0.0, 0.0, 0.0, 0.0};
+        if (!CUT || ay > 0.0) gf = face_flux<false>(g, mco, wprev, w);
         if (j > j0 && owner) {
           // Residual and update of the owned cell of row j - 1.
           const double dyj = g.dy[j - 1 + kGhosts];
-          const double r0 = xres.m - (gf.m - gprev.m) / dyj;
-          const double r1 = xres.x - (gf.x - gprev.x) / dyj;
-          const double r2 = xres.y - (gf.y - gprev.y) / dyj;
-          const double r3 = xres.e - (gf.e - gprev.e) / dyj;
+          const size_t kc = static_cast<size_t>(j - 1 + kGhosts) * g.mx + i + kGhosts;
+          double r0, r1, r2, r3;
+          bool fluid = true, moved = false;
+          if (CUT && g.vol[kc] <= 0.0) {
+            fluid = false;
+            r0 = r1 = r2 = r3 = 0.0;
+          } else if (!CUT || g.flag[kc] == 0.0) {
+            r0 = xres.m - (gf.m - gprev.m) / dyj;
+            r1 = xres.x - (gf.x - gprev.x) / dyj;
+            r2 = xres.y - (gf.y - gprev.y) / dyj;
+            r3 = xres.e - (gf.e - gprev.e) / dyj;
+          } else {
+            const double iv = 1.0 / g.vol[kc], pw = wprev.p;
+            r0 = -iv * (xres.m + ay * gf.m - ay_prev * gprev.m);
+            r1 = -iv * (xres.x + ay * gf.x - ay_prev * gprev.x);
+            r2 = -iv * (xres.y + ay * gf.y - ay_prev * gprev.y);
+            r3 = -iv * (xres.e + ay * gf.e - ay_prev * gprev.e);
+            r1 -= iv * pw * g.wax[kc];
+            r2 -= iv * pw * g.way[kc];
+          }
+          if (CUT && MODE == 3) moved = g.smallpos[kc] >= 0.0 || g.nbrcount[kc] != 1.0;
           const size_t ko = case_base + static_cast<size_t>(j - 1 + kGhosts) * g.mx;
           if (MODE == 0 || MODE == 4) {
             out[ko] = r0;
             out[ko + plane] = r1;
             out[ko + 2 * plane] = r2;
             out[ko + 3 * plane] = r3;
-            if (MODE == 4 && dt > 0.0) {
+            if (MODE == 4 && dt > 0.0 && fluid) {
               const double r[4] = {r0, r1, r2, r3}, q[4] = {q0, q1, q2, q3};
               double norm = 0.0;
               for (int v = 0; v < 4; ++v) {
@@ -455,8 +501,9 @@ __global__ __launch_bounds__(kThreads) void k_stage(Grid g, const double* __rest
             out[ko + plane] = n1;
             out[ko + 2 * plane] = n2;
             out[ko + 3 * plane] = n3;
-            if (inadmissible(n0, n1, n2, n3, g.gamma)) wave_bad = true;
-            if (MODE == 3) {
+            if (fluid && inadmissible(n0, n1, n2, n3, g.gamma)) wave_bad = true;
+            // the next step's bound (cells that SRD rewrites report theirs after it)
+            if (MODE == 3 && fluid && !moved) {
               const double b = wave_bound(n0, n1, n2, n3, g, dxi, dyj);
               const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(b));
               bound = bits > bound ? bits : bound;
@@ -464,6 +511,7 @@ __global__ __launch_bounds__(kThreads) void k_stage(Grid g, const double* __rest
           }
         }
         gprev = gf;
+        ay_prev = ay;
       }
       if (j >= j0 && j <= jl) {
         // x faces of row j: lane k evaluates the face west of its cell.
@@ -473,9 +521,16 @@ __global__ __launch_bounds__(kThreads) void k_stage(Grid g, const double* __rest
         wl.v = shfl_up1(w.v);
         wl.p = shfl_up1(w.p);
         if (lane == 0) wl = w;
-        const Flux fw = face_flux<true>(g, mco, wl, w);
+        double ax = 1.0;
+        if (CUT && i >= 0 && i <= g.nx) ax = g.axl[static_cast<size_t>(j) * (g.nx + 1) + i];
+        Flux fw{0.0, 0.0, 0.0, 0.0};
+        if (!CUT || ax > 0.0) fw = face_flux<true>(g, mco, wl, w);
         const Flux fe{shfl_down1(fw.m), shfl_down1(fw.x), shfl_down1(fw.y), shfl_down1(fw.e)};
-        xres = Flux{-(fe.m - fw.m) / dxi, -(fe.x - fw.x) / dxi, -(fe.y - fw.y) / dxi, -(fe.e - fw.e) / dxi};
+        const double ae = CUT ? shfl_down1(ax) : 1.0;
+        if (CUT && owner && g.flag[static_cast<size_t>(j + kGhosts) * g.mx + i + kGhosts] != 0.0)
+          xres = Flux{ae * fe.m - ax * fw.m, ae * fe.x - ax * fw.x, ae * fe.y - ax * fw.y, ae * fe.e - ax * fw.e};
+        else
+          xres = Flux{-(fe.m - fw.m) / dxi, -(fe.x - fw.x) / dxi, -(fe.y - fw.y) / dxi, -(fe.e - fw.e) / dxi};
       }
       wprev = w;
       q0 = c0, q1 = c1, q2 = c2, q3 = c3;
@@ -517,6 +572,7 @@ __global__ __launch_bounds__(kThreads) void k_bound(Grid g, int per_case, int ga
        q += static_cast<size_t>(per_case) * blockDim.x) {
     const int j = static_cast<int>(q / g.nx), i = static_cast<int>(q - static_cast<size_t>(j) * g.nx);
     const size_t k = static_cast<size_t>(j + kGhosts) * g.mx + i + kGhosts;
+    if (g.has_cut && g.vol[k] <= 0.0) continue;
     const double b = wave_bound(u[k], u[k + g.plane], u[k + 2 * g.plane], u[k + 3 * g.plane], g, g.dx[i + kGhosts],
                                 g.dy[j + kGhosts]);
     const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(b));
@@ -580,7 +636,9 @@ __global__ __launch_bounds__(kThreads) void k_check_state(Grid g) {
        q += static_cast<size_t>(gridDim.x) * blockDim.x) {
     const size_t c = q / inner, ij = q - c * inner;
     const int j = static_cast<int>(ij / g.nx), i = static_cast<int>(ij - static_cast<size_t>(j) * g.nx);
-    const size_t k = c * 4 * g.plane + static_cast<size_t>(j + kGhosts) * g.mx + i + kGhosts;
+    const size_t kc = static_cast<size_t>(j + kGhosts) * g.mx + i + kGhosts;
+    if (g.has_cut && g.vol[kc] <= 0.0) continue;
+    const size_t k = c * 4 * g.plane + kc;
     if (inadmissible(u[k], u[k + g.plane], u[k + 2 * g.plane], u[k + 3 * g.plane], g.gamma)) bad = true;
   }
   if (__any(bad) && threadIdx.x % kLanes == 0) atomicOr(&g.ctrl->bad, 1);
@@ -782,11 +840,27 @@ __global__ __launch_bounds__(kThreads) void k_be_trial(Grid g) {
        q += static_cast<size_t>(gridDim.x) * blockDim.x) {
     const size_t cs = q / inner, ij = q - cs * inner;
     const int j = static_cast<int>(ij / g.nx), i = static_cast<int>(ij - static_cast<size_t>(j) * g.nx);
-    const size_t k = cs * 4 * pl + static_cast<size_t>(j + kGhosts) * g.mx + i + kGhosts;
+    const size_t kc = static_cast<size_t>(j + kGhosts) * g.mx + i + kGhosts;
+    if (g.has_cut && g.vol[kc] <= 0.0) continue;
+    const size_t k = cs * 4 * pl + kc;
     const double d = g.dt[cs];
     if (d > 0.0) {
-      const double rate = wave_bound(Q[k], Q[k + pl], Q[k + 2 * pl], Q[k + 3 * pl], g, g.dx[i + kGhosts],
-                                     g.dy[j + kGhosts]);
+      double rate;
+      if (!g.has_cut) {
+        rate = wave_bound(Q[k], Q[k + pl], Q[k + 2 * pl], Q[k + 3 * pl], g, g.dx[i + kGhosts], g.dy[j + kGhosts]);
+      } else {
+        // implicit_spectral_rate of a cell of a cut grid (src/implicit2d.h:33-43)
+        const double r = Q[k], mu = Q[k + pl], mv = Q[k + 2 * pl], E = Q[k + 3 * pl];
+        const double u = mu / r, v = mv / r;
+        const double p = pressure(r, mu, mv, E, g.gamma);
+        const double a = sqrt(g.gamma * p / r);
+        const size_t fe = static_cast<size_t>(j) * (g.nx + 1) + i, fn = static_cast<size_t>(j) * g.nx + i;
+        const double wx = g.wax[kc], wy = g.way[kc];
+        const double wall = sqrt(wx * wx + wy * wy);
+        rate = ((fabs(u) + a) * (g.axl[fe + 1] + g.axl[fe]) + (fabs(v) + a) * (g.ayl[fn + g.nx] + g.ayl[fn]) +
+                fabs(u * wx + v * wy) + a * wall) /
+               (2.0 * g.vol[kc]);
+      }
       const double omega = pseudo / (pseudo + d * rate);
       for (int v = 0; v < 4; ++v) {
         const size_t kv = k + v * pl;
@@ -819,6 +893,74 @@ __global__ __launch_bounds__(kLanes) void k_be_check(Grid g) {
     c->phase = PH_RES;
   }
   c->events += 1;
+}
+
+// ------------------------------------------------------------------------------- SRD --
+// State redistribution of the stage's candidate (src/euler2d_cuda.cu:316-368, 461-468), when it is
+// admissible: neighbourhood averages over the members, then every rewritten cell sums its own
+// value (regular cells) and the averages of the neighbourhoods it belongs to, in ascending
+// order, and divides by its count.  MODE picks the candidate buffer as the stage kernels do.
+template <int MODE>
+__device__ __forceinline__ double* stage_out(const Grid& g, int cur) {
+  return MODE == 2 ? g.B[2] : g.B[cur ^ 1];
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_srd_avg(Grid g) {
+  const Control* c = g.ctrl;
+  if (__hip_atomic_load(&c->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != RUNNING ||
+      __hip_atomic_load(&c->bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+    return;
+  const double* u = stage_out<MODE>(g, c->cur);
+  const size_t n = static_cast<size_t>(g.ncase) * g.nsmall;
+  for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const size_t icase = t / g.nsmall;
+    const int q = static_cast<int>(t - icase * g.nsmall);
+    const double* uc = u + icase * 4 * g.plane;
+    for (int v = 0; v < 4; ++v) {
+      double sum = 0.0;
+      for (int m = g.mem_off[q]; m < g.mem_off[q + 1]; ++m) {
+        const int r = g.mem_idx[m];
+        sum += g.vol[r] / g.nbrcount[r] * uc[v * g.plane + r];
+      }
+      g.Qh[t * 4 + v] = sum / g.svhat[q];
+    }
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void k_srd_apply(Grid g) {
+  Control* c = g.ctrl;
+  if (__hip_atomic_load(&c->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != RUNNING ||
+      __hip_atomic_load(&c->bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+    return;
+  double* u = stage_out<MODE>(g, c->cur);
+  const size_t n = static_cast<size_t>(g.ncase) * g.n_srd;
+  bool bad = false;
+  for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const size_t icase = t / g.n_srd;
+    const int e = static_cast<int>(t - icase * g.n_srd);
+    const int k = g.srd_cell[e];
+    double* uc = u + icase * 4 * g.plane;
+    const double* qh = g.Qh + icase * g.nsmall * 4;
+    double x[4];
+    for (int v = 0; v < 4; ++v) {
+      double acc = g.smallpos[k] < 0.0 ? uc[v * g.plane + k] : 0.0;
+      for (int m = g.grp_off[e]; m < g.grp_off[e + 1]; ++m) acc += qh[g.grp_idx[m] * 4 + v];
+      x[v] = acc / g.nbrcount[k];
+      uc[v * g.plane + k] = x[v];
+    }
+    if (inadmissible(x[0], x[1], x[2], x[3], g.gamma)) bad = true;
+    if (MODE == 3) {
+      const int j = k / g.mx - kGhosts, i = k % g.mx - kGhosts;
+      const double b = wave_bound(x[0], x[1], x[2], x[3], g, g.dx[i + kGhosts], g.dy[j + kGhosts]);
+      const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(b));
+      if (bits) atomicMax(&g.smax[icase], bits);
+    }
+  }
+  if (__any(bad) && threadIdx.x % kLanes == 0) atomicOr(&c->bad, 1);
 }
 
 __global__ void k_zero(double* p, size_t n) {
@@ -932,7 +1074,28 @@ void launch_bound(ugrid_session* s) {
 }
 
 int grid_blocks(const Grid& g, size_t items) {
-  return static_cast<int>(std::min<size_t>((items + kThreads - 1) / kThreads, 8192));
+  return static_cast<int>(std::max<size_t>(1, std::min<size_t>((items + kThreads - 1) / kThreads, 8192)));
+}
+
+// A stage kernel in the regular or the cut-grid instantiation.
+template <int MODE>
+void launch_stage(ugrid_session* s, hipStream_t stream, const double* in = nullptr, double* out = nullptr) {
+  if (s->g.has_cut)
+    hipLaunchKernelGGL((k_stage<MODE, true>), dim3(s->launch_blocks), dim3(kThreads), 0, stream, s->g, in, out);
+  else
+    hipLaunchKernelGGL((k_stage<MODE, false>), dim3(s->launch_blocks), dim3(kThreads), 0, stream, s->g, in, out);
+}
+
+// A stage kernel followed, on a cut grid with small cells, by the redistribution of its candidate.
+template <int MODE>
+void launch_stage_srd(ugrid_session* s, hipStream_t stream) {
+  launch_stage<MODE>(s, stream);
+  const Grid& g = s->g;
+  if (!g.has_cut || g.nsmall == 0) return;
+  hipLaunchKernelGGL(k_srd_avg<MODE>, dim3(grid_blocks(g, static_cast<size_t>(g.ncase) * g.nsmall)), dim3(kThreads), 0,
+                     stream, g);
+  hipLaunchKernelGGL(k_srd_apply<MODE>, dim3(grid_blocks(g, static_cast<size_t>(g.ncase) * g.n_srd)),
+                     dim3(kThreads), 0, stream, g);
 }
 
 // One backward-Euler replay: every phase kernel once, in phase order.
@@ -945,7 +1108,7 @@ int launch_be_round(ugrid_session* s, hipStream_t stream) {
   hipLaunchKernelGGL(k_bound, dim3(per_case * g.ncase), dim3(kThreads), 0, stream, g, per_case, PH_BOUND);
   hipLaunchKernelGGL(k_be_open, dim3(1), dim3(kLanes), 0, stream, g);
   hipLaunchKernelGGL(k_be_copy, dim3(grid_blocks(g, 4 * cells)), dim3(kThreads), 0, stream, g);
-  hipLaunchKernelGGL(k_stage<4>, dim3(s->launch_blocks), dim3(kThreads), 0, stream, g, nullptr, nullptr);
+  launch_stage<4>(s, stream);
   hipLaunchKernelGGL(k_be_decide, dim3(1), dim3(kLanes), 0, stream, g);
   hipLaunchKernelGGL(k_be_trial, dim3(grid_blocks(g, cells)), dim3(kThreads), 0, stream, g);
   hipLaunchKernelGGL(k_be_check, dim3(1), dim3(kLanes), 0, stream, g);
@@ -953,9 +1116,9 @@ int launch_be_round(ugrid_session* s, hipStream_t stream) {
 }
 
 int launch_trial(ugrid_session* s, hipStream_t stream) {
-  hipLaunchKernelGGL(k_stage<1>, dim3(s->launch_blocks), dim3(kThreads), 0, stream, s->g, nullptr, nullptr);
-  hipLaunchKernelGGL(k_stage<2>, dim3(s->launch_blocks), dim3(kThreads), 0, stream, s->g, nullptr, nullptr);
-  hipLaunchKernelGGL(k_stage<3>, dim3(s->launch_blocks), dim3(kThreads), 0, stream, s->g, nullptr, nullptr);
+  launch_stage_srd<1>(s, stream);
+  launch_stage_srd<2>(s, stream);
+  launch_stage_srd<3>(s, stream);
   hipLaunchKernelGGL(k_commit, dim3(1), dim3(kLanes), 0, stream, s->g);
   return 4;
 }
@@ -983,6 +1146,85 @@ bool build_graph(ugrid_session* s, int trials, char* err, size_t len) {
   (void)hipGraphDestroy(graph);
   if (!ok(inst, err, len, "graph instantiate")) return false;
   s->graph_trials = trials;
+  return true;
+}
+
+// Cut-cell arrays of the device: per-cell data, apertures, and the SRD lists (members of each
+// neighbourhood; the cells SRD rewrites, each with its neighbourhoods in ascending order).
+bool upload_cut(ugrid_session* s, const ugrid_cut_view& c, char* err, size_t len) {
+  Grid& g = s->g;
+  const size_t cells = g.plane, nfx = static_cast<size_t>(g.nx + 1) * g.ny, nfy = static_cast<size_t>(g.nx) * (g.ny + 1);
+  std::vector<int> mem_off(1, 0), mem_idx;
+  std::vector<std::vector<int>> groups(cells);
+  for (int q = 0; q < c.nsmall; ++q) {
+    const int count = static_cast<int>(c.scnt[q]);
+    if (count < 1 || count > 25) {
+      fail(err, len, "ugrid_create: invalid SRD neighbourhood size");
+      return false;
+    }
+    for (int m = 0; m < count; ++m) {
+      const double r = c.smem[25 * static_cast<size_t>(q) + m];
+      if (!(r >= 0.0) || r >= static_cast<double>(cells)) {
+        fail(err, len, "ugrid_create: invalid SRD member");
+        return false;
+      }
+      mem_idx.push_back(static_cast<int>(r));
+      groups[static_cast<size_t>(r)].push_back(q);
+    }
+    mem_off.push_back(static_cast<int>(mem_idx.size()));
+  }
+  std::vector<int> srd_cell, grp_off(1, 0), grp_idx;
+  for (int j = 0; j < g.ny; ++j)
+    for (int i = 0; i < g.nx; ++i) {
+      const size_t k = static_cast<size_t>(j + kGhosts) * g.mx + i + kGhosts;
+      if (c.vol[k] <= 0.0 || (c.smallpos[k] < 0.0 && c.nbrcount[k] == 1.0)) continue;
+      srd_cell.push_back(static_cast<int>(k));
+      grp_idx.insert(grp_idx.end(), groups[k].begin(), groups[k].end());
+      grp_off.push_back(static_cast<int>(grp_idx.size()));
+    }
+  g.has_cut = 1;
+  g.nsmall = c.nsmall;
+  g.n_srd = static_cast<int>(srd_cell.size());
+  double *vol, *flag, *axl, *ayl, *wax, *way, *nbr, *pos, *svhat, *qh;
+  int *moff, *midx, *scell, *goff, *gidx;
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    return ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream), err, len, "upload cut geometry");
+  };
+  const size_t nq = static_cast<size_t>(std::max(c.nsmall, 1));
+  if (!device_alloc(s, &vol, cells, err, len) || !device_alloc(s, &flag, cells, err, len) ||
+      !device_alloc(s, &axl, nfx, err, len) || !device_alloc(s, &ayl, nfy, err, len) ||
+      !device_alloc(s, &wax, cells, err, len) || !device_alloc(s, &way, cells, err, len) ||
+      !device_alloc(s, &nbr, cells, err, len) || !device_alloc(s, &pos, cells, err, len) ||
+      !device_alloc(s, &svhat, nq, err, len) || !device_alloc(s, &qh, nq * 4 * g.ncase, err, len) ||
+      !device_alloc(s, &moff, mem_off.size(), err, len) || !device_alloc(s, &midx, std::max<size_t>(mem_idx.size(), 1), err, len) ||
+      !device_alloc(s, &scell, std::max<size_t>(srd_cell.size(), 1), err, len) ||
+      !device_alloc(s, &goff, grp_off.size(), err, len) || !device_alloc(s, &gidx, std::max<size_t>(grp_idx.size(), 1), err, len))
+    return false;
+  const size_t d = sizeof(double), n = sizeof(int);
+  if (!up(vol, c.vol, cells * d) || !up(flag, c.flag, cells * d) || !up(axl, c.axl, nfx * d) ||
+      !up(ayl, c.ayl, nfy * d) || !up(wax, c.wax, cells * d) || !up(way, c.way, cells * d) ||
+      !up(nbr, c.nbrcount, cells * d) || !up(pos, c.smallpos, cells * d) ||
+      (c.nsmall > 0 && !up(svhat, c.svhat, c.nsmall * d)) || !up(moff, mem_off.data(), mem_off.size() * n) ||
+      (!mem_idx.empty() && !up(midx, mem_idx.data(), mem_idx.size() * n)) ||
+      (!srd_cell.empty() && !up(scell, srd_cell.data(), srd_cell.size() * n)) ||
+      !up(goff, grp_off.data(), grp_off.size() * n) || (!grp_idx.empty() && !up(gidx, grp_idx.data(), grp_idx.size() * n)) ||
+      !ok(hipStreamSynchronize(s->stream), err, len, "upload cut geometry"))
+    return false;
+  g.vol = vol;
+  g.flag = flag;
+  g.axl = axl;
+  g.ayl = ayl;
+  g.wax = wax;
+  g.way = way;
+  g.nbrcount = nbr;
+  g.smallpos = pos;
+  g.svhat = svhat;
+  g.Qh = qh;
+  g.mem_off = moff;
+  g.mem_idx = midx;
+  g.srd_cell = scell;
+  g.grp_off = goff;
+  g.grp_idx = gidx;
   return true;
 }
 
@@ -1015,6 +1257,38 @@ extern "C" int ugrid_create(const ugrid_setup* setup, const double* u, uint64_t 
     if (p.bc[k] < 0 || p.bc[k] > 2) return fail(err, err_len, "ugrid_create: boundary kinds are 0, 1 or 2");
   if ((p.bc[0] == BC_PERIODIC) != (p.bc[1] == BC_PERIODIC) || (p.bc[2] == BC_PERIODIC) != (p.bc[3] == BC_PERIODIC))
     return fail(err, err_len, "ugrid_create: periodic boundary must be set on both sides of an axis");
+  if (setup->cut) {
+    const ugrid_cut_view& c = *setup->cut;
+    if (c.nbad_nbhd != 0)
+      return fail(err, err_len, "ugrid_create: the device backend refuses " + std::to_string(c.nbad_nbhd) +
+                                    " invalid SRD neighbourhoods");
+    if (c.nsmall < 0 || !c.vol || !c.flag || !c.axl || !c.ayl || !c.wax || !c.way || !c.nbrcount || !c.smallpos ||
+        (c.nsmall > 0 && (!c.scnt || !c.smem || !c.svhat)))
+      return fail(err, err_len, "ugrid_create: incomplete cut-cell geometry");
+    // The stage kernels take a ghost cell's state from the interior cell its boundary maps to,
+    // which holds for the ghosts' own fluid volumes only while the body keeps three cells
+    // clear of every side (as the geometry builders enforce).
+    const int nx = setup->nx, ny = setup->ny;
+    for (int j = -kGhosts; j < ny + kGhosts; ++j)
+      for (int i = -kGhosts; i < nx + kGhosts; ++i) {
+        const size_t k = static_cast<size_t>(j + kGhosts) * (nx + 2 * kGhosts) + (i + kGhosts);
+        const bool near = i < 3 || i >= nx - 3 || j < 3 || j >= ny - 3;
+        if (near && (c.flag[k] != 0.0 || !(c.vol[k] > 0.0)))
+          return fail(err, err_len, "ugrid_create: cut-cell geometry within three cells of the domain boundary (cell " +
+                                        std::to_string(i) + "," + std::to_string(j) + ")");
+      }
+    const double cells = static_cast<double>(nx + 2 * kGhosts) * (ny + 2 * kGhosts);
+    for (int q = 0; q < c.nsmall; ++q) {
+      const double count = c.scnt[q];
+      if (!(count >= 1.0 && count <= 25.0) || count != static_cast<int>(count))
+        return fail(err, err_len, "ugrid_create: invalid SRD neighbourhood size");
+      for (int m = 0; m < static_cast<int>(count); ++m) {
+        const double r = c.smem[25 * static_cast<size_t>(q) + m];
+        if (!(r >= 0.0 && r < cells) || r != static_cast<double>(static_cast<long long>(r)))
+          return fail(err, err_len, "ugrid_create: invalid SRD member");
+      }
+    }
+  }
   const long long mx = setup->nx + 2LL * kGhosts, my = setup->ny + 2LL * kGhosts;
   if (mx * my > (1LL << 40) / setup->ncase) return fail(err, err_len, "ugrid_create: grid too large");
   const size_t states = static_cast<size_t>(setup->ncase) * 4 * static_cast<size_t>(mx * my);
@@ -1104,6 +1378,10 @@ extern "C" int ugrid_create(const ugrid_setup* setup, const double* u, uint64_t 
   }
   s->uploaded.assign(u, u + states);
   s->mco = mco;
+  if (setup->cut && !upload_cut(s, *setup->cut, err, err_len)) {
+    release(s);
+    return 3;
+  }
   const int trials = g.nx * static_cast<long long>(g.ny) * g.ncase >= (1 << 22) ? 4 : 16;
   if (!build_graph(s, trials, err, err_len)) {
     release(s);
@@ -1240,7 +1518,7 @@ extern "C" int ugrid_rhs(ugrid_session* s, const double* u, uint64_t u_len, doub
   hipLaunchKernelGGL(k_zero, dim3(1024), dim3(kThreads), 0, s->stream, s->R, s->states);
   if (!ok(hipMemcpyAsync(s->rhs_in, u, bytes, hipMemcpyHostToDevice, s->stream), err, err_len, "upload state"))
     return 3;
-  hipLaunchKernelGGL(k_stage<0>, dim3(s->launch_blocks), dim3(kThreads), 0, s->stream, s->g, s->rhs_in, s->R);
+  launch_stage<0>(s, s->stream, s->rhs_in, s->R);
   if (!ok(hipGetLastError(), err, err_len, "residual launch") ||
       !ok(hipMemcpyAsync(r, s->R, bytes, hipMemcpyDeviceToHost, s->stream), err, err_len, "download residual") ||
       !ok(hipStreamSynchronize(s->stream), err, err_len, "residual"))
@@ -1276,11 +1554,11 @@ extern "C" int ugrid_profile(ugrid_session* s, int32_t trials, double* stage_us,
   for (int k = 0; good && k < trials; ++k) {
     hipEvent_t* e = &ev[4 * static_cast<size_t>(k)];
     (void)hipEventRecord(e[0], s->stream);
-    hipLaunchKernelGGL(k_stage<1>, dim3(s->launch_blocks), dim3(kThreads), 0, s->stream, g, nullptr, nullptr);
+    launch_stage_srd<1>(s, s->stream);
     (void)hipEventRecord(e[1], s->stream);
-    hipLaunchKernelGGL(k_stage<2>, dim3(s->launch_blocks), dim3(kThreads), 0, s->stream, g, nullptr, nullptr);
+    launch_stage_srd<2>(s, s->stream);
     (void)hipEventRecord(e[2], s->stream);
-    hipLaunchKernelGGL(k_stage<3>, dim3(s->launch_blocks), dim3(kThreads), 0, s->stream, g, nullptr, nullptr);
+    launch_stage_srd<3>(s, s->stream);
     (void)hipEventRecord(e[3], s->stream);
     hipLaunchKernelGGL(k_commit, dim3(1), dim3(kLanes), 0, s->stream, g);
   }

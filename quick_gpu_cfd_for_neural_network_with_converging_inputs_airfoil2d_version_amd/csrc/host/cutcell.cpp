@@ -74,6 +74,8 @@ inline bool solid_on_line(const Polygon& b, int axis, double c, double t) {
   return axis == 0 ? inside_polygon(b, c, t) : inside_polygon(b, t, c);
 }
 
+}  // namespace
+
 bool clip_to_rect(double* ax, double* ay, double* bx, double* by, double x0, double x1, double y0,
                   double y1) {
   double t0 = 0.0, t1 = 1.0;
@@ -116,10 +118,17 @@ void fluid_normal(const Polygon& b, int a, double mx, double my, double eps, dou
   *ny = uy;
 }
 
-}  // namespace
-
 void open_aperture(const Polygon& b, int axis, double c, double lo, double hi, double* length,
                    double* centroid) {
+  open_aperture_raw(b, axis, c, lo, hi, length, centroid);
+  // Snap near-full and near-empty apertures (src/cutcell2d.cpp:120-121).
+  const double len = *length;
+  if (std::fabs(len - (hi - lo)) < 1e-13 * (hi - lo)) *length = hi - lo;
+  if (len < 1e-13 * (hi - lo)) *length = 0.0;
+}
+
+void open_aperture_raw(const Polygon& b, int axis, double c, double lo, double hi, double* length,
+                       double* centroid) {
   std::vector<double> cuts;
   cuts.push_back(lo);
   for (int i = 0, j = b.n - 1; i < b.n; j = i++) {
@@ -143,9 +152,6 @@ void open_aperture(const Polygon& b, int axis, double c, double lo, double hi, d
     }
   }
   *centroid = (len > 0.0) ? moment / len : 0.5 * (lo + hi);
-  // Snap near-full and near-empty apertures (src/cutcell2d.cpp:120-121).
-  if (std::fabs(len - (hi - lo)) < 1e-13 * (hi - lo)) len = hi - lo;
-  if (len < 1e-13 * (hi - lo)) len = 0.0;
   *length = len;
 }
 

@@ -37,6 +37,16 @@ bool inside_polygon(const Polygon& body, double x, double y);
 // Fluid aperture of the segment {axis-coordinate = c, lo < t < hi}; length 0 when closed.
 void open_aperture(const Polygon& body, int axis, double c, double lo, double hi, double* length,
                    double* centroid);
+// The same without snapping near-full / near-empty lengths (src/cutcell2d.cpp:49-77).
+void open_aperture_raw(const Polygon& body, int axis, double c, double lo, double hi, double* length,
+                       double* centroid);
+// Liang-Barsky clip of the segment to the rectangle (src/cutcell2d.cpp:79-103); false when
+// nothing of parameter length above 1e-14 remains.
+bool clip_to_rect(double* ax, double* ay, double* bx, double* by, double x0, double x1, double y0,
+                  double y1);
+// Unit normal of edge a -> a + 1 pointing into the fluid, decided at (mx, my) + eps * normal
+// (src/cutcell2d.cpp:105-115).
+void fluid_normal(const Polygon& body, int a, double mx, double my, double eps, double* nx, double* ny);
 // Fluid sub-intervals of the same segment.
 std::vector<Point> open_intervals(const Polygon& body, int axis, double c, double lo, double hi);
 // Distance from the rectangle to the polygon boundary (0 when an edge enters it).

@@ -38,6 +38,15 @@ struct ImplicitView {
   long updates, retries;
   double initial_defect, final_defect, tolerance;
 };
+struct VectorView {  // std::vector (libstdc++): begin, end, capacity end
+  void* p[3];
+};
+struct CutView {  // src/euler2d.h:97-116
+  VectorView wall_fragments, wall_cells;
+  int nsmall, nbad_nbhd, nsolid, ncut;
+  ArrayView vol, cx, cy, flag, axl, axy, ayl, ayx, wax, way, wcx, wcy, nbrcount, smallpos, sidx, scnt, smem, svhat,
+      sxhat, syhat;
+};
 struct SolverView {
   GridView grid;
   cfd::Params2D par;
@@ -49,6 +58,7 @@ struct SolverView {
   const void* cut;
 };
 static_assert(sizeof(ArrayView) == 16 && sizeof(GridView) == 152 && sizeof(SolverView) == 560, "Solver2D layout");
+static_assert(sizeof(CutView) == 384, "Cut2D layout");
 
 }  // namespace
 
@@ -81,8 +91,9 @@ int cfd::solver2d_run_cuda(Solver2D* solver, double tend, long maxsteps) {
     std::fprintf(stderr, "CUDA backend supports regular grids and cut-cell+SRD, not the legacy IB path\n");
     return 3;
   }
-  if (s.cut != nullptr) {
-    std::fprintf(stderr, "HIP backend: cut-cell grids are not implemented on the device\n");
+  const CutView* cut = static_cast<const CutView*>(s.cut);
+  if (cut != nullptr && cut->nbad_nbhd != 0) {
+    std::fprintf(stderr, "CUDA backend refuses %d invalid SRD neighborhoods\n", cut->nbad_nbhd);
     return 3;
   }
   const GridView& g = s.grid;
@@ -113,6 +124,43 @@ int cfd::solver2d_run_cuda(Solver2D* solver, double tend, long maxsteps) {
   p.implicit_rtol = s.par.implicit_rtol;
   p.implicit_atol = s.par.implicit_atol;
   setup.mco = s.mco.p;
+  ugrid_cut_view cv{};
+  if (cut != nullptr) {
+    const size_t cells = static_cast<size_t>(g.mx) * g.my;
+    const size_t nfx = static_cast<size_t>(g.nx + 1) * g.ny, nfy = static_cast<size_t>(g.nx) * (g.ny + 1);
+    const size_t ns = static_cast<size_t>(cut->nsmall);
+    if (cut->vol.n != cells || cut->flag.n != cells || cut->wax.n != cells || cut->way.n != cells ||
+        cut->nbrcount.n != cells || cut->smallpos.n != cells || cut->axl.n != nfx || cut->ayl.n != nfy ||
+        cut->scnt.n != ns || cut->svhat.n != ns || cut->smem.n != 25 * ns) {
+      std::fprintf(stderr, "HIP backend: cut-cell arrays do not match the grid\n");
+      return 3;
+    }
+    cv.nsmall = cut->nsmall;
+    cv.nbad_nbhd = cut->nbad_nbhd;
+    cv.nsolid = cut->nsolid;
+    cv.ncut = cut->ncut;
+    cv.vol = cut->vol.p;
+    cv.cx = cut->cx.p;
+    cv.cy = cut->cy.p;
+    cv.flag = cut->flag.p;
+    cv.axl = cut->axl.p;
+    cv.axy = cut->axy.p;
+    cv.ayl = cut->ayl.p;
+    cv.ayx = cut->ayx.p;
+    cv.wax = cut->wax.p;
+    cv.way = cut->way.p;
+    cv.wcx = cut->wcx.p;
+    cv.wcy = cut->wcy.p;
+    cv.nbrcount = cut->nbrcount.p;
+    cv.smallpos = cut->smallpos.p;
+    cv.sidx = cut->sidx.p;
+    cv.scnt = cut->scnt.p;
+    cv.smem = cut->smem.p;
+    cv.svhat = cut->svhat.p;
+    cv.sxhat = cut->sxhat.p;
+    cv.syhat = cut->syhat.p;
+    setup.cut = &cv;
+  }
 
   char err[512] = {0};
   const auto t0 = std::chrono::steady_clock::now();
@@ -160,10 +208,12 @@ int cfd::solver2d_run_cuda(Solver2D* solver, double tend, long maxsteps) {
   return status;
 }
 
-// Offsets of the Solver2D fields this library reads, for the layout check of the tests:
+// Offsets of the Solver2D and Cut2D fields this library reads, for the layout check of the tests:
 // sizes of Solver2D, Grid2D, Array, Params2D, ImplicitStats2D, then the offsets of grid,
 // grid.{nx, ng, mx, my, xmin, xf, yf, xc, dx, yc, dy}, par, ncase, U, U1, U2, R, W, SX, SY, FX,
-// FY, t, dt, mco, nsteps, nreject, implicit, ib, cut (36 values).
+// FY, t, dt, mco, nsteps, nreject, implicit, ib, cut (36 values), then sizeof(Cut2D) and the
+// offsets of nsmall, nbad_nbhd, vol, flag, axl, ayl, wax, way, nbrcount, smallpos, scnt, smem,
+// svhat (14 values).
 extern "C" int ugrid_solver2d_layout(int64_t* out, int32_t capacity) {
   const int64_t v[] = {
       sizeof(SolverView), sizeof(GridView), sizeof(ArrayView), sizeof(cfd::Params2D), sizeof(ImplicitView),
@@ -175,7 +225,10 @@ extern "C" int ugrid_solver2d_layout(int64_t* out, int32_t capacity) {
       offsetof(SolverView, SY), offsetof(SolverView, FX), offsetof(SolverView, FY), offsetof(SolverView, t),
       offsetof(SolverView, dt), offsetof(SolverView, mco), offsetof(SolverView, nsteps),
       offsetof(SolverView, nreject), offsetof(SolverView, implicit), offsetof(SolverView, ib),
-      offsetof(SolverView, cut)};
+      offsetof(SolverView, cut), sizeof(CutView), offsetof(CutView, nsmall), offsetof(CutView, nbad_nbhd),
+      offsetof(CutView, vol), offsetof(CutView, flag), offsetof(CutView, axl), offsetof(CutView, ayl),
+      offsetof(CutView, wax), offsetof(CutView, way), offsetof(CutView, nbrcount), offsetof(CutView, smallpos),
+      offsetof(CutView, scnt), offsetof(CutView, smem), offsetof(CutView, svhat)};
   const int32_t n = static_cast<int32_t>(sizeof(v) / sizeof(v[0]));
   for (int32_t k = 0; k < n && k < capacity; ++k) out[k] = v[k];
   return n;

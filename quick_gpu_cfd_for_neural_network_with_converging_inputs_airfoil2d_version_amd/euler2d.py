@@ -3,13 +3,15 @@ src/euler2d.h:1-225) over the HIP backend of include/ugrid_hip.h.
 
 Mirrors the reference interface: `params2d_default`, `grid2d_stretch_faces`, `Solver2D`
 (`solver2d_init`'s grid and arrays, `set_primitive`, `get_primitive`, `set_mco`, `integral`,
-`u2idx`), `solver2d_run_cuda` / `solver2d_run_backend` / `solver2d_cuda_available` with the
-reference's return codes (0 every case at tend, 1 maxsteps, 2 nonphysical, 3 unsupported or
-unavailable, 4 backward-Euler step not converged).  The device run covers what the reference's
-GPU backend covers on a regular grid: FP64, first order (limiter 3), HLLC / AUSM+-up, SSP-RK3
-or backward Euler by pseudo-time defect relaxation, per-case time, step and AUSM+-up reference
-Mach number.  There is no CPU solver here: backend 0 is the
-reference's own CPU code and is refused.
+`u2idx`), `cut2d_build_polygon` / `set_cut` (cut-cell geometry and SRD neighbourhoods of a
+solid polygon, built natively by ugrid_cut_build), `solver2d_run_cuda` / `solver2d_run_backend`
+/ `solver2d_cuda_available` with the reference's return codes (0 every case at tend, 1
+maxsteps, 2 nonphysical, 3 unsupported or unavailable, 4 backward-Euler step not converged).
+The device run covers what the reference's GPU backend covers: FP64, first order (limiter 3),
+HLLC / AUSM+-up, SSP-RK3 or backward Euler by pseudo-time defect relaxation, regular or
+cut-cell grids (state redistribution after every RK stage), per-case time, step and AUSM+-up
+reference Mach number.  There is no CPU solver here: backend 0 is the reference's own CPU code
+and is refused.
 """
 from __future__ import annotations
 
@@ -104,6 +106,61 @@ class Grid2D:
         return self.ny + 2 * self.ng
 
 
+_CUT_CELL = ("vol", "cx", "cy", "flag", "wax", "way", "wcx", "wcy", "nbrcount", "smallpos")
+_CUT_SMALL = ("sidx", "scnt", "svhat", "sxhat", "syhat")
+
+
+class Cut2D:
+    """Cut-cell geometry and SRD neighbourhoods of a solid polygon on a Solver2D's grid (the
+    reference's Cut2D, src/euler2d.h:97-116): numpy copies of every array and the C view the
+    device backend reads."""
+
+    def __init__(self, s: "Solver2D", px, py, patches=None):
+        g = s.grid
+        px = np.ascontiguousarray(px, dtype=np.float64)
+        py = np.ascontiguousarray(py, dtype=np.float64)
+        pat = None if patches is None else np.ascontiguousarray(patches, dtype=np.int32)
+        keep = [np.ascontiguousarray(a) for a in (g.xf, g.xc, g.dx, g.yf, g.yc, g.dy)]
+        axes = N.ugrid_axes(g.nx, g.ny, *[N.dptr(a) for a in keep])
+        lib = N.load_library()
+        handle = C.c_void_p()
+        err = N.err_buffer()
+        self.status = lib.ugrid_cut_build(C.byref(axes), N.dptr(px), N.dptr(py), len(px),
+                                          None if pat is None else pat.ctypes.data_as(N.c_i32_p), C.byref(handle),
+                                          err, N.ERR_LEN)
+        if self.status:
+            raise ValueError(f"cut2d_build_polygon failed ({self.status}): {err.value.decode()}")
+        try:
+            v = N.ugrid_cut_view()
+            lib.ugrid_cut_get(handle, C.byref(v))
+            cells, nfx, nfy = g.mx * g.my, (g.nx + 1) * g.ny, g.nx * (g.ny + 1)
+            self.nsmall, self.nbad_nbhd, self.nsolid, self.ncut = v.nsmall, v.nbad_nbhd, v.nsolid, v.ncut
+            for name in _CUT_CELL:
+                setattr(self, name, N.as_array(getattr(v, name), cells, np.float64))
+            for name, count in (("axl", nfx), ("axy", nfx), ("ayl", nfy), ("ayx", nfy)):
+                setattr(self, name, N.as_array(getattr(v, name), count, np.float64))
+            for name in _CUT_SMALL:
+                setattr(self, name, N.as_array(getattr(v, name), v.nsmall, np.float64))
+            self.smem = N.as_array(v.smem, 25 * v.nsmall, np.float64)
+            self.walls = N.as_array(v.walls, v.n_walls, N.WALL_DTYPE)
+        finally:
+            lib.ugrid_cut_destroy(handle)
+
+    def view(self) -> N.ugrid_cut_view:
+        """The C view of these arrays (valid while this object lives)."""
+        v = N.ugrid_cut_view()
+        v.nsmall, v.nbad_nbhd, v.nsolid, v.ncut = self.nsmall, self.nbad_nbhd, self.nsolid, self.ncut
+        v.n_walls = 0
+        for name in _CUT_CELL + ("axl", "axy", "ayl", "ayx") + _CUT_SMALL + ("smem",):
+            setattr(v, name, N.dptr(getattr(self, name)))
+        return v
+
+
+def cut2d_build_polygon(s: "Solver2D", px, py, patches=None) -> Cut2D:
+    """src/euler2d.h:164-165: raises ValueError with the reference's code on failure."""
+    return Cut2D(s, px, py, patches)
+
+
 class Solver2D:
     """solver2d_init (src/euler2d.cpp:107-151): the grid of two stretched axes with two ghost
     layers, U[((case * 4 + var) * my + j + 2) * mx + i + 2] zeroed, per-case t, dt and mco."""
@@ -128,6 +185,11 @@ class Solver2D:
         # and the latest convergence check's largest first norm, norm and tolerance
         self.implicit = dict(updates=0, retries=0, initial_defect=0.0, final_defect=0.0, tolerance=0.0)
         self.last_error = ""
+        self.cut = None  # Cut2D (solver2d_set_cut)
+
+    def set_cut(self, cut: "Cut2D | None") -> None:
+        """solver2d_set_cut (src/euler2d.h:161-162)."""
+        self.cut = cut
 
     # -- field access (src/euler2d.h:137-140, src/euler2d.cpp:158-191) --
     def u2idx(self, c: int, v: int, i: int, j: int) -> int:
@@ -192,8 +254,10 @@ class Solver2D:
         dx = np.ascontiguousarray(g.dx)
         dy = np.ascontiguousarray(g.dy)
         mco = np.ascontiguousarray(self.mco, dtype=np.float64)
-        setup = N.ugrid_setup(g.nx, g.ny, self.ncase, 0, N.dptr(dx), N.dptr(dy), self.par.to_c(), N.dptr(mco))
-        return setup, (dx, dy, mco)
+        cut = self.cut.view() if self.cut is not None else None
+        setup = N.ugrid_setup(g.nx, g.ny, self.ncase, 0, N.dptr(dx), N.dptr(dy), self.par.to_c(), N.dptr(mco),
+                              C.pointer(cut) if cut is not None else None)
+        return setup, (dx, dy, mco, cut, self.cut)
 
     def device_session(self) -> "DeviceSession":
         return DeviceSession(self)
@@ -306,10 +370,18 @@ def solver2d_run_cuda(s: Solver2D, tend: float, maxsteps: int) -> int:
     else:
         ok, reason = solver2d_cuda_available()
         s.last_error = "" if ok else f"CUDA backend unavailable: {reason}"
+        if ok and s.cut is not None and s.cut.nbad_nbhd != 0:
+            s.last_error = f"CUDA backend refuses {s.cut.nbad_nbhd} invalid SRD neighborhoods"
     if s.last_error:
         print(s.last_error, file=sys.stderr)
         return 3
-    with DeviceSession(s) as session:
+    try:
+        session = DeviceSession(s)
+    except RuntimeError as exc:  # a setup the device backend refuses (nothing ran)
+        s.last_error = str(exc)
+        print(s.last_error, file=sys.stderr)
+        return 3
+    with session:
         status, message = session.run(tend, maxsteps)
         # The reference writes the state, clocks and steps back after every device run, a step
         # that cannot advance included (src/euler2d_cuda.cu:694-707); refused controls stop first.

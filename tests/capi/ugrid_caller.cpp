@@ -3,10 +3,13 @@
 // Solver2D the way solver2d_init does (src/euler2d.cpp:107-151; geometry from ugrid_axis), fills
 // one Riemann problem per case along x, and calls cfd::solver2d_run_cuda.
 //
-//   ugrid_caller layout                  print the 36 layout values of this header's Solver2D
+//   ugrid_caller layout                  print the 50 layout values of this header's Solver2D / Cut2D
 //   ugrid_caller probe                   print solver2d_cuda_available's answer
-//   ugrid_caller run OUT nx ny ncase flux bc0 bc1 bc2 bc3 stretch param tend maxsteps [limiter]
-//       write OUT: int64 status, nsteps, nreject; then the initial U, the final U, t, dt (doubles)
+//   ugrid_caller run OUT nx ny ncase flux bc0 bc1 bc2 bc3 stretch param tend maxsteps [limiter [R [be]]]
+//       write OUT: int64 status, nsteps, nreject; then the initial U, the final U, t, dt (doubles).
+//       R > 0: a solid disc of radius R (48 vertices) at (0.47, 0.53) as cut cells (Cut2D filled
+//       from ugrid_cut_build); be = 1: backward Euler.
+#include <cmath>
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
@@ -29,7 +32,10 @@ static void layout() {
       offsetof(Solver2D, W), offsetof(Solver2D, SX), offsetof(Solver2D, SY), offsetof(Solver2D, FX),
       offsetof(Solver2D, FY), offsetof(Solver2D, t), offsetof(Solver2D, dt), offsetof(Solver2D, mco),
       offsetof(Solver2D, nsteps), offsetof(Solver2D, nreject), offsetof(Solver2D, implicit), offsetof(Solver2D, ib),
-      offsetof(Solver2D, cut)};
+      offsetof(Solver2D, cut), sizeof(Cut2D), offsetof(Cut2D, nsmall), offsetof(Cut2D, nbad_nbhd),
+      offsetof(Cut2D, vol), offsetof(Cut2D, flag), offsetof(Cut2D, axl), offsetof(Cut2D, ayl), offsetof(Cut2D, wax),
+      offsetof(Cut2D, way), offsetof(Cut2D, nbrcount), offsetof(Cut2D, smallpos), offsetof(Cut2D, scnt),
+      offsetof(Cut2D, smem), offsetof(Cut2D, svhat)};
   for (int64_t x : v) std::printf("%lld\n", static_cast<long long>(x));
 }
 
@@ -59,7 +65,7 @@ int main(int argc, char** argv) {
   }
   if (argc < 15 || std::strcmp(argv[1], "run")) {
     std::fprintf(stderr, "usage: ugrid_caller layout | probe | run OUT nx ny ncase flux bc0 bc1 bc2 bc3 stretch param "
-                         "tend maxsteps [limiter]\n");
+                         "tend maxsteps [limiter [R [be]]]\n");
     return 2;
   }
   const char* out = argv[2];
@@ -83,6 +89,11 @@ int main(int argc, char** argv) {
   p.sy = {0, 0.0};
   const double tend = std::atof(argv[13]);
   const long maxsteps = std::atol(argv[14]);
+  const double radius = argc > 16 ? std::atof(argv[16]) : 0.0;
+  if (argc > 17 && std::atoi(argv[17]) == 1) {
+    p.time_integrator = 1;
+    p.cfl = 2.0;
+  }
 
   Grid2D& g = s.grid;
   g.nx = nx;
@@ -107,6 +118,56 @@ int main(int argc, char** argv) {
   s.mco.allocate(ncase);
   s.ib = nullptr;
   s.cut = nullptr;
+  Cut2D cut;
+  if (radius > 0.0) {
+    std::vector<double> px(48), py(48);
+    for (int k = 0; k < 48; ++k) {
+      const double a = 0.0131 + 2.0 * 3.141592653589793 * k / 48;
+      px[k] = 0.47 + radius * std::cos(a);
+      py[k] = 0.53 + radius * std::sin(a);
+    }
+    ugrid_axes axes{nx, ny, g.xf.data(), g.xc.data(), g.dx.data(), g.yf.data(), g.yc.data(), g.dy.data()};
+    ugrid_cut* built = nullptr;
+    char err[256];
+    if (ugrid_cut_build(&axes, px.data(), py.data(), 48, nullptr, &built, err, sizeof(err))) {
+      std::fprintf(stderr, "cut: %s\n", err);
+      return 2;
+    }
+    ugrid_cut_view v{};
+    ugrid_cut_get(built, &v);
+    const size_t cells = static_cast<size_t>(g.mx) * g.my, nfx = static_cast<size_t>(nx + 1) * ny,
+                 nfy = static_cast<size_t>(nx) * (ny + 1), ns = static_cast<size_t>(v.nsmall);
+    auto fill = [](Array& a, const double* src, size_t n) {
+      a.allocate(n);
+      for (size_t k = 0; k < n; ++k) a[k] = src[k];
+    };
+    cut.nsmall = v.nsmall;
+    cut.nbad_nbhd = v.nbad_nbhd;
+    cut.nsolid = v.nsolid;
+    cut.ncut = v.ncut;
+    fill(cut.vol, v.vol, cells);
+    fill(cut.cx, v.cx, cells);
+    fill(cut.cy, v.cy, cells);
+    fill(cut.flag, v.flag, cells);
+    fill(cut.axl, v.axl, nfx);
+    fill(cut.axy, v.axy, nfx);
+    fill(cut.ayl, v.ayl, nfy);
+    fill(cut.ayx, v.ayx, nfy);
+    fill(cut.wax, v.wax, cells);
+    fill(cut.way, v.way, cells);
+    fill(cut.wcx, v.wcx, cells);
+    fill(cut.wcy, v.wcy, cells);
+    fill(cut.nbrcount, v.nbrcount, cells);
+    fill(cut.smallpos, v.smallpos, cells);
+    fill(cut.sidx, v.sidx, ns);
+    fill(cut.scnt, v.scnt, ns);
+    fill(cut.smem, v.smem, 25 * ns);
+    fill(cut.svhat, v.svhat, ns);
+    fill(cut.sxhat, v.sxhat, ns);
+    fill(cut.syhat, v.syhat, ns);
+    ugrid_cut_destroy(built);
+    s.cut = &cut;
+  }
   // One Riemann problem per case (left state for x < 0.5), every case its own reference Mach.
   const double table[4][8] = {{1.0, 0.0, 0.0, 1.0, 0.125, 0.0, 0.0, 0.1},
                               {1.0, 0.75, 0.1, 1.0, 0.125, 0.0, -0.1, 0.1},

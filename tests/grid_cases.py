@@ -120,3 +120,44 @@ ALL_BCS = [
 ]
 STRETCHES = [(None, None), ((E.STRETCH_GEOMETRIC, 1.03), (E.STRETCH_TANH_ENDS, 1.5)),
              ((E.STRETCH_SINH_CENTER, 2.0), (E.STRETCH_GEOMETRIC, 0.97))]
+
+
+# ------------------------------------------------------------------ cut-cell bodies --
+def ellipse(cx, cy, rx, ry, n=64, phase=0.0137):
+    """Vertices of an ellipse, counter-clockwise (a phase keeps vertices off grid lines)."""
+    a = phase + 2.0 * math.pi * np.arange(n) / n
+    return cx + rx * np.cos(a), cy + ry * np.sin(a)
+
+
+def wedge():
+    """A triangle with one vertex pointing upstream."""
+    return np.array([-0.45, 0.357, 0.357]), np.array([0.013, -0.313, 0.341])
+
+
+BODIES = {
+    "circle": lambda: ellipse(0.1, 0.05, 0.33, 0.27),
+    "ellipse": lambda: ellipse(-0.07, 0.0, 0.52, 0.11, 96),
+    "wedge": wedge,
+}
+
+
+def cut_body(body="circle", nx=48, ny=40, par=None, states=((1.0, 0.6, 0.05, 1.0),), shock=None):
+    """Flow past a solid body on [-1, 1] x [-0.8, 0.8]: one case per uniform state (rho, u, v, p),
+    optionally with a normal shock upstream at x = shock (post-shock state left of it).  The
+    body's cut geometry is built and attached; returns (solver, px, py)."""
+    par = par or params(bc=(E.BC_TRANSMISSIVE, E.BC_TRANSMISSIVE, E.BC_REFLECT, E.BC_REFLECT))
+    s = E.Solver2D(nx, ny, -1.0, 1.0, -0.8, 0.8, len(states), par)
+    g = s.grid
+    X = np.broadcast_to(g.xc[2:2 + nx][None, :], (ny, nx))
+    for c, st in enumerate(states):
+        w = [np.full((ny, nx), float(st[k])) for k in range(4)]
+        if shock is not None:
+            pre = w[0], w[1], w[3]
+            left = X < shock
+            w[0] = np.where(left, 2.0 * pre[0], pre[0])
+            w[1] = np.where(left, 0.5 * pre[1] + 0.4, pre[1])
+            w[3] = np.where(left, 2.6 * pre[2], pre[2])
+        s.set_primitive_field(c, *w)
+    px, py = BODIES[body]()
+    s.set_cut(E.cut2d_build_polygon(s, px, py))
+    return s, px, py

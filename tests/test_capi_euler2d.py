@@ -4,9 +4,11 @@ cfd::solver2d_cuda_available) as a caller compiled against it sees it (tests/cap
 CPU: the Solver2D layout of the public header, of the library's private view and of the
 reference's own type (oracle/_ref) agree field for field; without a device the API reports
 unavailable and returns 3.  GPU: a run through the C++ API is bitwise the C ABI session's on the
-same initial state, and refusals return 3.
+same initial state (regular grids, and a Cut2D filled by the caller for SSP-RK3 and backward
+Euler), and refusals return 3.
 """
 import ctypes as C
+import math
 import subprocess
 from pathlib import Path
 
@@ -34,7 +36,7 @@ def library_layout():
 def test_layouts_agree():
     header = [int(x) for x in subprocess.run([str(CALLER), "layout"], capture_output=True, text=True,
                                              check=True).stdout.split()]
-    assert len(header) == 36
+    assert len(header) == 50
     assert header == library_layout()
     lib = O.ref_lib()
     if lib is None:
@@ -58,10 +60,10 @@ def test_unavailable_without_device():
     assert "status 3" in run.stdout and "unavailable" in run.stderr
 
 
-def caller_run(tmp_path, nx, ny, ncase, flux, bc, stretch, tend, maxsteps, limiter=3):
+def caller_run(tmp_path, nx, ny, ncase, flux, bc, stretch, tend, maxsteps, limiter=3, radius=0.0, be=0):
     out = tmp_path / "run.bin"
     args = [str(CALLER), "run", str(out), str(nx), str(ny), str(ncase), str(flux), *map(str, bc), str(stretch[0]),
-            str(stretch[1]), repr(tend), str(maxsteps), str(limiter)]
+            str(stretch[1]), repr(tend), str(maxsteps), str(limiter), repr(radius), str(be)]
     print(" ".join(args[2:]), flush=True)
     done = subprocess.run(args, capture_output=True, text=True, timeout=300)
     assert done.returncode == 0, done.stderr
@@ -106,3 +108,51 @@ def test_cpp_api_refusals(tmp_path):
     np.testing.assert_array_equal(res["u"], res["initial"])
     res = caller_run(tmp_path, 16, 4, 1, 0, (1, 1, 1, 1), (0, 0.0), 0.1, 0)
     assert res["status"] == 1 and res["nsteps"] == 0
+
+
+@pytest.mark.gpu
+@needs_caller
+@pytest.mark.parametrize("be, nx, ny, tend", [(0, 120, 90, 0.05), (1, 64, 48, 0.02)])
+def test_cpp_api_cut_run_bitwise(tmp_path, be, nx, ny, tend):
+    ncase, radius = 2, 0.21
+    res = caller_run(tmp_path, nx, ny, ncase, E.FLUX_HLLC, (1, 1, 2, 2), (0, 0.0), tend, 100000, radius=radius, be=be)
+    assert res["status"] == 0 and res["nsteps"] > 1, res["stderr"]
+    par = params(bc=(1, 1, 2, 2))
+    if be:
+        par.time_integrator = E.TIME_BACKWARD_EULER
+        par.cfl = 2.0
+    s = E.Solver2D(nx, ny, 0.0, 1.0, 0.0, 1.0, ncase, par)
+    s.U[:] = res["initial"]
+    s.mco[:] = [0.2 + 0.2 * c for c in range(ncase)]
+    # the caller's disc, vertex for vertex (libm's cos / sin, as the caller's)
+    a = [0.0131 + 2.0 * 3.141592653589793 * k / 48 for k in range(48)]
+    s.set_cut(E.cut2d_build_polygon(s, [0.47 + radius * math.cos(x) for x in a],
+                                    [0.53 + radius * math.sin(x) for x in a]))
+    assert s.cut.nsmall > 0
+    ora = (G.run_implicit if be else G.run)(s, tend, 100000)
+    np.testing.assert_array_equal(res["u"], ora["u"])
+    np.testing.assert_array_equal(res["t"], ora["t"])
+    np.testing.assert_array_equal(res["dt"], ora["dt"])
+    assert res["nsteps"] == ora["nsteps"]
+
+
+def test_cut_setup_refusals_before_any_device_work():
+    # ugrid_create validates the cut geometry before it touches a device
+    from .grid_cases import cut_body
+    s, px, py = cut_body("circle")
+    g = s.grid
+    k = (g.my - 3) * g.mx + g.mx // 2  # an interior cell two rows below the top side
+    s.cut.flag[k] = 1.0
+    with pytest.raises(RuntimeError, match="within three cells of the domain boundary"):
+        E.DeviceSession(s)
+    s.cut.flag[k] = 0.0
+    s.cut.vol[2] = 0.0  # a ghost cell
+    with pytest.raises(RuntimeError, match="within three cells"):
+        E.DeviceSession(s)
+    s.cut.vol[2] = s.grid.dx[2] * s.grid.dy[0]
+    s.cut.smem[0] = float(g.mx * g.my)
+    with pytest.raises(RuntimeError, match="invalid SRD member"):
+        E.DeviceSession(s)
+    s.cut.nbad_nbhd = 1
+    with pytest.raises(RuntimeError, match="refuses 1 invalid SRD neighbourhoods"):
+        E.DeviceSession(s)

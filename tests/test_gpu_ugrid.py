@@ -8,6 +8,8 @@ counts, return codes) on every boundary kind, both fluxes, stretched axes, sever
 their own reference Mach numbers, strip and row-band edges (grids not multiples of the 62-column
 strip or of the row band).  At sizes the oracle cannot follow, size-independent properties:
 conservation on a periodic box and the x / y mirror symmetry of a symmetric blast.
+Cut-cell grids (solid bodies with SRD redistribution after every stage) to the same bar, and
+at 1024 x 1024 mass and energy conservation with a body in a periodic box.
 """
 import numpy as np
 import pytest
@@ -16,7 +18,8 @@ from oracle import pygrid as G
 from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import _native as N
 from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import euler2d as E
 
-from .grid_cases import ALL_BCS, RIEMANN_SET, STRETCHES, blast, params, random_smooth, riemann, vortex
+from .grid_cases import (ALL_BCS, BODIES, RIEMANN_SET, STRETCHES, blast, cut_body, params, random_smooth, riemann,
+                         vortex)
 
 pytestmark = pytest.mark.gpu
 
@@ -280,3 +283,100 @@ def test_backward_euler_python_mirror():
     np.testing.assert_array_equal(s.t, ora["t"])
     assert s.nsteps == ora["nsteps"]
     assert s.implicit["updates"] == ora["updates"] and s.implicit["final_defect"] == ora["final_defect"]
+
+
+# ------------------------------------------------------------------ cut cells --
+@pytest.mark.parametrize("body", sorted(BODIES))
+@pytest.mark.parametrize("flux", [E.FLUX_HLLC, E.FLUX_AUSMPUP])
+def test_cut_residual_bitwise(body, flux):
+    s, px, py = cut_body(body, 150, 70, params(flux=flux, bc=ALL_BCS[4], sx=STRETCHES[1][0], sy=STRETCHES[1][1]),
+                         states=((1.0, 0.6, 0.05, 1.0), (1.3, -0.2, 0.4, 0.7), (0.8, 0.1, -0.3, 2.0)), shock=-0.4)
+    s.mco[:] = [0.3, 0.8, 0.05]
+    print(f"cut residual {body} flux={flux}", flush=True)
+    u = G.apply_bc(s, s.U)
+    with s.device_session() as d:
+        r = d.rhs(u)
+    np.testing.assert_array_equal(r, G.rhs(s, u))
+
+
+CUT_RUNS = {
+    "circle_shock_hllc": lambda: (cut_body("circle", shock=-0.6), 0.25),
+    # the first case finishes early and keeps receiving SRD on every later stage
+    "ellipse_two_cases_ausm": lambda: (cut_body("ellipse", 40, 36, params(flux=E.FLUX_AUSMPUP, bc=ALL_BCS[4]),
+                                                ((1.0, 0.5, 0.0, 1.0), (1.0, 0.2, 0.1, 1.5))), 0.2),
+    "wedge_periodic_stretched": lambda: (cut_body("wedge", 44, 38, params(bc=ALL_BCS[0], sx=(E.STRETCH_TANH_ENDS, 1.2))),
+                                         0.15),
+    "circle_strips_three_cases": lambda: (cut_body("circle", 200, 131, params(bc=ALL_BCS[4]),
+                                                   ((1.0, 0.6, 0.05, 1.0), (1.0, 1.5, 0.0, 1.0),
+                                                    (0.5, -0.3, 0.2, 0.4)), shock=-0.7), 0.005),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CUT_RUNS))
+def test_cut_run_bitwise(name):
+    (s, px, py), tend = CUT_RUNS[name]()
+    print(f"cut run {name} {s.grid.nx}x{s.grid.ny}x{s.ncase} nsmall={s.cut.nsmall}", flush=True)
+    ora = G.run(s, tend, 5000)
+    assert ora["status"] == 0 and ora["nsteps"] > 10
+    assert_same_run(device_run(s, tend, 5000), ora)
+
+
+def test_cut_run_split_calls_and_stops():
+    s, px, py = cut_body("circle", shock=-0.6)
+    print("cut split calls", flush=True)
+    ora = G.run(s, 0.25, 40)
+    dev = device_run(s, 0.25, 40, calls=4)
+    assert dev["statuses"] == [1, 1, 1, 1]
+    assert_same_run(dev, ora)
+    # an unstable fixed step fails after twelve retries with the state untouched
+    s.par.dt_fixed = 1e6
+    ora = G.run(s, 1e6, 5)
+    assert ora["status"] == 2 and ora["nreject"] == 13
+    assert_same_run(device_run(s, 1e6, 5), ora)
+
+
+def test_cut_backward_euler_bitwise():
+    s, px, py = cut_body("circle", 52, 40, implicit_params(bc=ALL_BCS[4]), shock=-0.6)
+    print("cut backward Euler", flush=True)
+    ora = G.run_implicit(s, 0.1, 100)
+    assert ora["status"] == 0 and ora["updates"] > ora["nsteps"] > 1
+    assert_same_implicit(device_implicit(s, 0.1, 100), ora)
+
+
+def test_cut_python_mirror_and_refusal():
+    s, px, py = cut_body("ellipse", 90, 64, params(flux=E.FLUX_AUSMPUP, bc=ALL_BCS[4]),
+                         ((1.0, 0.5, 0.0, 1.0), (1.0, 0.4, -0.1, 1.2)))
+    print("cut Python mirror", flush=True)
+    ora = G.run(s, 0.1, 5000)
+    assert E.solver2d_run_cuda(s, 0.1, 5000) == 0
+    np.testing.assert_array_equal(s.U, ora["u"])
+    np.testing.assert_array_equal(s.t, ora["t"])
+    assert (s.nsteps, s.nreject) == (ora["nsteps"], ora["nreject"])
+    # invalid SRD neighbourhoods are refused before anything runs
+    s.cut.nbad_nbhd = 2
+    before = s.U.copy()
+    assert E.solver2d_run_cuda(s, 0.2, 10) == 3
+    assert "invalid SRD" in s.last_error
+    np.testing.assert_array_equal(s.U, before)
+
+
+def test_cut_large_grid_conservation():
+    """Two cases of 1024 x 1024 flowing past a circle in a periodic box: total mass and energy
+    (cut cells weighted by their fluid area) are conserved; momentum is not (the wall)."""
+    s, px, py = cut_body("circle", 1024, 1024, params(bc=ALL_BCS[0]),
+                         ((1.0, 0.6, 0.05, 1.0), (1.0, 1.3, -0.2, 1.0)), shock=-0.6)
+    g, c = s.grid, s.cut
+    full = np.outer(g.dy, g.dx).ravel()
+    vol = np.where(c.flag == 1, c.vol, np.where(c.flag == 2, 0.0, full)).reshape(g.my, g.mx)[2:-2, 2:-2]
+
+    def sums():
+        return (s.U.reshape(s.ncase, 4, g.my, g.mx)[:, :, 2:-2, 2:-2] * vol).sum(axis=(2, 3))
+
+    before = sums()
+    print(f"cut large nsmall={c.nsmall}", flush=True)
+    assert E.solver2d_run_cuda(s, 0.02, 100000) == 0
+    assert s.nsteps > 20
+    after = sums()
+    np.testing.assert_allclose(after[:, [0, 3]], before[:, [0, 3]], rtol=1e-12)
+    assert np.all(np.abs(after[:, 1] - before[:, 1]) > 1e-6)
+    assert np.all(np.isfinite(s.U))

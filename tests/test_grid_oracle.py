@@ -12,6 +12,12 @@ the checker of the HIP ugrid path.
    antisymmetry, a uniform state's zero residual on every boundary kind, discrete conservation
    on a periodic box, x / y symmetry of a transposed problem, the 12-retry failure of an
    unstable fixed step, and the control semantics (maxsteps, done, per-case clocks).
+3. Cut cells (src/euler2d.cpp cut2d_build_polygon, src/euler2d_cuda.cu:182-258, 271-314): the
+   host cut builder (ugrid_cut_build) bit for bit against the reference's on three bodies; the
+   cut residual, RK3 + SRD runs and backward-Euler runs against the reference's CPU solver within
+   round-off; gas at rest stays at rest next to the wall (aperture closure, wall-pressure sign);
+   mass and energy conservation of RK3 + SRD on a periodic box; the transposed body's transposed
+   solution.
 A plausible slip (a dropped term, a wrong sign, a transposed index) fails at least one.
 """
 import math
@@ -22,7 +28,8 @@ import pytest
 from oracle import pygrid as G
 from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import euler2d as E
 
-from .grid_cases import ALL_BCS, RIEMANN_SET, STRETCHES, blast, params, random_smooth, riemann, vortex
+from .grid_cases import (ALL_BCS, BODIES, RIEMANN_SET, STRETCHES, blast, cut_body, params, random_smooth, riemann,
+                         vortex)
 
 needs_ref = pytest.mark.skipif(not G.ref_available(), reason="reference CPU build (oracle/_ref) not present")
 
@@ -366,3 +373,177 @@ def test_backward_euler_closed_forms():
     assert r["status"] == 3 and r["reason"] == 5
     assert r["dt"][0] > 0.0 and 0.0 < r["dt"][1] < 1.0 and r["dt"][2] == -1.0
     np.testing.assert_array_equal(r["t"], s.t)
+
+
+# ------------------------------------------------------------------ cut cells --
+CUT_CELL = ("vol", "cx", "cy", "flag", "wax", "way", "wcx", "wcy", "nbrcount", "smallpos")
+CUT_FACE = ("axl", "axy", "ayl", "ayx")
+CUT_SMALL = ("sidx", "scnt", "svhat", "sxhat", "syhat", "smem")
+
+
+@needs_ref
+@pytest.mark.parametrize("body", sorted(BODIES))
+@pytest.mark.parametrize("grid", [(48, 40, None, None), (37, 52, (E.STRETCH_GEOMETRIC, 1.02),
+                                                         (E.STRETCH_SINH_CENTER, 1.5))])
+def test_cut_builder_matches_reference_bitwise(body, grid):
+    nx, ny, sx, sy = grid
+    s, px, py = cut_body(body, nx, ny, params(bc=ALL_BCS[4], sx=sx, sy=sy))
+    ref = G.ref_cut_build(s, px, py)
+    assert ref["status"] == 0
+    c = s.cut
+    assert (c.nsmall, c.nbad_nbhd, c.nsolid, c.ncut) == (ref["nsmall"], ref["nbad_nbhd"], ref["nsolid"], ref["ncut"])
+    assert c.nsmall > 0 and c.nsolid > 0 and c.ncut > 0
+    for name in CUT_CELL + CUT_FACE + CUT_SMALL:
+        np.testing.assert_array_equal(getattr(c, name), ref[name], err_msg=name)
+    # wall records field by field (the C structs carry padding)
+    assert len(c.walls) == len(ref["walls"])
+    for name in c.walls.dtype.names:
+        np.testing.assert_array_equal(c.walls[name], ref["walls"][name], err_msg=name)
+
+
+def test_cut_builder_invariants():
+    s, px, py = cut_body("circle")
+    c, g = s.cut, s.grid
+    area = np.outer(g.dy, g.dx).ravel()
+    interior = np.zeros((g.my, g.mx), dtype=bool)
+    interior[2:-2, 2:-2] = True
+    interior = interior.ravel()
+    solid, cut = interior & (c.flag == 2), interior & (c.flag == 1)
+    assert solid.sum() == c.nsolid and cut.sum() == c.ncut
+    np.testing.assert_array_equal(c.vol[solid], 0.0)
+    assert np.all((c.vol[cut] > 0) & (c.vol[cut] < area[cut]))
+    # the fluid area is the box less the polygon's (shoelace) area
+    poly = 0.5 * abs(np.dot(px, np.roll(py, -1)) - np.dot(py, np.roll(px, -1)))
+    fluid = np.where(interior & (c.flag == 0), area, 0.0).sum() + c.vol[cut].sum()
+    assert abs(fluid - (2.0 * 1.6 - poly)) < 1e-12
+    # the wall vectors close every cut cell: open x apertures + open y apertures + wall = 0
+    j, i = np.divmod(np.flatnonzero(cut), g.mx)
+    e = (j - 2) * (g.nx + 1) + (i - 2)
+    n = (j - 2) * g.nx + (i - 2)
+    np.testing.assert_allclose(c.axl[e + 1] - c.axl[e] + c.wax[cut], 0.0, atol=1e-15)
+    np.testing.assert_allclose(c.ayl[n + g.nx] - c.ayl[n] + c.way[cut], 0.0, atol=1e-15)
+
+
+@needs_ref
+@pytest.mark.parametrize("body", sorted(BODIES))
+@pytest.mark.parametrize("flux", [E.FLUX_HLLC, E.FLUX_AUSMPUP])
+def test_cut_residual_matches_reference_within_roundoff(body, flux):
+    s, px, py = cut_body(body, par=params(flux=flux, bc=ALL_BCS[4]),
+                         states=((1.0, 0.6, 0.05, 1.0), (1.3, -0.2, 0.4, 0.7)), shock=-0.4)
+    s.mco[:] = [0.3, 0.8]
+    a = G.rhs(s, G.apply_bc(s, s.U))
+    b = G.ref_cut_rhs(s, px, py, G.apply_bc(s, s.U))
+    assert rel(interior(s, a), interior(s, b)) < 1e-13
+
+
+CUT_RUNS = {
+    "circle_shock_hllc": lambda: (cut_body("circle", shock=-0.6), 0.25),
+    "ellipse_two_cases_ausm": lambda: (cut_body("ellipse", 40, 36, params(flux=E.FLUX_AUSMPUP, bc=ALL_BCS[4]),
+                                                ((1.0, 0.5, 0.0, 1.0), (1.0, 0.2, 0.1, 1.5))), 0.2),
+    "wedge_periodic_stretched": lambda: (cut_body("wedge", 44, 38, params(bc=ALL_BCS[0], sx=(E.STRETCH_TANH_ENDS, 1.2))),
+                                         0.15),
+}
+
+
+@needs_ref
+@pytest.mark.parametrize("name", sorted(CUT_RUNS))
+def test_cut_run_matches_reference_within_roundoff(name):
+    (s, px, py), tend = CUT_RUNS[name]()
+    a, b = G.run(s, tend, 2000), G.ref_cut_run(s, px, py, tend, 2000)
+    assert a["status"] == b["status"] == 0
+    assert (a["nsteps"], a["nreject"]) == (b["nsteps"], b["nreject"])
+    np.testing.assert_array_equal(a["t"], b["t"])
+    # A case that reached tend before the last step: the GPU backend still applies SRD to it on
+    # every later stage (apply_srd covers all cases, src/euler2d_cuda.cu:410-424, 651-666) while
+    # the CPU solver leaves it alone, so it agrees with the CPU solver only outside the SRD
+    # neighbourhoods.
+    early = G.run(s, tend, a["nsteps"] - 1)["t"] >= tend
+    ua, ub = interior(s, a["u"]), interior(s, b["u"])
+    g = s.grid
+    untouched = ((s.cut.nbrcount == 1.0) & (s.cut.smallpos < 0.0)).reshape(g.my, g.mx)[2:-2, 2:-2]
+    for c in range(s.ncase):
+        if early[c]:
+            assert rel(ua[c][:, untouched], ub[c][:, untouched]) < 1e-12
+            assert rel(ua[c], ub[c]) > 1e-6
+        else:
+            np.testing.assert_allclose(a["dt"][c], b["dt"][c], rtol=1e-12)
+            assert rel(ua[c], ub[c]) < 1e-12
+    assert not early.all()
+
+
+@needs_ref
+def test_cut_backward_euler_matches_reference_within_roundoff():
+    s, px, py = cut_body("circle", 40, 32, implicit_params(bc=ALL_BCS[4]), shock=-0.6)
+    a, b = G.run_implicit(s, 0.1, 100), G.ref_cut_run(s, px, py, 0.1, 100)
+    assert a["status"] == b["status"] == 0
+    assert (a["nsteps"], a["updates"], a["retries"]) == (b["nsteps"], b["updates"], b["retries"])
+    assert a["updates"] > a["nsteps"]
+    np.testing.assert_array_equal(a["t"], b["t"])
+    assert rel(interior(s, a["u"]), interior(s, b["u"])) < 1e-12
+
+
+@pytest.mark.parametrize("body", sorted(BODIES))
+@pytest.mark.parametrize("flux", [E.FLUX_HLLC, E.FLUX_AUSMPUP])
+def test_cut_gas_at_rest_stays_at_rest(body, flux):
+    # uniform pressure on the open apertures and on the wall balances in every cut cell
+    s, px, py = cut_body(body, par=params(flux=flux, bc=ALL_BCS[2]), states=((1.0, 0.0, 0.0, 1.0), (0.7, 0.0, 0.0, 2.5)))
+    r = G.rhs(s, G.apply_bc(s, s.U)).reshape(s.ncase, 4, -1)
+    scale = (s.cut.vol / np.outer(s.grid.dy, s.grid.dx).ravel())[None, None, :]
+    assert np.max(np.abs(r * scale)) < 1e-12
+    out = G.run(s, 0.05, 500)
+    assert out["status"] == 0 and out["nsteps"] > 5
+    assert rel(interior(s, out["u"]), interior(s, s.U)) < 1e-12
+
+
+def _cell_volumes(s):
+    g, c = s.grid, s.cut
+    full = np.outer(g.dy, g.dx).ravel()
+    vol = np.where(c.flag == 1, c.vol, np.where(c.flag == 2, 0.0, full)).reshape(g.my, g.mx)
+    return vol[2:-2, 2:-2]
+
+
+@pytest.mark.parametrize("body", sorted(BODIES))
+def test_cut_run_conserves_mass_and_energy(body):
+    # RK3 with a wall and SRD redistribution on a periodic box: only momentum meets the wall
+    s, px, py = cut_body(body, par=params(bc=ALL_BCS[0]), shock=-0.6)
+    vol = _cell_volumes(s)
+    g = s.grid
+
+    def totals(u):
+        return (u.reshape(s.ncase, 4, g.my, g.mx)[:, :, 2:-2, 2:-2] * vol).sum(axis=(2, 3))[0]
+
+    out = G.run(s, 0.3, 2000)
+    assert out["status"] == 0 and out["nsteps"] > 20
+    before, after = totals(s.U), totals(out["u"])
+    for v in (0, 3):
+        assert abs(after[v] - before[v]) <= 1e-13 * abs(before[v]), v
+    assert abs(after[1] - before[1]) > 1e-3  # the body takes momentum out of the flow
+
+
+@pytest.mark.parametrize("body", ["circle", "ellipse"])
+def test_cut_transposed_body_gives_transposed_solution(body):
+    n = 40
+    px, py = BODIES[body]()
+    s = E.Solver2D(n, n, -1.0, 1.0, -1.0, 1.0, 1, params(bc=(E.BC_TRANSMISSIVE, E.BC_REFLECT, E.BC_REFLECT,
+                                                               E.BC_TRANSMISSIVE)))
+    t = E.Solver2D(n, n, -1.0, 1.0, -1.0, 1.0, 1, params(bc=(E.BC_REFLECT, E.BC_TRANSMISSIVE, E.BC_TRANSMISSIVE,
+                                                               E.BC_REFLECT)))
+    s.set_cut(E.cut2d_build_polygon(s, px, py))
+    t.set_cut(E.cut2d_build_polygon(t, py[::-1], px[::-1]))
+    X, Y = s.grid.xc[2:-2][None, :], s.grid.yc[2:-2][:, None]
+    rho = 1.0 + 0.2 * np.sin(3 * X + 1) * np.cos(2 * Y)
+    u, v = 0.3 + 0.1 * np.cos(X * Y), -0.2 + 0.1 * np.sin(X + 2 * Y)
+    p = 1.0 + 0.1 * np.cos(X - Y)
+    s.set_primitive_field(0, rho, u, v, p)
+    t.set_primitive_field(0, rho.T, v.T, u.T, p.T)
+    swap = [0, 2, 1, 3]
+
+    def transposed(a):
+        f = a.reshape(4, n + 4, n + 4)[:, 2:-2, 2:-2]
+        return np.stack([f[swap[k]].T for k in range(4)])
+
+    ra, rb = G.rhs(s, G.apply_bc(s, s.U)), G.rhs(t, G.apply_bc(t, t.U))
+    assert rel(transposed(rb), ra.reshape(4, n + 4, n + 4)[:, 2:-2, 2:-2]) < 1e-12
+    a, b = G.run(s, 0.2, 1000), G.run(t, 0.2, 1000)
+    assert (a["nsteps"], a["nreject"]) == (b["nsteps"], b["nreject"]) and a["nsteps"] > 10
+    assert rel(transposed(b["u"]), a["u"].reshape(4, n + 4, n + 4)[:, 2:-2, 2:-2]) < 1e-12
