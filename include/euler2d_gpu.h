@@ -7,7 +7,7 @@
 //   Array .................... src/array.h:12-58
 //   Grid2D ................... src/euler2d.h:54-60
 //   ImplicitStats2D .......... src/euler2d.h:118-121
-//   WallCell2D, Cut2D ........ src/euler2d.h:92-116
+//   WallCell2D, Cut2D ........ src/euler2d.h:92-94, 98-117
 //   Solver2D ................. src/euler2d.h:123-135 (IB2D only by pointer)
 //   Params2D, Stretch, enums . amr2d_types.h (src/euler2d.h:23-52, src/types.h:7-13)
 //   solver2d_run_cuda ........ replaces src/euler2d.h:211 (src/euler2d_cuda.cu:530-708)

@@ -63,7 +63,7 @@ typedef struct ugrid_params {
   double pseudo_cfl, implicit_rtol, implicit_atol;
 } ugrid_params;
 
-/* Cut-cell geometry of a grid (the arrays of Cut2D, src/euler2d.h:97-116, same layouts and
+/* Cut-cell geometry of a grid (the arrays of Cut2D, src/euler2d.h:98-117, same layouts and
  * meanings; mx * my cell arrays include the ghosts).  The device backend reads nsmall,
  * nbad_nbhd, vol, flag, axl, ayl, wax, way, nbrcount, smallpos, scnt, smem and svhat. */
 typedef struct ugrid_cut_view {

@@ -41,7 +41,7 @@ struct ImplicitView {
 struct VectorView {  // std::vector (libstdc++): begin, end, capacity end
   void* p[3];
 };
-struct CutView {  // src/euler2d.h:97-116
+struct CutView {  // src/euler2d.h:98-117
   VectorView wall_fragments, wall_cells;
   int nsmall, nbad_nbhd, nsolid, ncut;
   ArrayView vol, cx, cy, flag, axl, axy, ayl, ayx, wax, way, wcx, wcy, nbrcount, smallpos, sidx, scnt, smem, svhat,

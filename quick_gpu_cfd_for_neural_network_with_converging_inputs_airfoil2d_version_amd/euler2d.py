@@ -112,7 +112,7 @@ _CUT_SMALL = ("sidx", "scnt", "svhat", "sxhat", "syhat")
 
 class Cut2D:
     """Cut-cell geometry and SRD neighbourhoods of a solid polygon on a Solver2D's grid (the
-    reference's Cut2D, src/euler2d.h:97-116): numpy copies of every array and the C view the
+    reference's Cut2D, src/euler2d.h:98-117): numpy copies of every array and the C view the
     device backend reads."""
 
     def __init__(self, s: "Solver2D", px, py, patches=None):
@@ -157,7 +157,7 @@ class Cut2D:
 
 
 def cut2d_build_polygon(s: "Solver2D", px, py, patches=None) -> Cut2D:
-    """src/euler2d.h:164-165: raises ValueError with the reference's code on failure."""
+    """src/euler2d.h:167-170: raises ValueError with the reference's code on failure."""
     return Cut2D(s, px, py, patches)
 
 
@@ -188,7 +188,7 @@ class Solver2D:
         self.cut = None  # Cut2D (solver2d_set_cut)
 
     def set_cut(self, cut: "Cut2D | None") -> None:
-        """solver2d_set_cut (src/euler2d.h:161-162)."""
+        """solver2d_set_cut (src/euler2d.h:164-165)."""
         self.cut = cut
 
     # -- field access (src/euler2d.h:137-140, src/euler2d.cpp:158-191) --
