@@ -326,6 +326,10 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--fp64-check", type=int, default=-1,
                     help="meshes re-solved in FP64 for dCL/dCD/dCM (-1: every mesh of the batch)")
+    ap.add_argument("--few-slots", type=int, default=16,
+                    help="running slots at or below which the batch's flux launches take two threads per cell")
+    ap.add_argument("--like-samples", type=int, default=1,
+                    help="batch airfoils (seeded choice) also solved as plain FP64 single sessions")
     ap.add_argument("--serve-leg", type=int, default=48,
                     help="requests of the bounded serving leg that gives p50/p99 (0: none)")
     ap.add_argument("--cpu-threads", type=int, default=0,
@@ -437,6 +441,8 @@ def main():
     if args.solve:
         # 3. Batched FP32 steady solve of the whole batch (the headline value).
         with B.Batch.of_meshes(meshes, params, states, precision=32, tile=tile) as b:
+            if tile * 2 <= 256 and args.few_slots > 0:
+                b.set_launch_shape(1, args.few_slots, 2)  # two flux threads per cell once few slots run
             t0 = time.perf_counter()
             res32 = b.solve(cases, mode="steady", budget=args.budget)
             wall32 = time.perf_counter() - t0
@@ -480,6 +486,36 @@ def main():
         fp64_value = 3600.0 / wall64
         log(f"plain FP64 single solve: converged={r64['converged']} steps={r64['steps']} wall {wall64:.2f} s "
             f"-> {fp64_value:.0f} solves/hour; CL {r64['cl']:.9f} CD {r64['cd']:.9f} CM {r64['cm']:.9f}")
+        # Like for like: the batch's own airfoils as plain FP64 single solves.  A seeded sample is
+        # solved and timed; every mesh's FP64 step count is known from the FP64 batch (bitwise the
+        # FP64 session's), so the whole batch's plain-path time is its steps times the measured
+        # per-step time of the single sessions (the session's own tile may round the face sums in
+        # another order than the batch's, so its step count can differ by a few).
+        like = None
+        singles = [dict(mesh="config-2 airfoil", steps=int(r64["steps"]), wall_seconds=round(wall64, 3),
+                        us_per_step=round(wall64 / r64["steps"] * 1e6, 2))]
+        if deltas is not None and args.like_samples > 0:
+            pick = np.random.default_rng(args.seed + 1).choice(k, size=min(args.like_samples, k), replace=False)
+            for i in sorted(int(x) for x in pick):
+                with Session(meshes[i], params[i], states[i], precision=64) as s:
+                    t0 = time.perf_counter()
+                    ri = s.solve(cases[i], mode="steady", budget=args.budget, history=0)
+                    wi = time.perf_counter() - t0
+                singles.append(dict(mesh=f"batch airfoil {i}", steps=int(ri["steps"]),
+                                    fp64_batch_steps=int(res64[i]["steps"]), wall_seconds=round(wi, 3),
+                                    us_per_step=round(wi / ri["steps"] * 1e6, 2)))
+            us_step = float(np.mean([s_["us_per_step"] for s_ in singles]))
+            steps64 = [res64[i]["steps"] for i in range(k) if res64[i]["converged"]]
+            per_solve = float(np.mean(steps64)) * us_step * 1e-6
+            like = dict(fp64_single_solves_per_hour=round(3600.0 / per_solve, 2),
+                        speedup_vs_fp64_single=round(value / (3600.0 / per_solve), 2),
+                        fp64_steps_mean=round(float(np.mean(steps64)), 1), meshes=len(steps64),
+                        fp64_single_us_per_step=round(us_step, 2), timed_single_solves=singles)
+            log(f"like for like: the batch's {len(steps64)} airfoils as plain FP64 single solves, mean "
+                f"{np.mean(steps64):.0f} steps x {us_step:.2f} us/step (timed: " +
+                ", ".join(f"{s_['mesh']} {s_['steps']} steps {s_['wall_seconds']:.2f} s" for s_ in singles) +
+                f") -> {3600.0 / per_solve:.0f} solves/hour; batched FP32 / plain FP64 = "
+                f"{value / (3600.0 / per_solve):.2f}x")
         with Session(mesh1, params1, u1, precision=32) as s:
             t0 = time.perf_counter()
             r32 = s.solve(case1, mode="steady", budget=args.budget, history=0)
@@ -494,6 +530,7 @@ def main():
             "fp64_single_solves_per_hour": round(fp64_value, 2), "fp64_single_wall_seconds": round(wall64, 3),
             "fp64_single_steps": r64["steps"], "speedup_vs_fp64_single": round(value / fp64_value, 2),
             "fp32_vs_fp64_batch": deltas,
+            "like_for_like": like,
             "single_case_fp64_cl_cd_cm": [r64["cl"], r64["cd"], r64["cm"]],
             "single_case_fp32_minus_fp64": [r32[q] - r64[q] for q in ("cl", "cd", "cm")],
             "single_case_fp32_wall_seconds": round(wall32_single, 3),
@@ -575,6 +612,8 @@ def main():
             "batch": args.batch, "seed": args.seed, "mesh": "LF2 (6 levels, wall h/c 1/128)",
             "cells_total": int(sum(cells)), "cells_min": int(min(cells)), "cells_max": int(max(cells)),
             "tile": tile, "parallelism": "single GPU",
+            "few_slot_flux_threads_per_cell": {"slots": args.few_slots, "threads": 2} if tile * 2 <= 256 and
+            args.few_slots > 0 else None,
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": RL.HBM_PEAK_GBPS, "unit": "GB/s",

@@ -330,6 +330,13 @@ int amr_batch_advance(amr_batch* b, int32_t until, double wall_budget_seconds, a
                       size_t err_len);
 /* Exactly `trials` trials, then one poll. */
 int amr_batch_run(amr_batch* b, int32_t trials, amr_slot_state* states, char* err, size_t err_len);
+/* Launch shape of the trials (no counterpart in the reference; a batch tuning knob): the flux
+ * launches run flux_spread threads per cell (1 or 2; tile * spread <= 256), and while at most
+ * few_slots slots are running (0: never) few_flux_spread instead, so that a nearly drained batch,
+ * whose trials are bound by latency, spreads each tile's face items over more threads.  Results
+ * are bitwise independent of the shape.  Recaptures the trial graphs.  0 ok, 1 invalid or error. */
+int amr_batch_set_launch_shape(amr_batch* b, int32_t flux_spread, int32_t few_slots, int32_t few_flux_spread, char* err,
+                               size_t err_len);
 /* Slot states of the last poll (no device access). */
 int amr_batch_slots(const amr_batch* b, amr_slot_state* states);
 int32_t amr_batch_size(const amr_batch* b);

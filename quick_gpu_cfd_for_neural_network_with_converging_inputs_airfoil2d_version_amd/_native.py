@@ -245,6 +245,7 @@ def _bind_batch_api(lib):
         "amr_batch_set_targets": (C.c_int, [H, c_double_p, C.POINTER(C.c_int64), *E]),
         "amr_batch_advance": (C.c_int, [H, C.c_int32, C.c_double, C.POINTER(amr_slot_state), *E]),
         "amr_batch_run": (C.c_int, [H, C.c_int32, C.POINTER(amr_slot_state), *E]),
+        "amr_batch_set_launch_shape": (C.c_int, [H, C.c_int32, C.c_int32, C.c_int32, *E]),
         "amr_batch_slots": (C.c_int, [H, C.POINTER(amr_slot_state)]),
         "amr_batch_size": (C.c_int32, [H]),
         "amr_batch_download": (C.c_int, [H, C.c_int32, c_double_p, C.c_uint64, C.POINTER(amr_state_meta), *E]),

@@ -166,6 +166,13 @@ class Batch:
         self._check(self._lib.amr_batch_run(self.handle, trials, self._state_buf, err, N.ERR_LEN), err)
         return _states(self._state_buf)
 
+    def set_launch_shape(self, flux_spread: int = 1, few_slots: int = 0, few_flux_spread: int = 1) -> None:
+        """Threads per cell of the flux launches, and the spread used while at most `few_slots`
+        slots run (latency-bound trials); results are bitwise independent of the shape."""
+        err = N.err_buffer()
+        self._check(self._lib.amr_batch_set_launch_shape(self.handle, flux_spread, few_slots, few_flux_spread, err,
+                                                          N.ERR_LEN), err)
+
     def states(self) -> list[dict]:
         self._lib.amr_batch_slots(self.handle, self._state_buf)
         return _states(self._state_buf)

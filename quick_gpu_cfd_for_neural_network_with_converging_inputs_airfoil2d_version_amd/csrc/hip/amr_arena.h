@@ -85,6 +85,9 @@ struct MeshDesc {
 template <class T>
 struct ArenaPtrs {
   int n_meshes, nc, nf, nw, tile;
+  // Threads per cell of the flux launches (1 or 2; tile * flux_spread <= 256): the face items of
+  // a tile are spread over flux_spread * tile threads, the cell work stays on the first tile.
+  int flux_spread;
   int n_cell_blocks, n_face_blocks, n_wall_blocks;
   int n_grid_blocks;  // blocks of a cell launch (>= n_cell_blocks, a multiple of 8)
   int commit_first, commit_stride;  // slots a trial's commit covers: first, first + stride, ...

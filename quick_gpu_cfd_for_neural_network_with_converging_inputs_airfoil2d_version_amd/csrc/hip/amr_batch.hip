@@ -39,9 +39,14 @@ struct amr_batch {
   std::vector<hipEvent_t> joined;      // per side stream: its part of a batch is done
   Control* snap[2] = {nullptr, nullptr};  // pinned control snapshots of the pipelined advance
   hipEvent_t snap_done[2] = {nullptr, nullptr};
-  std::vector<hipGraph_t> graphs;      // per branch: one trial, then batch_trials trials
+  std::vector<hipGraph_t> graphs;      // every captured graph (owned)
   std::vector<hipGraphExec_t> execs;   // per branch: one trial
   std::vector<hipGraphExec_t> execs_batch;  // per branch: batch_trials trials in one graph
+  // Launch shape while few slots run: at most few_slots running slots replay the trials captured
+  // with few_spread threads per cell in the flux launches (the per-trial latency then dominates,
+  // and spreading a tile's face items shortens each flux launch); otherwise spread.
+  int spread = 1, few_slots = 0, few_spread = 1;
+  std::vector<hipGraphExec_t> execs_few, execs_batch_few;
   Control* host_control = nullptr;  // pinned, n slots
   double* host_targets = nullptr;   // pinned, n slots
   long long* host_maxsteps = nullptr;
@@ -52,6 +57,8 @@ struct amr_batch {
     for (hipStream_t s : sides) (void)hipStreamSynchronize(s);
     for (hipGraphExec_t e : execs) (void)hipGraphExecDestroy(e);
     for (hipGraphExec_t e : execs_batch) (void)hipGraphExecDestroy(e);
+    for (hipGraphExec_t e : execs_few) (void)hipGraphExecDestroy(e);
+    for (hipGraphExec_t e : execs_batch_few) (void)hipGraphExecDestroy(e);
     for (hipGraph_t g : graphs) (void)hipGraphDestroy(g);
     arena.reset();
     if (host_control) (void)hipHostFree(host_control);
@@ -79,6 +86,11 @@ bool options_ok(const amr_batch_options* o) {
 
 hipamr::SlotCapacity to_cap(const amr_slot_capacity& c) {
   return hipamr::SlotCapacity{c.cells, c.faces, c.walls, c.csr, c.terms, c.groups, c.members};
+}
+
+// Whether the next trials run with the few-slot launch shape (`running` slots are running).
+bool few_shape(const amr_batch* b, int running) {
+  return b->few_slots > 0 && b->few_spread != b->spread && running <= b->few_slots;
 }
 
 // Join the branches, then read every control record.
@@ -130,15 +142,21 @@ int queue_batch(amr_batch* b, int k) {
   }
   const int branches = b->arena->n_branches();
   std::vector<char> busy(branches, branches == 1);
-  for (int m = 0; m < b->n && branches > 1; ++m) {
+  int running = 0;
+  for (int m = 0; m < b->n; ++m) {
     const Control& c = b->host_control[m];
-    busy[m % branches] |= !c.finished && !c.fatal;
+    const bool on = !c.finished && !c.fatal;
+    running += on;
+    if (branches > 1) busy[m % branches] |= on;
   }
+  const bool few = few_shape(b, running);
+  const std::vector<hipGraphExec_t>& poll_graphs = few ? b->execs_batch_few : b->execs_batch;
+  if (poll_graphs.empty()) b->arena->set_flux_spread(few ? b->few_spread : b->spread);
   for (int j = 0; j < branches; ++j) {
     if (!busy[j]) continue;
     hipStream_t s = branch_stream(b, j);
-    if (!b->execs_batch.empty()) {  // the batch's trials as one graph
-      if (hipGraphLaunch(b->execs_batch[j], s) != hipSuccess) return 1;
+    if (!poll_graphs.empty()) {  // the batch's trials as one graph
+      if (hipGraphLaunch(poll_graphs[j], s) != hipSuccess) return 1;
     } else {
       for (int trial = 0; trial < b->options.batch_trials; ++trial) b->arena->launch_trial_branch(j, s);
     }
@@ -173,16 +191,87 @@ int launch_one(amr_batch* b) {
     b->side_must_wait = false;
   }
   const int branches = b->arena->n_branches();
+  int running = 0;
+  for (int m = 0; m < b->n; ++m) running += !b->host_control[m].finished && !b->host_control[m].fatal;
+  const bool few = few_shape(b, running);
+  const std::vector<hipGraphExec_t>& graphs = few ? b->execs_few : b->execs;
+  if (graphs.empty()) b->arena->set_flux_spread(few ? b->few_spread : b->spread);
   for (int j = 0; j < branches; ++j) {
     hipStream_t s = branch_stream(b, j);
-    if (!b->execs.empty()) {
-      if (hipGraphLaunch(b->execs[j], s) != hipSuccess) return 1;
+    if (!graphs.empty()) {
+      if (hipGraphLaunch(graphs[j], s) != hipSuccess) return 1;
     } else {
       b->arena->launch_trial_branch(j, s);
     }
   }
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
+
+// The trial graphs of every branch at the arena's current launch shape: one trial (into
+// `one`), and the trials of one poll as one graph (into `poll`).
+int capture_graphs(amr_batch* b, std::vector<hipGraphExec_t>* one, std::vector<hipGraphExec_t>* poll, char* err,
+                   size_t err_len) {
+  const int branches = b->arena->n_branches();
+  b->stats.graph_nodes_per_trial = 0;
+  for (int j = 0; j < branches; ++j) {
+    hipStream_t s = branch_stream(b, j);
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    if (!checked(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), err, err_len, "AMR graph begin"))
+      return 1;
+    b->arena->launch_trial_branch(j, s);
+    if (!checked(hipStreamEndCapture(s, &g), err, err_len, "AMR graph end")) return 1;
+    b->graphs.push_back(g);
+    if (!checked(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), err, err_len, "AMR graph instantiate")) return 1;
+    one->push_back(e);
+    size_t nodes = 0;
+    if (!checked(hipGraphGetNodes(g, nullptr, &nodes), err, err_len, "AMR graph node count")) return 1;
+    b->stats.graph_nodes_per_trial += nodes;
+  }
+  // The trials of one poll as a single graph per branch: consecutive graph launches on a
+  // stream leave a gap between them (about 6 us on one LF2 mesh, where a trial takes 70 us),
+  // consecutive kernels inside one graph none.
+  if (b->options.batch_trials > 1)
+    for (int j = 0; j < branches; ++j) {
+      hipStream_t s = branch_stream(b, j);
+      hipGraph_t g = nullptr;
+      hipGraphExec_t e = nullptr;
+      if (!checked(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), err, err_len, "AMR graph begin"))
+        return 1;
+      for (int trial = 0; trial < b->options.batch_trials; ++trial) b->arena->launch_trial_branch(j, s);
+      if (!checked(hipStreamEndCapture(s, &g), err, err_len, "AMR graph end")) return 1;
+      b->graphs.push_back(g);
+      if (!checked(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), err, err_len, "AMR graph instantiate"))
+        return 1;
+      poll->push_back(e);
+    }
+  return 0;
+}
+
+// Drops the captured graphs (the streams are idle).
+void release_graphs(amr_batch* b) {
+  for (auto* v : {&b->execs, &b->execs_batch, &b->execs_few, &b->execs_batch_few}) {
+    for (hipGraphExec_t e : *v) (void)hipGraphExecDestroy(e);
+    v->clear();
+  }
+  for (hipGraph_t g : b->graphs) (void)hipGraphDestroy(g);
+  b->graphs.clear();
+}
+
+// The graphs of both launch shapes (the few-slot set only when it differs).
+int capture_all(amr_batch* b, char* err, size_t err_len) {
+  if (!b->arena->set_flux_spread(b->spread)) return fail(err, err_len, "invalid AMR launch shape");
+  if (capture_graphs(b, &b->execs, &b->execs_batch, err, err_len)) return 1;
+  if (b->few_slots > 0 && b->few_spread != b->spread) {
+    const size_t nodes = b->stats.graph_nodes_per_trial;
+    if (!b->arena->set_flux_spread(b->few_spread) || capture_graphs(b, &b->execs_few, &b->execs_batch_few, err, err_len))
+      return fail(err, err_len, "invalid AMR launch shape");
+    b->stats.graph_nodes_per_trial = nodes;
+  }
+  b->arena->set_flux_spread(b->spread);
+  return 0;
+}
+
 
 // Stream, pinned control mirrors, optional graph; the arena is already built.
 int finish_setup(amr_batch* b, Clock::time_point t0, char* err, size_t err_len) {
@@ -215,40 +304,7 @@ int finish_setup(amr_batch* b, Clock::time_point t0, char* err, size_t err_len) 
   if (branches > 1 &&
       !checked(hipEventCreateWithFlags(&b->host_done, hipEventDisableTiming), err, err_len, "AMR branch event"))
     return 1;
-  if (b->options.graph) {
-    for (int j = 0; j < branches; ++j) {
-      hipStream_t s = branch_stream(b, j);
-      hipGraph_t g = nullptr;
-      hipGraphExec_t e = nullptr;
-      if (!checked(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), err, err_len, "AMR graph begin"))
-        return 1;
-      b->arena->launch_trial_branch(j, s);
-      if (!checked(hipStreamEndCapture(s, &g), err, err_len, "AMR graph end")) return 1;
-      b->graphs.push_back(g);
-      if (!checked(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), err, err_len, "AMR graph instantiate")) return 1;
-      b->execs.push_back(e);
-      size_t nodes = 0;
-      if (!checked(hipGraphGetNodes(g, nullptr, &nodes), err, err_len, "AMR graph node count")) return 1;
-      b->stats.graph_nodes_per_trial += nodes;
-    }
-    // The trials of one poll as a single graph per branch: consecutive graph launches on a
-    // stream leave a gap between them (about 6 us on one LF2 mesh, where a trial takes 70 us),
-    // consecutive kernels inside one graph none.
-    if (b->options.batch_trials > 1)
-      for (int j = 0; j < branches; ++j) {
-        hipStream_t s = branch_stream(b, j);
-        hipGraph_t g = nullptr;
-        hipGraphExec_t e = nullptr;
-        if (!checked(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), err, err_len, "AMR graph begin"))
-          return 1;
-        for (int trial = 0; trial < b->options.batch_trials; ++trial) b->arena->launch_trial_branch(j, s);
-        if (!checked(hipStreamEndCapture(s, &g), err, err_len, "AMR graph end")) return 1;
-        b->graphs.push_back(g);
-        if (!checked(hipGraphInstantiate(&e, g, nullptr, nullptr, 0), err, err_len, "AMR graph instantiate"))
-          return 1;
-        b->execs_batch.push_back(e);
-      }
-  }
+  if (b->options.graph && capture_all(b, err, err_len)) return 1;
   b->stats.setup_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
   return 0;
 }
@@ -455,7 +511,10 @@ int amr_batch_advance(amr_batch* b, int32_t until, double wall_budget_seconds, a
 int amr_batch_run(amr_batch* b, int32_t trials, amr_slot_state* states, char* err, size_t err_len) {
   if (!b || trials < 1) return fail(err, err_len, "invalid AMR batch run");
   int i = 0;
-  if (!b->execs_batch.empty() && trials >= b->options.batch_trials) {
+  int running = 0;
+  for (int m = 0; m < b->n; ++m) running += !b->host_control[m].finished && !b->host_control[m].fatal;
+  const std::vector<hipGraphExec_t>& poll_graphs = few_shape(b, running) ? b->execs_batch_few : b->execs_batch;
+  if (!poll_graphs.empty() && trials >= b->options.batch_trials) {
     if (!b->sides.empty() && b->side_must_wait) {  // as launch_one: side streams after host-side work
       if (hipEventRecord(b->host_done, b->stream) != hipSuccess) return fail(err, err_len, "AMR trial launch failed");
       for (hipStream_t s : b->sides)
@@ -464,13 +523,31 @@ int amr_batch_run(amr_batch* b, int32_t trials, amr_slot_state* states, char* er
     }
     for (; i + b->options.batch_trials <= trials; i += b->options.batch_trials)
       for (int j = 0; j < b->arena->n_branches(); ++j)
-        if (hipGraphLaunch(b->execs_batch[j], branch_stream(b, j)) != hipSuccess)
+        if (hipGraphLaunch(poll_graphs[j], branch_stream(b, j)) != hipSuccess)
           return fail(err, err_len, "AMR trial launch failed");
   }
   for (; i < trials; ++i)
     if (launch_one(b)) return fail(err, err_len, "AMR trial launch failed");
   if (!checked(hipGetLastError(), err, err_len, "AMR trial kernels") || poll(b, err, err_len)) return 1;
   fill_states(b, states);
+  return 0;
+}
+
+int amr_batch_set_launch_shape(amr_batch* b, int32_t flux_spread, int32_t few_slots, int32_t few_flux_spread, char* err,
+                               size_t err_len) {
+  const auto ok_spread = [&](int s) { return s == 1 || s == 2; };
+  if (!b || !ok_spread(flux_spread) || !ok_spread(few_flux_spread) || few_slots < 0 ||
+      b->options.tile * std::max(flux_spread, few_flux_spread) > 256)
+    return fail(err, err_len, "invalid AMR launch shape (threads per cell 1 or 2, at most 256 per block)");
+  if (!checked(hipStreamSynchronize(b->stream), err, err_len, "AMR launch shape")) return 1;
+  for (hipStream_t s : b->sides)
+    if (!checked(hipStreamSynchronize(s), err, err_len, "AMR launch shape")) return 1;
+  release_graphs(b);
+  b->spread = flux_spread;
+  b->few_slots = few_slots;
+  b->few_spread = few_flux_spread;
+  if (b->options.graph) return capture_all(b, err, err_len);
+  b->arena->set_flux_spread(flux_spread);
   return 0;
 }
 

@@ -66,6 +66,10 @@ class Arena {
   // Base primitives of every running slot whose host-written state has none yet; follows every
   // launch_target before trials are launched.
   virtual int launch_prims(hipStream_t s) = 0;
+  // Threads per cell of the flux launches (1 or 2, at most 256 threads per block); launches
+  // issued after the call use it.  False: not a valid spread at this tile.
+  virtual bool set_flux_spread(int spread) = 0;
+  virtual int flux_spread() const = 0;
   // Independent branches of the trial (slots m with m % n_branches() == j) and the trial of one
   // branch: its slots' kernels and commit only, for its own stream.
   virtual int n_branches() const = 0;

@@ -854,6 +854,7 @@ class DeviceArena final : public Arena {
     a.nf = nf;
     a.nw = nw;
     a.tile = tile_;
+    a.flux_spread = 1;
     a.n_cell_blocks = n_cell_blocks_;
     a.n_grid_blocks = table_offsets_[1];  // the whole-arena table
     a.commit_first = 0;
@@ -966,6 +967,12 @@ class DeviceArena final : public Arena {
     return hipamr::launch_trial<T>(a, s, nullptr);
   }
   int launch_prims(hipStream_t s) override { return hipamr::launch_prims<T>(ptrs_, s, nullptr); }
+  bool set_flux_spread(int spread) override {
+    if ((spread != 1 && spread != 2) || tile_ * spread > 256) return false;  // launch bound
+    ptrs_.flux_spread = spread;
+    return true;
+  }
+  int flux_spread() const override { return ptrs_.flux_spread; }
   Control* control() override { return ptrs_.ctrl; }
   double* target_array() override { return targets_; }
   long long* maxsteps_array() override { return maxsteps_; }

@@ -712,7 +712,10 @@ __device__ __forceinline__ void arrive_and_commit(Control* c, int n_blocks, bool
 // stage passes (72 VGPRs, 2 spilled; they need 78), which the LDS of a 128-cell tile also admits.
 // On 256 LF2 meshes: base flux 187 -> 177 us, stage flux 188 -> 181 us against the unbounded
 // kernels; 8 waves (64 VGPRs) spills 11-14 and is slower.  FP64 (190-200 VGPRs) has no bound.
-template <class T, int STAGE>
+// SPREAD threads per cell (1 or 2): a tile's face items are spread over SPREAD * tile threads,
+// the cell work stays on the first tile of them (a separate instantiation, so that the one-thread
+// kernel keeps its registers).
+template <class T, int STAGE, int SPREAD>
 __global__ __launch_bounds__(kMaxTile) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (STAGE == 0 ? 6 : 7) : 1)))
 void k_flux(ArenaPtrs<T> a) {
   constexpr bool BASE = STAGE == 0;
@@ -727,13 +730,16 @@ void k_flux(ArenaPtrs<T> a) {
   const bool live = running(c);
   const MeshDesc<T>& d = a.desc[m];
   const int nc = a.nc;
-  const int k = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
-  const bool valid = k < d.cell_begin + d.n_cells;
+  // tn cells per tile; nt = SPREAD * tn threads share the tile's face items, and thread tid < tn
+  // owns cell first + tid.
+  const int nt = static_cast<int>(blockDim.x), tid = static_cast<int>(threadIdx.x);
+  const int tn = SPREAD == 1 ? nt : nt / SPREAD;
+  const int first = tile * tn;
+  const int k = first + tid;
+  const bool valid = (SPREAD == 1 || tid < tn) && k < d.cell_begin + d.n_cells;
   const bool sa = d.sa != 0, laminar = d.laminar != 0;
   const int rb = read_buffer(c, STAGE);
   const int bit = BASE ? BAD_BASE : BAD_STAGE;
-  const int tn = static_cast<int>(blockDim.x), tid = static_cast<int>(threadIdx.x);
-  const int first = tile * tn;
   const int staged = min(tn, d.cell_begin + d.n_cells - first);  // valid cells of the tile
   const TileCells<T> lds(tn);
   const FaceSlots<T> slots(tn, BASE);
@@ -754,7 +760,7 @@ void k_flux(ArenaPtrs<T> a) {
   bool ok = true;
   for (int c0 = range.x; c0 < range.y; c0 += chunk) {
     const int c1 = min(range.y, c0 + chunk);
-    for (int i = c0 + tid; i < c1; i += tn) {
+    for (int i = c0 + tid; i < c1; i += nt) {
       const FaceItem<T> it = load_item(a.ITEM_A, a.ITEM_B, static_cast<size_t>(i));
       CellPrim<T> L{}, R{};
       CellGrad<T> Lg{}, Rg{};
@@ -1177,6 +1183,17 @@ inline int blocks_of(int n, int tile) { return (n + tile - 1) / tile; }
     ++launches;                                                      \
   } while (0)
 #define HIPAMR_LAUNCH(name, kernel, grid, ...) HIPAMR_LAUNCH_LDS(name, kernel, grid, 0, __VA_ARGS__)
+// A flux launch of stage STAGE: the tile's LDS, flux_spread threads per cell.
+#define HIPAMR_LAUNCH_FLUX(name, STAGE, grid, lds, ...)                                                   \
+  do {                                                                                                   \
+    if (hook) hook->before(name);                                                                        \
+    if (a.flux_spread == 2)                                                                              \
+      hipLaunchKernelGGL((k_flux<T, STAGE, 2>), grid, dim3(2 * a.tile), lds, s, __VA_ARGS__);          \
+    else                                                                                                 \
+      hipLaunchKernelGGL((k_flux<T, STAGE, 1>), grid, dim3(a.tile), lds, s, __VA_ARGS__);              \
+    if (hook) hook->after(name);                                                                         \
+    ++launches;                                                                                          \
+  } while (0)
 
 // The base pass proper (gradients, faces, residual, local dt); the base primitives are valid.
 template <class T>
@@ -1185,7 +1202,7 @@ int launch_base_pass(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
   const size_t base_lds = flux_lds_bytes<T>(a.tile, true), vel_lds = grad_lds_bytes<T>(a.tile);
   int launches = 0;
   if (a.any_laminar) HIPAMR_LAUNCH_LDS("gradient_base", (k_grad<T, 0>), cells, vel_lds, a);
-  HIPAMR_LAUNCH_LDS("flux_base", (k_flux<T, 0>), cells, base_lds, a);
+  HIPAMR_LAUNCH_FLUX("flux_base", 0, cells, base_lds, a);
   if (a.any_visc_group) {
     HIPAMR_LAUNCH("diffusion_group", k_diffusion_group<T>, cells, a);
     HIPAMR_LAUNCH("dt_group", k_dt_group<T>, cells, a);
@@ -1219,13 +1236,13 @@ int launch_trial(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
     HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 0);
   }
   if (a.any_laminar) HIPAMR_LAUNCH_LDS("gradient_stage", (k_grad<T, 1>), cells, vel_lds, a);
-  HIPAMR_LAUNCH_LDS("flux_rk1", (k_flux<T, 1>), cells, stage_lds, a);
+  HIPAMR_LAUNCH_FLUX("flux_rk1", 1, cells, stage_lds, a);
   if (a.any_small) {
     HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 1);
     HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 1);
   }
   if (a.any_laminar) HIPAMR_LAUNCH_LDS("gradient_stage", (k_grad<T, 2>), cells, vel_lds, a);
-  HIPAMR_LAUNCH_LDS("flux_rk2", (k_flux<T, 2>), cells, stage_lds, a);
+  HIPAMR_LAUNCH_FLUX("flux_rk2", 2, cells, stage_lds, a);
   if (a.any_small) {
     HIPAMR_LAUNCH("srd_group", k_group<T>, cells, a, 2);
     HIPAMR_LAUNCH("srd_redistribute", k_redist<T>, cells, a, 2);
@@ -1240,6 +1257,7 @@ int launch_trial(const ArenaPtrs<T>& a, hipStream_t s, LaunchHook* hook) {
 }
 #undef HIPAMR_LAUNCH
 #undef HIPAMR_LAUNCH_LDS
+#undef HIPAMR_LAUNCH_FLUX
 
 template <class T>
 void launch_monitor(const ArenaPtrs<T>& a, hipStream_t s) {

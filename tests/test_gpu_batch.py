@@ -143,6 +143,40 @@ def test_fp32_sa_branches_at_production_tiles_match_single_slot_bitwise(tile):
             assert np.array_equal(u, ref_u)
 
 
+@pytest.mark.parametrize("precision", [32, 64])
+def test_launch_shapes_change_no_bit(precision):
+    """Flux launches with two threads per cell (the face items of a tile spread over
+    more threads; each cell still sums its faces in its own order), also switched on only once
+    few slots run and without graphs: every slot ends with the bits of the default shape."""
+    items = []
+    for src, preset in (("G0002", "LF1"), ("G0100", "LF2"), ("G0001", "LF2"), ("G0010", "LF1")):
+        case, mesh, params = case_mesh(src, preset)
+        items.append((mesh, params, mesh.initialize(params, case.freestream)))
+    targets = [0.004, 0.006, 0.01, 0.01]
+
+    def run(shape, graph=True, tile=128):
+        with B.Batch.of_meshes([m for m, _, _ in items], [p for _, p, _ in items], [u for _, _, u in items],
+                               precision=precision, tile=tile, graph=graph) as b:
+            if shape is not None:
+                b.set_launch_shape(*shape)
+            b.set_targets(targets)
+            assert b.advance("all") == 0
+            return [b.download(k) for k in range(len(items))]
+
+    refs = {}  # the default shape at each tile (another tile rounds the face sums in another order)
+    for shape, graph, tile in (((2, 0, 2), True, 128), ((1, 2, 2), True, 128), ((1, 3, 2), False, 128),
+                               ((2, 0, 2), True, 64)):
+        print(precision, "shape", shape, "graph", graph, "tile", tile, flush=True)
+        if tile not in refs:
+            refs[tile] = run(None, True, tile)
+        for k, ((u, meta), (ref_u, ref_meta)) in enumerate(zip(run(shape, graph, tile), refs[tile])):
+            assert meta == ref_meta, (shape, k)
+            assert np.array_equal(u, ref_u), (shape, k)
+    with B.Batch.of_meshes([items[0][0]], [items[0][1]], [items[0][2]], precision=precision, tile=128) as b:
+        with pytest.raises(BackendError):
+            b.set_launch_shape(1, 4, 4)  # four threads per cell
+
+
 def test_refill_and_release_keep_neighbours_bitwise():
     items = four_variable_meshes()
     (ca, ma, pa, ua), (cb, mb, pb, ub), (cc, mc, pc, uc) = items
