@@ -411,39 +411,64 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
   if (failed || k >= d.cell_begin + d.n_cells) return;
   const T rho = own.rho, u = own.u, v = own.v, phi = own.phi;
   T g0 = 0, g1 = 0, g2 = 0, g3 = 0, sx = 0, sy = 0, rx = 0, ry = 0;
-  const int end = a.term_off[k + 1];
-  for (int j = a.term_off[k]; j < end; ++j) {
-    const int src = a.term_src[j];
-    const T wx = a.term_wx[j], wy = a.term_wy[j];
-    T uu = T(0), vv = T(0), ph = T(0), rr = rho;
-    if (src >= 0) {
-      CellVel<T> nb;
-      const int local = src - first;
-      if (local >= 0 && local < staged)
-        load4(vel + 4 * local, nb.rho, nb.u, nb.v, nb.phi);
-      else
-        nb = load_vel(a, rb, src);
-      uu = nb.u;
-      vv = nb.v;
-      if (sa) {
-        ph = nb.phi;
-        rr = nb.rho;
-      }
-    } else if (src <= -2) {
-      const Side<T>& s = d.side[-2 - src];
-      uu = s.wall_velocity[0];
-      vv = s.wall_velocity[1];
+  // The terms in batches of kTerms (nine cells in ten have four): every load of a batch (term
+  // records, then the neighbours' vectors) is issued before the first sum, so a cell waits for
+  // two memory round trips per batch instead of two per term.  The sums run term by term in the
+  // stored order, as one term at a time would.
+  constexpr int kTerms = 4;
+  const int begin = a.term_off[k], end = a.term_off[k + 1];
+  for (int j0 = begin; j0 < end; j0 += kTerms) {
+    int src[kTerms];
+    T wx[kTerms], wy[kTerms];
+#pragma unroll
+    for (int q = 0; q < kTerms; ++q) {
+      const int j = min(j0 + q, end - 1);  // terms past the end are loaded in range and not summed
+      src[q] = a.term_src[j];
+      wx[q] = a.term_wx[j];
+      wy[q] = a.term_wy[j];
     }
-    const T du = uu - u, dv = vv - v;
-    g0 += wx * du;
-    g1 += wy * du;
-    g2 += wx * dv;
-    g3 += wy * dv;
-    if (sa) {
-      sx += wx * (ph - phi);
-      sy += wy * (ph - phi);
-      rx += wx * (rr - rho);
-      ry += wy * (rr - rho);
+    // Neighbour vectors without branches: one generic-address load per term, from the tile's LDS
+    // copy when the neighbour is in the tile, else from the record array (a side term reads the
+    // cell's own LDS copy and discards it), so the batch's loads are all in flight together.
+    CellVel<T> nb[kTerms];
+#pragma unroll
+    for (int q = 0; q < kTerms; ++q) {
+      const int local = src[q] - first;
+      const T* from = src[q] < 0 ? vel + 4 * threadIdx.x
+                                 : (local >= 0 && local < staged ? vel + 4 * local : a.VREC[rb] + 4 * static_cast<size_t>(src[q]));
+      load4(from, nb[q].rho, nb[q].u, nb[q].v, nb[q].phi);
+    }
+    // Neighbour (u, v, phi, rho) of each term: 0, 0, 0 and the cell's own density for a side
+    // without a wall velocity (-1), the side's wall velocity for a wall side (<= -2).
+    T uu[kTerms], vv[kTerms], ph[kTerms], rr[kTerms];
+#pragma unroll
+    for (int q = 0; q < kTerms; ++q) {
+      const bool cell = src[q] >= 0;
+      uu[q] = cell ? nb[q].u : T(0);
+      vv[q] = cell ? nb[q].v : T(0);
+      ph[q] = cell && sa ? nb[q].phi : T(0);
+      rr[q] = cell && sa ? nb[q].rho : rho;
+      if (src[q] <= -2) {
+        const Side<T>& s = d.side[-2 - src[q]];
+        uu[q] = s.wall_velocity[0];
+        vv[q] = s.wall_velocity[1];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kTerms; ++q) {
+      if (j0 + q < end) {
+        const T du = uu[q] - u, dv = vv[q] - v;
+        g0 += wx[q] * du;
+        g1 += wy[q] * du;
+        g2 += wx[q] * dv;
+        g3 += wy[q] * dv;
+        if (sa) {
+          sx += wx[q] * (ph[q] - phi);
+          sy += wy[q] * (ph[q] - phi);
+          rx += wx[q] * (rr[q] - rho);
+          ry += wy[q] * (rr[q] - rho);
+        }
+      }
     }
   }
   CellGrad<T> g{g0, g1, g2, g3, sx, sy, T(0)};
