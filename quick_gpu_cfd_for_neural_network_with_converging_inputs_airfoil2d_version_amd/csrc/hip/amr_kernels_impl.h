@@ -251,6 +251,31 @@ __device__ __forceinline__ FaceItem<double> load_item(const double* A, const dou
   const double2 g = *reinterpret_cast<const double2*>(A + 4 * i + 2);
   return finish_item<double>(a.x, a.y, g.x, g.y, B + 4 * i);
 }
+// The ITEM_A words of an item as loaded (so that the next item's can be in flight while the
+// current one is evaluated), and the item they make.
+template <class T>
+struct ItemWords;
+template <>
+struct ItemWords<float> {
+  int4 a;
+};
+template <>
+struct ItemWords<double> {
+  int4 a;
+  double2 g;
+};
+__device__ __forceinline__ ItemWords<float> load_item_words(const float* A, size_t i) {
+  return ItemWords<float>{*reinterpret_cast<const int4*>(A + 4 * i)};
+}
+__device__ __forceinline__ ItemWords<double> load_item_words(const double* A, size_t i) {
+  return ItemWords<double>{*reinterpret_cast<const int4*>(A + 4 * i), *reinterpret_cast<const double2*>(A + 4 * i + 2)};
+}
+__device__ __forceinline__ FaceItem<float> item_of(const ItemWords<float>& w, const float* B, size_t i) {
+  return finish_item<float>(w.a.x, w.a.y, __int_as_float(w.a.z), __int_as_float(w.a.w), B + 4 * i);
+}
+__device__ __forceinline__ FaceItem<double> item_of(const ItemWords<double>& w, const double* B, size_t i) {
+  return finish_item<double>(w.a.x, w.a.y, w.g.x, w.g.y, B + 4 * i);
+}
 
 // U -> cell primitives (computed in FP64 from the FP64 state, stored as T), plus molecular
 // and eddy viscosity for viscous meshes; false if not admissible (reference primitive rules,
@@ -733,8 +758,8 @@ __device__ __forceinline__ void arrive_and_commit(Control* c, int n_blocks, bool
 // ------------------------------------------------------------------------------------------
 // Residual of every cell from its faces and wall fragments.  STAGE 0: base residual R0 and the
 // local dt.  1, 2: second and third SSP-RK3 combinations, then the candidate's primitives.
-// FP32 register budget: 6 waves per SIMD in the base pass (80 VGPRs; it needs 83) and 7 in the
-// stage passes (72 VGPRs, 2 spilled; they need 78), which the LDS of a 128-cell tile also admits.
+// FP32 register budget: 6 waves per SIMD in the base pass (80 VGPRs, no spill) and 7 in the
+// stage passes (72 VGPRs; the last stage spills 2), which the LDS of a 128-cell tile also admits.
 // On 256 LF2 meshes: base flux 187 -> 177 us, stage flux 188 -> 181 us against the unbounded
 // kernels; 8 waves (64 VGPRs) spills 11-14 and is slower.  FP64 (190-200 VGPRs) has no bound.
 // SPREAD threads per cell (1 or 2): a tile's face items are spread over SPREAD * tile threads,
@@ -785,8 +810,15 @@ void k_flux(ArenaPtrs<T> a) {
   bool ok = true;
   for (int c0 = range.x; c0 < range.y; c0 += chunk) {
     const int c1 = min(range.y, c0 + chunk);
+    // The base pass loads the next item's words while this one is evaluated (2 us less on 256
+    // LF2 meshes); the stage passes, held to 72 registers, would spill them and lose 9-11 us.
+    constexpr bool kPrefetch = BASE;
+    ItemWords<T> next{};
+    if (kPrefetch && c0 + tid < c1) next = load_item_words(a.ITEM_A, static_cast<size_t>(c0 + tid));
     for (int i = c0 + tid; i < c1; i += nt) {
-      const FaceItem<T> it = load_item(a.ITEM_A, a.ITEM_B, static_cast<size_t>(i));
+      const ItemWords<T> words = kPrefetch ? next : load_item_words(a.ITEM_A, static_cast<size_t>(i));
+      if (kPrefetch && i + nt < c1) next = load_item_words(a.ITEM_A, static_cast<size_t>(i + nt));
+      const FaceItem<T> it = item_of(words, a.ITEM_B, static_cast<size_t>(i));
       CellPrim<T> L{}, R{};
       CellGrad<T> Lg{}, Rg{};
       const int ll = it.left - first, rl = it.right - first;
