@@ -167,7 +167,7 @@ def legacy_workload(args) -> dict:
     states = [(1.0, 0.0, 0.0, 1.0, 0.125, 0.0, 0.0, 0.1), (1.0, 0.0, 0.0, 1000.0, 1.0, 0.0, 0.0, 0.01),
               (1.0, 0.75, 0.0, 1.0, 0.125, 0.0, 0.0, 0.1), (1.0, 0.0, 0.3, 10.0, 1.0, 0.0, -0.2, 0.1)]
 
-    def blast_batch(n, ncase):
+    def blast_batch(n, ncase, body=False):
         par = E.params2d_default()
         par.limiter = E.LIM_FIRST_ORDER
         s = E.Solver2D(n, n, -1.0, 1.0, -1.0, 1.0, ncase, par)
@@ -177,6 +177,9 @@ def legacy_workload(args) -> dict:
         for c in range(ncase):
             st = states[c % len(states)]
             s.set_primitive_field(c, *[np.where(inside, st[k], st[4 + k]) for k in range(4)])
+        if body:  # a solid disc of radius 0.2 beside the blast, as cut cells with SRD
+            a = 0.0137 + 2.0 * np.pi * np.arange(256) / 256
+            s.set_cut(E.cut2d_build_polygon(s, 0.55 + 0.2 * np.cos(a), 0.1 + 0.2 * np.sin(a)))
         return s
 
     n, ncase = args.legacy_n, args.legacy_cases
@@ -208,6 +211,32 @@ def legacy_workload(args) -> dict:
     log(f"legacy grid: {ms_per_step:.3f} ms/step, {value / 1e9:.2f} G cell-steps/s; stage us "
         f"{us[0]:.1f} / {us[1]:.1f} / {us[2]:.1f} -> {rates[0]:.0f} / {rates[1]:.0f} / {rates[2]:.0f} GB/s; "
         f"launch {info}")
+    # The same batch with a solid body (cut cells, SRD after every stage): steps, rejected trials
+    # and stage times including the redistribution kernels.
+    cut = None
+    if args.legacy_cut:
+        sc = blast_batch(n, ncase, body=True)
+        with sc.device_session() as d:
+            status, msg = d.run(1e9, args.warmup)
+            assert status == 1, msg
+            r0 = d.meta.nreject
+            t0 = time.perf_counter()
+            status, msg = d.run(1e9, args.steps)
+            wall_c = time.perf_counter() - t0
+            assert status == 1, msg
+            rejected = d.meta.nreject - r0
+            d.pull()
+        assert np.all(np.isfinite(sc.U))
+        # Every step's first candidates fail the admissibility test in the smallest cut cells
+        # (it runs before the redistribution, as in the reference's backend) until the step has
+        # been halved enough: rejected trials run the first stage only.
+        cut = {"cell_steps_per_second": round(cells * args.steps / wall_c), "ms_per_step": round(wall_c / args.steps * 1e3, 4),
+               "rejected_trials": int(rejected), "ms_per_trial": round(wall_c / (args.steps + rejected) * 1e3, 4),
+               "small_cells": int(sc.cut.nsmall), "cut_cells": int(sc.cut.ncut), "solid_cells": int(sc.cut.nsolid),
+               "body": "disc of radius 0.2 at (0.55, 0.1), 256 vertices"}
+        log(f"legacy grid with a body: {cut['ms_per_step']:.3f} ms/step ({rejected} rejected trials in "
+            f"{args.steps} steps, {cut['ms_per_trial']:.3f} ms per trial), {cut['cell_steps_per_second'] / 1e9:.2f} "
+            f"G cell-steps/s; {sc.cut.ncut} cut, {sc.cut.nsmall} small cells")
     # CPU baseline: the oracle on one case of 256 x 256.
     cpu_n, cpu_steps = 256, 0
     c = blast_batch(cpu_n, 1)
@@ -234,6 +263,7 @@ def legacy_workload(args) -> dict:
         "cpu_baseline": {"value": round(cpu_rate), "unit": "cell-steps/s", "cores": 1, "kind": "port",
                          "sample": f"{cpu_steps} steps of one {cpu_n}x{cpu_n} blast case on the FP64 oracle"},
         "launch": info,
+        "detail": {"with_body": cut},
     }
 
 
@@ -353,6 +383,8 @@ def main():
                     help="bench the legacy tensor-product grid solver instead (its own JSON line)")
     ap.add_argument("--legacy-n", type=int, default=1024, help="legacy grid: cells per side")
     ap.add_argument("--legacy-cases", type=int, default=16, help="legacy grid: cases in the batch")
+    ap.add_argument("--legacy-cut", action=argparse.BooleanOptionalAction, default=True,
+                    help="legacy grid: also time the batch with a solid body (cut cells, SRD)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU (this build has no multi-GPU path); got --gpus {args.gpus}",
