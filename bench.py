@@ -20,7 +20,7 @@ and on-device checks included).  The plain path is measured the same way: one FP
 mesh solved in FP64 (every mesh by default, --fp64-check; an FP64 batch, bitwise equal to FP64
 sessions, untimed): max and p99 of |dCL|, |dCD|, |dCM|, and the rejected trials of both runs.
 
-latency: a bounded serving leg (--serve-leg requests, config-3 airfoils on LF1 / LF2 / level-7
+latency: a bounded serving leg (--serve-leg requests, seeded airfoils on LF1 / LF2 / level-7
 meshes, 50/35/15 %, Poisson arrivals at 0.8 x the estimated capacity, continuous batching):
 p50 / p99 time-to-steady and its solves/hour.
 
@@ -98,8 +98,8 @@ def build_one(spec: dict, preset: str = "LF2"):
 SERVE_MIX = (("LF1", 0.50, 48_000), ("LF2", 0.35, 90_000), ("L7", 0.15, 180_000))  # preset, share, ~steps
 
 
-def config1_workload(args) -> dict:
-    """Config 1: FP64 Euler, slip wall, HLLC, first order, SSP-RK3, CFL 0.15 on LF1 around a
+def euler_lf1_workload(args) -> dict:
+    """FP64 Euler, slip wall, HLLC, first order, SSP-RK3, CFL 0.15 on LF1 around a
     symmetric 12% NACA-4-style section (200 points), M 1.65, AoA 1.5 deg, farfield, to t* = 0.01,
     with SRD small cells and agglomerated.  The FP64 CPU oracle (one core) and the FP64 device
     session run the same case; steps must agree and the states within 1e-10 relative L2."""
@@ -135,20 +135,20 @@ def config1_workload(args) -> dict:
                            gpu_steps=gmeta["steps"], cpu_seconds=round(cpu, 4), gpu_seconds=round(gpu, 4),
                            cpu_cell_steps_per_s=round(n * meta["steps"] / cpu), gpu_cell_steps_per_s=round(
                                n * gmeta["steps"] / gpu), state_relative_l2=err)
-        log(f"config 1 {label}: {n} cells, {meta['steps']} steps (device {gmeta['steps']}), CPU {cpu:.3f} s, "
+        log(f"Euler LF1 {label}: {n} cells, {meta['steps']} steps (device {gmeta['steps']}), CPU {cpu:.3f} s, "
             f"device {gpu:.4f} s, relative L2 {err:.2e}")
         if gmeta["steps"] != meta["steps"] or err > 1e-10:
-            raise SystemExit(f"config 1 {label}: device result differs from the oracle")
+            raise SystemExit(f"Euler LF1 {label}: device result differs from the oracle")
     cpu_rate = rows["agglomerated"]["cpu_cell_steps_per_s"]
     return {
-        "metric": "config 1 FP64 Euler LF1 NACA 0012 to t*=0.01: cell-steps per second", "unit": "cell-steps/s",
+        "metric": "FP64 Euler LF1 NACA 0012 to t*=0.01: cell-steps per second", "unit": "cell-steps/s",
         "value": rows["agglomerated"]["gpu_cell_steps_per_s"], "n_gpus": 1, "steps": rows["agglomerated"]["steps"],
         "warmup": 1, "ms_per_step": round(rows["agglomerated"]["gpu_seconds"] / rows["agglomerated"]["steps"] * 1e3, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: symmetric 12% NACA-4-style section, uniform freestream start",
-        "config": {"workload": "config 1: FP64 Euler LF1, t*=0.01, SRD and agglomerated", "parallelism": "single GPU"},
+        "config": {"workload": "FP64 Euler LF1 NACA 0012, t*=0.01, SRD and agglomerated", "parallelism": "single GPU"},
         "cpu_baseline": {"value": cpu_rate, "unit": "cell-steps/s", "cores": 1, "kind": "port",
-                         "sample": f"the full config 1 agglomerated run on the FP64 oracle ({rows['agglomerated']['steps']} steps)"},
+                         "sample": f"the full agglomerated run on the FP64 oracle ({rows['agglomerated']['steps']} steps)"},
         "detail": rows,
     }
 
@@ -268,7 +268,7 @@ def legacy_workload(args) -> dict:
 
 
 def serve_workload(args, n_requests: int | None = None) -> dict:
-    """Continuous batching: requests (config-3 airfoils on LF1 / LF2 / level-7 meshes, 50/35/15 %)
+    """Continuous batching: requests (seeded airfoils on LF1 / LF2 / level-7 meshes, 50/35/15 %)
     arrive as a Poisson stream and are admitted into an arena of slot classes sized by the HBM
     budget; each runs to the steady criteria.  Reports p50/p99 time-to-steady and solves/hour."""
     from quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd import batch as B
@@ -372,8 +372,8 @@ def main():
     ap.add_argument("--graph", action=argparse.BooleanOptionalAction, default=True,
                     help="replay captured hipGraphs in the timed steps (--no-graph: direct launches)")
     ap.add_argument("--tile", type=int, default=0, help="launch tile (0: autotune over 64/128/256)")
-    ap.add_argument("--config1", action="store_true",
-                    help="instead of the batch workload: config 1 (FP64 Euler LF1, t*=0.01) on the oracle and the device")
+    ap.add_argument("--euler-lf1", action="store_true",
+                    help="instead of the batch workload: FP64 Euler on LF1 to t*=0.01 on the oracle and the device")
     ap.add_argument("--serve", type=int, default=0,
                     help="instead of the batch workload: serve this many Poisson-arriving requests (LF1/LF2/L7 mix)")
     ap.add_argument("--serve-slots", type=int, default=192, help="slots of the serving arena")
@@ -403,8 +403,8 @@ def main():
     if device_count() < 1:
         raise SystemExit("bench.py needs a HIP device")
     lib = N.load_library()
-    if args.config1:
-        print(json.dumps(config1_workload(args)), flush=True)
+    if args.euler_lf1:
+        print(json.dumps(euler_lf1_workload(args)), flush=True)
         return
     if args.serve:
         print(json.dumps(serve_workload(args)), flush=True)
@@ -524,7 +524,7 @@ def main():
         # per-step time of the single sessions (the session's own tile may round the face sums in
         # another order than the batch's, so its step count can differ by a few).
         like = None
-        singles = [dict(mesh="config-2 airfoil", steps=int(r64["steps"]), wall_seconds=round(wall64, 3),
+        singles = [dict(mesh="single-case airfoil", steps=int(r64["steps"]), wall_seconds=round(wall64, 3),
                         us_per_step=round(wall64 / r64["steps"] * 1e6, 2))]
         if deltas is not None and args.like_samples > 0:
             pick = np.random.default_rng(args.seed + 1).choice(k, size=min(args.like_samples, k), replace=False)

@@ -1,12 +1,12 @@
-"""Configs 3 and 5 at test scale on the GPU.
+"""The ragged batch and the serving loop at test scale on the GPU.
 
-Config 3 (ragged batch): airfoils drawn as the workload draws them (thickness U[0.06, 0.14],
+Ragged batch: airfoils drawn as the workload draws them (thickness U[0.06, 0.14],
 camber U[0, 0.04], camber position U[0.3, 0.5], M U[1.2, 2.0], AoA U[-2, 4] deg), meshes LF1 or
 LF2 50/50, SA-neg, FP32 work arrays, tile 128, three trial branches.  Every slot equals a
 one-slot arena bit for bit and the FP64 oracle within 2e-4 relative L2 at t* = 0.01 (step counts
 within 1); a 256-mesh mixed arena is advanced a fixed number of trials and every slot checked.
 
-Config 5 (serving): SA-neg requests on LF1 / LF2 / level-7 meshes served to steady from an arena
+Serving: SA-neg requests on LF1 / LF2 / level-7 meshes served to steady from an arena
 of three slot classes sized through the HBM-budget path; each result equals the same request
 solved alone, bit for bit.  An FP64 serving window matches the oracle within 1e-10.
 """
@@ -69,9 +69,9 @@ def one_slot(mesh, params, u0, tile, target=None, trials=None):
         return b.download(0)
 
 
-def test_config3_mixed_batch_fp32():
+def test_ragged_mixed_batch_fp32():
     items, presets = mixed(16, 303)
-    print("config 3: presets", presets, "cells", [m.n_cells for _, m, _, _ in items], flush=True)
+    print("ragged batch: presets", presets, "cells", [m.n_cells for _, m, _, _ in items], flush=True)
     assert 4 <= presets.count("LF1") <= 12
     with B.Batch.of_meshes([m for _, m, _, _ in items], [p for _, _, p, _ in items], [u for _, _, _, u in items],
                            precision=32, tile=128) as b:
@@ -88,13 +88,13 @@ def test_config3_mixed_batch_fp32():
         err = max(per_variable_l2(u, oracle_u, 5))
         worst = max(worst, err)
         assert err <= 2e-4, (k, err)
-    print("config 3: worst FP32 vs FP64 oracle relative L2", worst, flush=True)
+    print("ragged batch: worst FP32 vs FP64 oracle relative L2", worst, flush=True)
 
 
-def test_config3_256_mesh_arena_fixed_trials():
+def test_ragged_256_mesh_arena_fixed_trials():
     items, presets = mixed(256, 2027)
     trials = 120
-    print("config 3: 256 meshes,", presets.count("LF1"), "LF1,", sum(m.n_cells for _, m, _, _ in items), "cells",
+    print("ragged batch: 256 meshes,", presets.count("LF1"), "LF1,", sum(m.n_cells for _, m, _, _ in items), "cells",
           flush=True)
     with B.Batch.of_meshes([m for _, m, _, _ in items], [p for _, _, p, _ in items], [u for _, _, _, u in items],
                            precision=32, tile=128) as b:
@@ -118,7 +118,7 @@ def test_config3_256_mesh_arena_fixed_trials():
 SERVE_MIX = (("LF1", 2), ("LF2", 2), ("L7", 1))
 
 
-def test_config5_serving_sa_neg_to_steady_equals_alone():
+def test_serving_sa_neg_to_steady_equals_alone():
     specs = draw(5, 505)
     presets = [p for p, count in SERVE_MIX for _ in range(count)]
     arrivals = S.poisson_arrivals(5.0, len(specs), seed=11)
@@ -138,7 +138,7 @@ def test_config5_serving_sa_neg_to_steady_equals_alone():
         fit = B.slots_fitting(caps[preset], features, budget // 3, precision=32, tile=64)
         slot_caps += [caps[preset]] * min(2, fit)
     assert len(slot_caps) >= 3 and B.arena_bytes(slot_caps, features, 32, 64) <= budget * 1.01
-    print("config 5: slots", len(slot_caps), "cells", [m.n_cells for m, _, _ in built], flush=True)
+    print("serving: slots", len(slot_caps), "cells", [m.n_cells for m, _, _ in built], flush=True)
     backend = S.DeviceBackend(slot_caps, features, precision=32, tile=64)
     try:
         rep = S.Server(backend, mode="steady", budget=600, workers=4,
@@ -158,7 +158,7 @@ def test_config5_serving_sa_neg_to_steady_equals_alone():
             assert got[key] == ref[key], key
 
 
-def test_config5_fp64_serving_window_matches_oracle():
+def test_fp64_serving_window_matches_oracle():
     specs = draw(3, 55)
     presets = ["LF1", "LF2", "LF1"]
     reqs = [S.Request(i, polygon(specs[i]), case_of(specs[i], presets[i]), 0.01 * i) for i in range(3)]

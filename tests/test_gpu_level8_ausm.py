@@ -20,7 +20,7 @@ OVER = {"freestream": {"Mach": 0.75, "AOA_degrees": 2.0}, "physics": {"turbulenc
         "solver": {"flux": "ausm"}, "boundaries": {"xmax": "pressure-outlet"}}
 
 
-def config4():
+def level8_ausm_case():
     rng = np.random.default_rng(4)
     camber, position = rng.uniform(0.02, 0.04), rng.uniform(0.3, 0.5)
     case, mesh, params = case_mesh(naca_polygon(0.14, camber, position), "L8", OVER)
@@ -35,8 +35,8 @@ def wall_cp(mesh, u, case):
     return (p[owner] - 1.0 / case.gamma) / case.q_inf
 
 
-def test_config4_fp64_matches_oracle_over_30_steps():
-    case, mesh, params, u0 = config4()
+def test_level8_ausm_fp64_matches_oracle_over_30_steps():
+    case, mesh, params, u0 = level8_ausm_case()
     print("L8 cells", mesh.n_cells, flush=True)
     ref, meta = O.run(mesh, params, u0, 1e30, maxsteps=30)
     assert meta["steps"] == 30
@@ -49,8 +49,8 @@ def test_config4_fp64_matches_oracle_over_30_steps():
     assert max(errs) <= 1e-10
 
 
-def test_config4_fp32_against_fp64():
-    case, mesh, params, u0 = config4()
+def test_level8_ausm_fp32_against_fp64():
+    case, mesh, params, u0 = level8_ausm_case()
     frame = dict(region=(-1.0, 3.0, -1.0, 1.0), moment=(0.25, 0.0), mach=case.mach, aoa=case.aoa)
     out = {}
     for prec in (64, 32):

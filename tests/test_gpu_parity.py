@@ -82,8 +82,8 @@ def test_published_step_counts(preset):
         assert s.meta["steps"] == expected and s.meta["rejected"] == 0
 
 
-def test_config1_euler_naca0012_srd():
-    """Config 1 geometry: symmetric 12% NACA-style section, Euler, SRD small cells, t* = 0.01."""
+def test_euler_lf1_naca0012_srd():
+    """The Euler LF1 case: symmetric 12% NACA-style section, Euler, SRD small cells, t* = 0.01."""
     case, mesh, params = case_mesh(naca_polygon(0.12), "LF1", merge(EULER, SRD))
     assert mesh.info.small_cells > 0
     u0 = mesh.initialize(params, case.freestream)
