@@ -118,6 +118,10 @@ struct Grid {
   const int* grp_off;
   const int* grp_idx;
   double* Qh;  // ncase x nsmall x 4 neighbourhood averages
+  // The interior cut cells (plane indices, ascending): the first RK stage is tried on them alone
+  // before the whole-grid pass (k_stage1_precheck).
+  const int* pre_cell;
+  int n_pre;
   Control* ctrl;
 };
 
@@ -900,6 +904,62 @@ __global__ __launch_bounds__(kLanes) void k_be_check(Grid g) {
 // admissible: neighbourhood averages over the members, then every rewritten cell sums its own
 // value (regular cells) and the averages of the neighbourhoods it belongs to, in ascending
 // order, and divides by its count.  MODE picks the candidate buffer as the stage kernels do.
+// A trial is rejected when the first stage's candidate is inadmissible in any cell (tested
+// before the redistribution, src/euler2d_cuda.cu:648-651); on a cut grid that happens in the
+// smallest cut cells, several times per step, until the step has been halved enough.  This
+// kernel forms the first-stage candidate of every cut cell (and case) alone, with k_stage<1>'s
+// formulas in its order (so the same bits: neighbours are interior, the body keeps three cells
+// from every side), and raises the trial's flag if one is inadmissible; the stage kernels then
+// return at once.  The decision is the one the whole-grid pass would take.
+__global__ __launch_bounds__(kThreads) void k_stage1_precheck(Grid g) {
+  const Control* c = g.ctrl;
+  if (__hip_atomic_load(&c->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != RUNNING ||
+      __hip_atomic_load(&c->bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+    return;
+  const double* in = g.B[c->cur];
+  const size_t plane = g.plane;
+  const size_t n = static_cast<size_t>(g.ncase) * g.n_pre;
+  bool bad = false;
+  for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
+       t += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const size_t icase = t / g.n_pre;
+    const int kc = g.pre_cell[t - icase * g.n_pre];
+    const int jr = kc / g.mx - kGhosts, i = kc % g.mx - kGhosts;
+    const double* u = in + icase * 4 * plane;
+    const double mco = g.mco[icase], dt = g.dt[icase];
+    const auto prim_at = [&](int k) {
+      return g.vol[k] <= 0.0 ? Prim{1.0, 0.0, 0.0, 1.0}
+                             : primitive(u[k], u[k + plane], u[k + 2 * plane], u[k + 3 * plane], g.gamma);
+    };
+    const Prim w = prim_at(kc);
+    const double ax = g.axl[static_cast<size_t>(jr) * (g.nx + 1) + i];
+    const double ae = g.axl[static_cast<size_t>(jr) * (g.nx + 1) + i + 1];
+    const double as = g.ayl[static_cast<size_t>(jr) * g.nx + i];
+    const double an = g.ayl[static_cast<size_t>(jr + 1) * g.nx + i];
+    Flux fw{0.0, 0.0, 0.0, 0.0}, fe{0.0, 0.0, 0.0, 0.0}, gs{0.0, 0.0, 0.0, 0.0}, gn{0.0, 0.0, 0.0, 0.0};
+    if (ax > 0.0) fw = face_flux<true>(g, mco, prim_at(kc - 1), w);
+    if (ae > 0.0) fe = face_flux<true>(g, mco, w, prim_at(kc + 1));
+    if (as > 0.0) gs = face_flux<false>(g, mco, prim_at(kc - g.mx), w);
+    if (an > 0.0) gn = face_flux<false>(g, mco, w, prim_at(kc + g.mx));
+    const Flux xres{ae * fe.m - ax * fw.m, ae * fe.x - ax * fw.x, ae * fe.y - ax * fw.y, ae * fe.e - ax * fw.e};
+    const double iv = 1.0 / g.vol[kc], pw = w.p;
+    const double r0 = -iv * (xres.m + an * gn.m - as * gs.m);
+    double r1 = -iv * (xres.x + an * gn.x - as * gs.x);
+    double r2 = -iv * (xres.y + an * gn.y - as * gs.y);
+    const double r3 = -iv * (xres.e + an * gn.e - as * gs.e);
+    r1 -= iv * pw * g.wax[kc];
+    r2 -= iv * pw * g.way[kc];
+    constexpr double A = 0.0, B = 1.0;  // the first stage's combination, as k_stage<1> forms it
+    const double q0 = u[kc], q1 = u[kc + plane], q2 = u[kc + 2 * plane], q3 = u[kc + 3 * plane];
+    const double n0 = A * q0 + B * (q0 + dt * r0);
+    const double n1 = A * q1 + B * (q1 + dt * r1);
+    const double n2 = A * q2 + B * (q2 + dt * r2);
+    const double n3 = A * q3 + B * (q3 + dt * r3);
+    if (inadmissible(n0, n1, n2, n3, g.gamma)) bad = true;
+  }
+  if (__any(bad) && threadIdx.x % kLanes == 0) atomicOr(&g.ctrl->bad, 1);
+}
+
 template <int MODE>
 __device__ __forceinline__ double* stage_out(const Grid& g, int cur) {
   return MODE == 2 ? g.B[2] : g.B[cur ^ 1];
@@ -1089,8 +1149,11 @@ void launch_stage(ugrid_session* s, hipStream_t stream, const double* in = nullp
 // A stage kernel followed, on a cut grid with small cells, by the redistribution of its candidate.
 template <int MODE>
 void launch_stage_srd(ugrid_session* s, hipStream_t stream) {
-  launch_stage<MODE>(s, stream);
   const Grid& g = s->g;
+  if (MODE == 1 && g.has_cut && g.n_pre > 0)
+    hipLaunchKernelGGL(k_stage1_precheck, dim3(grid_blocks(g, static_cast<size_t>(g.ncase) * g.n_pre)), dim3(kThreads), 0,
+                       stream, g);
+  launch_stage<MODE>(s, stream);
   if (!g.has_cut || g.nsmall == 0) return;
   hipLaunchKernelGGL(k_srd_avg<MODE>, dim3(grid_blocks(g, static_cast<size_t>(g.ncase) * g.nsmall)), dim3(kThreads), 0,
                      stream, g);
@@ -1182,11 +1245,18 @@ bool upload_cut(ugrid_session* s, const ugrid_cut_view& c, char* err, size_t len
       grp_idx.insert(grp_idx.end(), groups[k].begin(), groups[k].end());
       grp_off.push_back(static_cast<int>(grp_idx.size()));
     }
+  std::vector<int> pre_cell;  // interior cut cells, for the first-stage precheck
+  for (int j = 0; j < g.ny; ++j)
+    for (int i = 0; i < g.nx; ++i) {
+      const size_t k = static_cast<size_t>(j + kGhosts) * g.mx + i + kGhosts;
+      if (c.flag[k] != 0.0 && c.vol[k] > 0.0) pre_cell.push_back(static_cast<int>(k));
+    }
   g.has_cut = 1;
   g.nsmall = c.nsmall;
   g.n_srd = static_cast<int>(srd_cell.size());
+  g.n_pre = static_cast<int>(pre_cell.size());
   double *vol, *flag, *axl, *ayl, *wax, *way, *nbr, *pos, *svhat, *qh;
-  int *moff, *midx, *scell, *goff, *gidx;
+  int *moff, *midx, *scell, *goff, *gidx, *pcell;
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream), err, len, "upload cut geometry");
   };
@@ -1198,7 +1268,8 @@ bool upload_cut(ugrid_session* s, const ugrid_cut_view& c, char* err, size_t len
       !device_alloc(s, &svhat, nq, err, len) || !device_alloc(s, &qh, nq * 4 * g.ncase, err, len) ||
       !device_alloc(s, &moff, mem_off.size(), err, len) || !device_alloc(s, &midx, std::max<size_t>(mem_idx.size(), 1), err, len) ||
       !device_alloc(s, &scell, std::max<size_t>(srd_cell.size(), 1), err, len) ||
-      !device_alloc(s, &goff, grp_off.size(), err, len) || !device_alloc(s, &gidx, std::max<size_t>(grp_idx.size(), 1), err, len))
+      !device_alloc(s, &goff, grp_off.size(), err, len) || !device_alloc(s, &gidx, std::max<size_t>(grp_idx.size(), 1), err, len) ||
+      !device_alloc(s, &pcell, std::max<size_t>(pre_cell.size(), 1), err, len))
     return false;
   const size_t d = sizeof(double), n = sizeof(int);
   if (!up(vol, c.vol, cells * d) || !up(flag, c.flag, cells * d) || !up(axl, c.axl, nfx * d) ||
@@ -1208,6 +1279,7 @@ bool upload_cut(ugrid_session* s, const ugrid_cut_view& c, char* err, size_t len
       (!mem_idx.empty() && !up(midx, mem_idx.data(), mem_idx.size() * n)) ||
       (!srd_cell.empty() && !up(scell, srd_cell.data(), srd_cell.size() * n)) ||
       !up(goff, grp_off.data(), grp_off.size() * n) || (!grp_idx.empty() && !up(gidx, grp_idx.data(), grp_idx.size() * n)) ||
+      (!pre_cell.empty() && !up(pcell, pre_cell.data(), pre_cell.size() * n)) ||
       !ok(hipStreamSynchronize(s->stream), err, len, "upload cut geometry"))
     return false;
   g.vol = vol;
@@ -1225,6 +1297,7 @@ bool upload_cut(ugrid_session* s, const ugrid_cut_view& c, char* err, size_t len
   g.srd_cell = scell;
   g.grp_off = goff;
   g.grp_idx = gidx;
+  g.pre_cell = pcell;
   return true;
 }
 
