@@ -335,6 +335,25 @@ def test_cut_run_split_calls_and_stops():
     assert_same_run(device_run(s, 1e6, 5), ora)
 
 
+def test_cut_rejections_away_from_the_body():
+    """Gas at rest around the body and a strong blast far from it under a fixed step: the first
+    stage fails outside the cut cells (their pre-check passes, the whole-grid stage rejects), so
+    both halves of the rejection test decide; then a step so large that every trial fails."""
+    s, px, py = cut_body("circle", 48, 40, params(bc=ALL_BCS[1], dt_fixed=0.004),
+                         states=((1.0, 0.0, 0.0, 1.0), (1.0, 0.0, 0.0, 1.0)))
+    X = np.broadcast_to(s.grid.xc[2:-2][None, :], (40, 48))
+    for c, p_blast in enumerate((1000.0, 50.0)):
+        s.set_primitive_field(c, 1.0, 0.0, 0.0, np.where(X < -0.75, p_blast, 1.0))
+    print("cut rejections away from the body", flush=True)
+    ora = G.run(s, 0.05, 40)
+    assert ora["status"] == 1 and ora["nreject"] > 40
+    assert_same_run(device_run(s, 0.05, 40), ora)
+    s.par.dt_fixed = 1e6
+    ora = G.run(s, 1e6, 5)
+    assert ora["status"] == 2 and ora["nreject"] == 13
+    assert_same_run(device_run(s, 1e6, 5), ora)
+
+
 def test_cut_backward_euler_bitwise():
     s, px, py = cut_body("circle", 52, 40, implicit_params(bc=ALL_BCS[4]), shock=-0.6)
     print("cut backward Euler", flush=True)
