@@ -356,8 +356,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--fp64-check", type=int, default=-1,
                     help="meshes re-solved in FP64 for dCL/dCD/dCM (-1: every mesh of the batch)")
-    ap.add_argument("--few-slots", type=int, default=16,
-                    help="running slots at or below which the batch's flux launches take two threads per cell")
+    ap.add_argument("--few-slots", type=int, default=-1,
+                    help="running slots at or below which the batch's flux launches take two threads per cell "
+                         "(-1: autotune, 0: never)")
     ap.add_argument("--like-samples", type=int, default=1,
                     help="batch airfoils (seeded choice) also solved as plain FP64 single sessions")
     ap.add_argument("--serve-leg", type=int, default=48,
@@ -432,6 +433,13 @@ def main():
     else:
         tile, tile_times = AT.pick_tile(meshes, params, states, precision=32, trials=30, warmup=10)
     log("tile us/step: " + ", ".join(f"{t}: {s * 1e6:.1f}" for t, s in tile_times.items()) + f" -> {tile}")
+    # Few-slot launch shape (two flux threads per cell once at most few_slots slots run).
+    if args.few_slots < 0:
+        few_slots, few_times = AT.pick_few_slots(meshes, params, states, precision=32, tile=tile)
+        log("few-slot trial us (1 / 2 flux threads per cell): " +
+            ", ".join(f"{k} running: {a} / {b}" for k, (a, b) in sorted(few_times.items())) + f" -> {few_slots}")
+    else:
+        few_slots, few_times = args.few_slots, {}
 
     # 2. Timed steps of the batch: exactly --steps graph replays after --warmup.
     with B.Batch.of_meshes(meshes, params, states, precision=32, tile=tile, graph=args.graph) as b:
@@ -473,8 +481,8 @@ def main():
     if args.solve:
         # 3. Batched FP32 steady solve of the whole batch (the headline value).
         with B.Batch.of_meshes(meshes, params, states, precision=32, tile=tile) as b:
-            if tile * 2 <= 256 and args.few_slots > 0:
-                b.set_launch_shape(1, args.few_slots, 2)  # two flux threads per cell once few slots run
+            if tile * 2 <= 256 and few_slots > 0:
+                b.set_launch_shape(1, few_slots, 2)  # two flux threads per cell once few slots run
             t0 = time.perf_counter()
             res32 = b.solve(cases, mode="steady", budget=args.budget)
             wall32 = time.perf_counter() - t0
@@ -644,8 +652,8 @@ def main():
             "batch": args.batch, "seed": args.seed, "mesh": "LF2 (6 levels, wall h/c 1/128)",
             "cells_total": int(sum(cells)), "cells_min": int(min(cells)), "cells_max": int(max(cells)),
             "tile": tile, "parallelism": "single GPU",
-            "few_slot_flux_threads_per_cell": {"slots": args.few_slots, "threads": 2} if tile * 2 <= 256 and
-            args.few_slots > 0 else None,
+            "few_slot_flux_threads_per_cell": {"slots": few_slots, "threads": 2, "probe_us": few_times}
+            if tile * 2 <= 256 and few_slots > 0 else None,
         },
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": RL.HBM_PEAK_GBPS, "unit": "GB/s",

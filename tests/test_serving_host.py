@@ -64,6 +64,15 @@ def test_batch_without_device_raises():
         B.Batch.with_slots([cap], feat)
 
 
+def test_autotuner_few_slot_threshold():
+    # two threads per cell must win at every probe up to the threshold (measured shape: they win
+    # at 16 running slots and below, lose from 32 up)
+    assert AT.choose_few({2: (73.0, 69.0), 8: (85.1, 81.1), 16: (107.0, 102.5), 32: (129.5, 138.4)}) == 16
+    assert AT.choose_few({2: (73.0, 75.0), 8: (85.1, 81.1)}) == 0
+    assert AT.choose_few({2: (73.0, 70.0), 8: (85.1, 86.0), 16: (107.0, 102.5)}) == 2
+    assert AT.choose_few({}) == 0
+
+
 def test_autotuner_keeps_the_larger_tile_within_the_tie_margin():
     # A clear winner is kept whatever its size.
     assert AT.choose({64: 1.00e-3, 128: 1.10e-3, 256: 1.20e-3}) == 64
