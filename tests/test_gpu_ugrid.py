@@ -354,12 +354,23 @@ def test_cut_rejections_away_from_the_body():
     assert_same_run(device_run(s, 1e6, 5), ora)
 
 
-def test_cut_backward_euler_bitwise():
-    s, px, py = cut_body("circle", 52, 40, implicit_params(bc=ALL_BCS[4]), shock=-0.6)
-    print("cut backward Euler", flush=True)
-    ora = G.run_implicit(s, 0.1, 100)
+CUT_BE_RUNS = {
+    "circle_hllc": lambda: (cut_body("circle", 52, 40, implicit_params(bc=ALL_BCS[4]), shock=-0.6), 0.1),
+    "wedge_ausm_periodic_stretched": lambda: (cut_body("wedge", 44, 38, implicit_params(
+        flux=E.FLUX_AUSMPUP, bc=ALL_BCS[0], sx=(E.STRETCH_TANH_ENDS, 1.2)), ((1.0, 0.5, 0.0, 1.0), (1.0, 0.2, 0.1, 1.5))),
+        0.05),
+    "ellipse_reflect_two_cases": lambda: (cut_body("ellipse", 40, 36, implicit_params(bc=ALL_BCS[2], cfl=1.5),
+                                                   ((1.0, 0.3, 0.0, 1.0), (0.8, 0.0, 0.2, 2.0)), shock=-0.5), 0.05),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CUT_BE_RUNS))
+def test_cut_backward_euler_bitwise(name):
+    (s, px, py), tend = CUT_BE_RUNS[name]()
+    print(f"cut backward Euler {name}", flush=True)
+    ora = G.run_implicit(s, tend, 100)
     assert ora["status"] == 0 and ora["updates"] > ora["nsteps"] > 1
-    assert_same_implicit(device_implicit(s, 0.1, 100), ora)
+    assert_same_implicit(device_implicit(s, tend, 100), ora)
 
 
 def test_cut_python_mirror_and_refusal():

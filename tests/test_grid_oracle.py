@@ -471,10 +471,21 @@ def test_cut_run_matches_reference_within_roundoff(name):
     assert not early.all()
 
 
+CUT_BE_RUNS = {
+    "circle_hllc": lambda: (cut_body("circle", 40, 32, implicit_params(bc=ALL_BCS[4]), shock=-0.6), 0.1),
+    "wedge_ausm_periodic_stretched": lambda: (cut_body("wedge", 44, 38, implicit_params(
+        flux=E.FLUX_AUSMPUP, bc=ALL_BCS[0], sx=(E.STRETCH_TANH_ENDS, 1.2)), ((1.0, 0.5, 0.0, 1.0), (1.0, 0.2, 0.1, 1.5))),
+        0.05),
+    "ellipse_reflect_two_cases": lambda: (cut_body("ellipse", 40, 36, implicit_params(bc=ALL_BCS[2], cfl=1.5),
+                                                   ((1.0, 0.3, 0.0, 1.0), (0.8, 0.0, 0.2, 2.0)), shock=-0.5), 0.05),
+}
+
+
 @needs_ref
-def test_cut_backward_euler_matches_reference_within_roundoff():
-    s, px, py = cut_body("circle", 40, 32, implicit_params(bc=ALL_BCS[4]), shock=-0.6)
-    a, b = G.run_implicit(s, 0.1, 100), G.ref_cut_run(s, px, py, 0.1, 100)
+@pytest.mark.parametrize("name", sorted(CUT_BE_RUNS))
+def test_cut_backward_euler_matches_reference_within_roundoff(name):
+    (s, px, py), tend = CUT_BE_RUNS[name]()
+    a, b = G.run_implicit(s, tend, 100), G.ref_cut_run(s, px, py, tend, 100)
     assert a["status"] == b["status"] == 0
     assert (a["nsteps"], a["updates"], a["retries"]) == (b["nsteps"], b["updates"], b["retries"])
     assert a["updates"] > a["nsteps"]
