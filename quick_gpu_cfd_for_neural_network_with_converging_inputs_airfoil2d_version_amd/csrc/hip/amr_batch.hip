@@ -258,18 +258,22 @@ void release_graphs(amr_batch* b) {
   b->graphs.clear();
 }
 
-// The graphs of both launch shapes (the few-slot set only when it differs).
+// The graphs of both launch shapes (the few-slot set only when it differs).  On failure no
+// graph is kept (trials are then launched directly), so no partial set is ever replayed.
 int capture_all(amr_batch* b, char* err, size_t err_len) {
-  if (!b->arena->set_flux_spread(b->spread)) return fail(err, err_len, "invalid AMR launch shape");
-  if (capture_graphs(b, &b->execs, &b->execs_batch, err, err_len)) return 1;
-  if (b->few_slots > 0 && b->few_spread != b->spread) {
+  bool good = b->arena->set_flux_spread(b->spread);
+  if (!good) fail(err, err_len, "invalid AMR launch shape");
+  good = good && !capture_graphs(b, &b->execs, &b->execs_batch, err, err_len);
+  if (good && b->few_slots > 0 && b->few_spread != b->spread) {
     const size_t nodes = b->stats.graph_nodes_per_trial;
-    if (!b->arena->set_flux_spread(b->few_spread) || capture_graphs(b, &b->execs_few, &b->execs_batch_few, err, err_len))
-      return fail(err, err_len, "invalid AMR launch shape");
+    good = b->arena->set_flux_spread(b->few_spread);
+    if (!good) fail(err, err_len, "invalid AMR launch shape");
+    good = good && !capture_graphs(b, &b->execs_few, &b->execs_batch_few, err, err_len);
     b->stats.graph_nodes_per_trial = nodes;
   }
   b->arena->set_flux_spread(b->spread);
-  return 0;
+  if (!good) release_graphs(b);
+  return good ? 0 : 1;
 }
 
 
