@@ -48,6 +48,8 @@ constexpr int kThreads = kLanes * kWaves;
 constexpr int kGhosts = 2;
 constexpr int kXcds = 8;
 constexpr int kMaxRetries = 12;
+// First-stage pre-checks per trial on a cut grid (the body bench needs about six halvings a step).
+constexpr int kPrechecks = 8;
 enum : int { BC_PERIODIC = 0, BC_TRANSMISSIVE = 1, BC_REFLECT = 2 };
 enum : int { RUNNING = -1 };
 // Backward-Euler phases of a physical step (each kernel of the replayed sequence acts only in its
@@ -69,7 +71,8 @@ struct Control {
   int cur;         // B[cur] holds the accepted state
   int ghost_from;  // download ghosts: 0 as uploaded, 1 boundary fill of B[cur], 2 of B[cur ^ 1]
   // backward Euler
-  int phase, iter, pretries, qbuf, reason, pad;
+  int phase, iter, pretries, qbuf, reason;
+  int pre_done;    // SSP-RK3 on a cut grid: this trial's first-stage pre-check has passed
   double pseudo;   // pseudo CFL of the current physical step
   long long updates, iretries, checks;
   double initial_defect, final_defect, tolerance;
@@ -309,28 +312,36 @@ __device__ __forceinline__ int logical_block() {
 // ---------------------------------------------------------------------------------------------
 // Commit of a trial (one wave, after the last stage): src/euler2d_cuda.cu:670-693.
 // ---------------------------------------------------------------------------------------------
+// Rejected trial (one wave): halve every positive step; give up after 12 retries or on an
+// unusable step.
+__device__ void reject(const Grid& g) {
+  Control* c = g.ctrl;
+  const int lane = threadIdx.x % kLanes;
+  int unusable = 0;
+  for (int k = lane; k < g.ncase; k += kLanes) {
+    double d = g.dt[k];
+    if (d > 0.0) d *= 0.5;
+    g.dt[k] = d;
+    if (d < 0.0 || !isfinite(d)) unusable = 1;
+    g.smax[k] = 0;
+  }
+  unusable = __any(unusable);
+  if (lane == 0) {
+    const int retries = c->retries + 1;
+    c->retries = retries;
+    c->nreject += 1;
+    if (retries > kMaxRetries || unusable) c->status = 2;
+    c->bad = 0;
+    c->pre_done = 0;
+    c->events += 1;
+  }
+}
+
 __device__ void commit(const Grid& g) {
   Control* c = g.ctrl;
   const int lane = threadIdx.x % kLanes;
   if (__hip_atomic_load(&c->bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-    // Rejected: halve every positive step; give up after 12 retries or on an unusable step.
-    int unusable = 0;
-    for (int k = lane; k < g.ncase; k += kLanes) {
-      double d = g.dt[k];
-      if (d > 0.0) d *= 0.5;
-      g.dt[k] = d;
-      if (d < 0.0 || !isfinite(d)) unusable = 1;
-      g.smax[k] = 0;
-    }
-    unusable = __any(unusable);
-    if (lane == 0) {
-      const int retries = c->retries + 1;
-      c->retries = retries;
-      c->nreject += 1;
-      if (retries > kMaxRetries || unusable) c->status = 2;
-      c->bad = 0;
-      c->events += 1;
-    }
+    reject(g);
     return;
   }
   // Accepted: advance the clocks of the cases that moved, then either stop (maxsteps reached,
@@ -359,6 +370,7 @@ __device__ void commit(const Grid& g) {
     c->cur ^= 1;
     c->nsteps += 1;
     c->retries = 0;
+    c->pre_done = 0;
     c->ghost_from = next ? 1 : 2;
     if (!next) c->status = maxed ? 1 : 0;
     c->events += 1;
@@ -615,7 +627,8 @@ __global__ __launch_bounds__(kThreads) void k_open(Grid g, double tend, long lon
     c->steps = 0;
     c->retries = 0;
     c->bad = 0;
-        if (maxsteps <= 0)
+    c->pre_done = 0;
+    if (maxsteps <= 0)
       c->status = 1;
     else if (!s_open)
       c->status = 0;
@@ -914,7 +927,7 @@ __global__ __launch_bounds__(kLanes) void k_be_check(Grid g) {
 __global__ __launch_bounds__(kThreads) void k_stage1_precheck(Grid g) {
   const Control* c = g.ctrl;
   if (__hip_atomic_load(&c->status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != RUNNING ||
-      __hip_atomic_load(&c->bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+      __hip_atomic_load(&c->bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || c->pre_done)
     return;
   const double* in = g.B[c->cur];
   const size_t plane = g.plane;
@@ -958,6 +971,19 @@ __global__ __launch_bounds__(kThreads) void k_stage1_precheck(Grid g) {
     if (inadmissible(n0, n1, n2, n3, g.gamma)) bad = true;
   }
   if (__any(bad) && threadIdx.x % kLanes == 0) atomicOr(&g.ctrl->bad, 1);
+}
+
+// After a pre-check (one wave): a failed candidate is rejected at once, as the trial's commit
+// would reject it (halve every positive step, count the retry); a passing one ends the
+// pre-checks of this trial.  A trial runs kPrechecks of these pairs before its first stage, so
+// a step that needs several halvings costs them here instead of a whole-grid pass each.
+__global__ __launch_bounds__(kLanes) void k_precheck_reject(Grid g) {
+  Control* c = g.ctrl;
+  if (c->status != RUNNING || c->pre_done) return;
+  if (__hip_atomic_load(&c->bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+    reject(g);
+  else if (threadIdx.x == 0)
+    c->pre_done = 1;
 }
 
 template <int MODE>
@@ -1150,9 +1176,15 @@ void launch_stage(ugrid_session* s, hipStream_t stream, const double* in = nullp
 template <int MODE>
 void launch_stage_srd(ugrid_session* s, hipStream_t stream) {
   const Grid& g = s->g;
-  if (MODE == 1 && g.has_cut && g.n_pre > 0)
-    hipLaunchKernelGGL(k_stage1_precheck, dim3(grid_blocks(g, static_cast<size_t>(g.ncase) * g.n_pre)), dim3(kThreads), 0,
-                       stream, g);
+  if (MODE == 1 && g.has_cut && g.n_pre > 0) {
+    // kPrechecks pre-checks, each failed one rejected at once; the last one's failure is left to
+    // the trial's commit (the stage kernels then return at once).
+    const dim3 pre_blocks(grid_blocks(g, static_cast<size_t>(g.ncase) * g.n_pre));
+    for (int r = 0; r < kPrechecks; ++r) {
+      hipLaunchKernelGGL(k_stage1_precheck, pre_blocks, dim3(kThreads), 0, stream, g);
+      if (r + 1 < kPrechecks) hipLaunchKernelGGL(k_precheck_reject, dim3(1), dim3(kLanes), 0, stream, g);
+    }
+  }
   launch_stage<MODE>(s, stream);
   if (!g.has_cut || g.nsmall == 0) return;
   hipLaunchKernelGGL(k_srd_avg<MODE>, dim3(grid_blocks(g, static_cast<size_t>(g.ncase) * g.nsmall)), dim3(kThreads), 0,
