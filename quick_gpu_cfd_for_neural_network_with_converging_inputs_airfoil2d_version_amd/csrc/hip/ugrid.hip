@@ -121,6 +121,12 @@ struct Grid {
   const int* grp_off;
   const int* grp_idx;
   double* Qh;  // ncase x nsmall x 4 neighbourhood averages
+  // One byte per plane cell for the stage kernels (so that they read the double arrays above only
+  // where a value is needed): bit 0 solid (vol <= 0), bit 1 cut (flag != 0), bit 2 rewritten by
+  // SRD (smallpos >= 0 or nbrcount != 1), bits 3-4 the west x face, bits 5-6 the south y face of
+  // the cell: 0 aperture +0, 1 aperture equal to the full width (dy, dx; the builders store that
+  // value for open faces), 2 another value (read from axl / ayl).
+  const uint8_t* code;
   // The interior cut cells (plane indices, ascending): the first RK stage is tried on them alone
   // before the whole-grid pass (k_stage1_precheck).
   const int* pre_cell;
@@ -387,6 +393,9 @@ __device__ void commit(const Grid& g) {
 //   MODE 4: R = L(Q), per case max over cells of max_v |dt R - (Q - U)| / max(1, |U|)
 //           (infinite for a non-finite defect), src/euler2d_cuda.cu:331-350
 // ---------------------------------------------------------------------------------------------
+// The cut-grid instantiation takes 140-144 VGPRs (3 waves per SIMD against the regular kernels'
+// 4); holding it to 4 waves spills 10-13 registers and was measured slower (2.40 against 2.37 ms
+// per step on 16 x 1024^2 with a body).
 template <int MODE, bool CUT>
 __global__ __launch_bounds__(kThreads) void k_stage(Grid g, const double* __restrict__ rhs_in,
                                                      double* __restrict__ rhs_out) {
@@ -437,13 +446,19 @@ __global__ __launch_bounds__(kThreads) void k_stage(Grid g, const double* __rest
     Flux gprev{}, xres{};  // xres: x residual of a full cell, or ae FX_e - aw FX_w of a cut cell
     double ay_prev = 1.0;  // aperture of the previous y face (cut grids)
     double q0 = 0, q1 = 0, q2 = 0, q3 = 0;  // conserved state of the owned cell of the previous row
+    unsigned code_prev = 0;                  // cut grids: code of the lane's cell of the previous row
+    const bool in_x = i >= 0 && i <= g.nx;   // the lane's west x face is a grid face
     for (int j = j0 - 1; j <= jl + 1; ++j) {
       bool flip_v = false;
       const int jc = map_axis(j, g.ny, g.bc[2], g.bc[3], &flip_v);
       const size_t k = case_base + static_cast<size_t>(jc + kGhosts) * g.mx;
       const double c0 = in[k], c1 = in[k + plane], c2 = in[k + 2 * plane], c3 = in[k + 3 * plane];
+      // Cut grids: codes of the mapped cell (solid or not) and of the lane's own column at row j
+      // (its faces, cut or not, rewritten by SRD or not).
+      const unsigned code_m = CUT ? g.code[static_cast<size_t>(jc + kGhosts) * g.mx + ic + kGhosts] : 0u;
+      const unsigned code_u = CUT && in_x ? g.code[static_cast<size_t>(j + kGhosts) * g.mx + i + kGhosts] : 0u;
       Prim w;
-      if (CUT && g.vol[static_cast<size_t>(jc + kGhosts) * g.mx + ic + kGhosts] <= 0.0) {
+      if (CUT && (code_m & 1u)) {
         w = Prim{1.0, 0.0, 0.0, 1.0};  // solid cell (never a ghost's source: the body keeps clear of the sides)
       } else {
         w = primitive(c0, c1, c2, c3, g.gamma);
@@ -453,7 +468,10 @@ __global__ __launch_bounds__(kThreads) void k_stage(Grid g, const double* __rest
       if (j >= j0) {
         // y face between rows j - 1 and j (closed faces carry no flux)
         double ay = 1.0;
-        if (CUT && i >= 0 && i < g.nx) ay = g.ayl[static_cast<size_t>(j) * g.nx + i];
+        if (CUT && i >= 0 && i < g.nx) {
+          const unsigned st = (code_u >> 5) & 3u;
+          ay = st == 0u ? 0.0 : st == 1u ? g.dx[i + kGhosts] : g.ayl[static_cast<size_t>(j) * g.nx + i];
+        }
         Flux gf{0.0, 0.0, 0.0, 0.0};
         if (!CUT || ay > 0.0) gf = face_flux<false>(g, mco, wprev, w);
         if (j > j0 && owner) {
@@ -462,10 +480,10 @@ __global__ __launch_bounds__(kThreads) void k_stage(Grid g, const double* __rest
           const size_t kc = static_cast<size_t>(j - 1 + kGhosts) * g.mx + i + kGhosts;
           double r0, r1, r2, r3;
           bool fluid = true, moved = false;
-          if (CUT && g.vol[kc] <= 0.0) {
+          if (CUT && (code_prev & 1u)) {
             fluid = false;
             r0 = r1 = r2 = r3 = 0.0;
-          } else if (!CUT || g.flag[kc] == 0.0) {
+          } else if (!CUT || !(code_prev & 2u)) {
             r0 = xres.m - (gf.m - gprev.m) / dyj;
             r1 = xres.x - (gf.x - gprev.x) / dyj;
             r2 = xres.y - (gf.y - gprev.y) / dyj;
@@ -479,7 +497,7 @@ __global__ __launch_bounds__(kThreads) void k_stage(Grid g, const double* __rest
             r1 -= iv * pw * g.wax[kc];
             r2 -= iv * pw * g.way[kc];
           }
-          if (CUT && MODE == 3) moved = g.smallpos[kc] >= 0.0 || g.nbrcount[kc] != 1.0;
+          if (CUT && MODE == 3) moved = (code_prev & 4u) != 0u;
           const size_t ko = case_base + static_cast<size_t>(j - 1 + kGhosts) * g.mx;
           if (MODE == 0 || MODE == 4) {
             out[ko] = r0;
@@ -538,18 +556,22 @@ __global__ __launch_bounds__(kThreads) void k_stage(Grid g, const double* __rest
         wl.p = shfl_up1(w.p);
         if (lane == 0) wl = w;
         double ax = 1.0;
-        if (CUT && i >= 0 && i <= g.nx) ax = g.axl[static_cast<size_t>(j) * (g.nx + 1) + i];
+        if (CUT && in_x) {
+          const unsigned st = (code_u >> 3) & 3u;
+          ax = st == 0u ? 0.0 : st == 1u ? g.dy[j + kGhosts] : g.axl[static_cast<size_t>(j) * (g.nx + 1) + i];
+        }
         Flux fw{0.0, 0.0, 0.0, 0.0};
         if (!CUT || ax > 0.0) fw = face_flux<true>(g, mco, wl, w);
         const Flux fe{shfl_down1(fw.m), shfl_down1(fw.x), shfl_down1(fw.y), shfl_down1(fw.e)};
         const double ae = CUT ? shfl_down1(ax) : 1.0;
-        if (CUT && owner && g.flag[static_cast<size_t>(j + kGhosts) * g.mx + i + kGhosts] != 0.0)
+        if (CUT && owner && (code_u & 2u))
           xres = Flux{ae * fe.m - ax * fw.m, ae * fe.x - ax * fw.x, ae * fe.y - ax * fw.y, ae * fe.e - ax * fw.e};
         else
           xres = Flux{-(fe.m - fw.m) / dxi, -(fe.x - fw.x) / dxi, -(fe.y - fw.y) / dxi, -(fe.e - fw.e) / dxi};
       }
       wprev = w;
       q0 = c0, q1 = c1, q2 = c2, q3 = c3;
+      code_prev = code_u;
     }
   }
   if (MODE == 0) return;
@@ -1023,30 +1045,46 @@ __global__ __launch_bounds__(kThreads) void k_srd_apply(Grid g) {
     return;
   double* u = stage_out<MODE>(g, c->cur);
   const size_t n = static_cast<size_t>(g.ncase) * g.n_srd;
+  const int lane = threadIdx.x % kLanes;
   bool bad = false;
-  for (size_t t = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; t < n;
-       t += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    const size_t icase = t / g.n_srd;
-    const int e = static_cast<int>(t - icase * g.n_srd);
-    const int k = g.srd_cell[e];
-    double* uc = u + icase * 4 * g.plane;
-    const double* qh = g.Qh + icase * g.nsmall * 4;
-    double x[4];
-    for (int v = 0; v < 4; ++v) {
-      double acc = g.smallpos[k] < 0.0 ? uc[v * g.plane + k] : 0.0;
-      for (int m = g.grp_off[e]; m < g.grp_off[e + 1]; ++m) acc += qh[g.grp_idx[m] * 4 + v];
-      x[v] = acc / g.nbrcount[k];
-      uc[v * g.plane + k] = x[v];
+  // Wave-uniform iterations (64 consecutive items per wave), so that the wave can reduce the
+  // stage-3 bounds of one case to a single atomic.
+  for (size_t base = blockIdx.x * static_cast<size_t>(blockDim.x) + (threadIdx.x - lane); base < n;
+       base += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const size_t t = base + lane;
+    const bool valid = t < n;
+    const size_t icase = valid ? t / g.n_srd : 0;
+    unsigned long long bits = 0;
+    if (valid) {
+      const int e = static_cast<int>(t - icase * g.n_srd);
+      const int k = g.srd_cell[e];
+      double* uc = u + icase * 4 * g.plane;
+      const double* qh = g.Qh + icase * g.nsmall * 4;
+      double x[4];
+      for (int v = 0; v < 4; ++v) {
+        double acc = g.smallpos[k] < 0.0 ? uc[v * g.plane + k] : 0.0;
+        for (int m = g.grp_off[e]; m < g.grp_off[e + 1]; ++m) acc += qh[g.grp_idx[m] * 4 + v];
+        x[v] = acc / g.nbrcount[k];
+        uc[v * g.plane + k] = x[v];
+      }
+      if (inadmissible(x[0], x[1], x[2], x[3], g.gamma)) bad = true;
+      if (MODE == 3) {
+        const int j = k / g.mx - kGhosts, i = k % g.mx - kGhosts;
+        const double b = wave_bound(x[0], x[1], x[2], x[3], g, g.dx[i + kGhosts], g.dy[j + kGhosts]);
+        bits = static_cast<unsigned long long>(__double_as_longlong(b));
+      }
     }
-    if (inadmissible(x[0], x[1], x[2], x[3], g.gamma)) bad = true;
     if (MODE == 3) {
-      const int j = k / g.mx - kGhosts, i = k % g.mx - kGhosts;
-      const double b = wave_bound(x[0], x[1], x[2], x[3], g, g.dx[i + kGhosts], g.dy[j + kGhosts]);
-      const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(b));
-      if (bits) atomicMax(&g.smax[icase], bits);
+      const size_t first = __shfl(icase, 0, kLanes);
+      if (__all(!valid || icase == first)) {  // one case in the wave: one atomic
+        const unsigned long long m = wave_max(bits);
+        if (lane == 0 && m) atomicMax(&g.smax[first], m);
+      } else if (bits) {
+        atomicMax(&g.smax[icase], bits);
+      }
     }
   }
-  if (__any(bad) && threadIdx.x % kLanes == 0) atomicOr(&c->bad, 1);
+  if (__any(bad) && lane == 0) atomicOr(&c->bad, 1);
 }
 
 __global__ void k_zero(double* p, size_t n) {
@@ -1246,7 +1284,7 @@ bool build_graph(ugrid_session* s, int trials, char* err, size_t len) {
 
 // Cut-cell arrays of the device: per-cell data, apertures, and the SRD lists (members of each
 // neighbourhood; the cells SRD rewrites, each with its neighbourhoods in ascending order).
-bool upload_cut(ugrid_session* s, const ugrid_cut_view& c, char* err, size_t len) {
+bool upload_cut(ugrid_session* s, const ugrid_cut_view& c, const double* dx, const double* dy, char* err, size_t len) {
   Grid& g = s->g;
   const size_t cells = g.plane, nfx = static_cast<size_t>(g.nx + 1) * g.ny, nfy = static_cast<size_t>(g.nx) * (g.ny + 1);
   std::vector<int> mem_off(1, 0), mem_idx;
@@ -1277,6 +1315,27 @@ bool upload_cut(ugrid_session* s, const ugrid_cut_view& c, char* err, size_t len
       grp_idx.insert(grp_idx.end(), groups[k].begin(), groups[k].end());
       grp_off.push_back(static_cast<int>(grp_idx.size()));
     }
+  // Cell codes of the stage kernels (see Grid::code).  An aperture is coded 1 only when it equals
+  // the full width bit for bit, 0 only when it is +0, so the kernels read the same values.
+  std::vector<uint8_t> code(cells, 0);
+  const auto face_state = [](double a, double full) -> unsigned {
+    if (a == 0.0 && !std::signbit(a)) return 0u;
+    return a == full ? 1u : 2u;
+  };
+  for (int jj = 0; jj < g.my; ++jj)
+    for (int ii = 0; ii < g.mx; ++ii) {
+      const size_t k = static_cast<size_t>(jj) * g.mx + ii;
+      const int j = jj - kGhosts, i = ii - kGhosts;
+      unsigned v = 0;
+      if (c.vol[k] <= 0.0) v |= 1u;
+      if (c.flag[k] != 0.0) v |= 2u;
+      if (c.smallpos[k] >= 0.0 || c.nbrcount[k] != 1.0) v |= 4u;
+      if (j >= 0 && j < g.ny && i >= 0 && i <= g.nx)
+        v |= face_state(c.axl[static_cast<size_t>(j) * (g.nx + 1) + i], dy[jj]) << 3;
+      if (j >= 0 && j <= g.ny && i >= 0 && i < g.nx)
+        v |= face_state(c.ayl[static_cast<size_t>(j) * g.nx + i], dx[ii]) << 5;
+      code[k] = static_cast<uint8_t>(v);
+    }
   std::vector<int> pre_cell;  // interior cut cells, for the first-stage precheck
   for (int j = 0; j < g.ny; ++j)
     for (int i = 0; i < g.nx; ++i) {
@@ -1289,6 +1348,7 @@ bool upload_cut(ugrid_session* s, const ugrid_cut_view& c, char* err, size_t len
   g.n_pre = static_cast<int>(pre_cell.size());
   double *vol, *flag, *axl, *ayl, *wax, *way, *nbr, *pos, *svhat, *qh;
   int *moff, *midx, *scell, *goff, *gidx, *pcell;
+  uint8_t* dcode;
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream), err, len, "upload cut geometry");
   };
@@ -1301,7 +1361,8 @@ bool upload_cut(ugrid_session* s, const ugrid_cut_view& c, char* err, size_t len
       !device_alloc(s, &moff, mem_off.size(), err, len) || !device_alloc(s, &midx, std::max<size_t>(mem_idx.size(), 1), err, len) ||
       !device_alloc(s, &scell, std::max<size_t>(srd_cell.size(), 1), err, len) ||
       !device_alloc(s, &goff, grp_off.size(), err, len) || !device_alloc(s, &gidx, std::max<size_t>(grp_idx.size(), 1), err, len) ||
-      !device_alloc(s, &pcell, std::max<size_t>(pre_cell.size(), 1), err, len))
+      !device_alloc(s, &pcell, std::max<size_t>(pre_cell.size(), 1), err, len) ||
+      !device_alloc(s, &dcode, cells, err, len))
     return false;
   const size_t d = sizeof(double), n = sizeof(int);
   if (!up(vol, c.vol, cells * d) || !up(flag, c.flag, cells * d) || !up(axl, c.axl, nfx * d) ||
@@ -1311,7 +1372,7 @@ bool upload_cut(ugrid_session* s, const ugrid_cut_view& c, char* err, size_t len
       (!mem_idx.empty() && !up(midx, mem_idx.data(), mem_idx.size() * n)) ||
       (!srd_cell.empty() && !up(scell, srd_cell.data(), srd_cell.size() * n)) ||
       !up(goff, grp_off.data(), grp_off.size() * n) || (!grp_idx.empty() && !up(gidx, grp_idx.data(), grp_idx.size() * n)) ||
-      (!pre_cell.empty() && !up(pcell, pre_cell.data(), pre_cell.size() * n)) ||
+      (!pre_cell.empty() && !up(pcell, pre_cell.data(), pre_cell.size() * n)) || !up(dcode, code.data(), cells) ||
       !ok(hipStreamSynchronize(s->stream), err, len, "upload cut geometry"))
     return false;
   g.vol = vol;
@@ -1330,6 +1391,7 @@ bool upload_cut(ugrid_session* s, const ugrid_cut_view& c, char* err, size_t len
   g.grp_off = goff;
   g.grp_idx = gidx;
   g.pre_cell = pcell;
+  g.code = dcode;
   return true;
 }
 
@@ -1483,7 +1545,7 @@ extern "C" int ugrid_create(const ugrid_setup* setup, const double* u, uint64_t 
   }
   s->uploaded.assign(u, u + states);
   s->mco = mco;
-  if (setup->cut && !upload_cut(s, *setup->cut, err, err_len)) {
+  if (setup->cut && !upload_cut(s, *setup->cut, setup->dx, setup->dy, err, err_len)) {
     release(s);
     return 3;
   }
