@@ -3,7 +3,7 @@
 Pins: the 20 (airfoil, level) fluid cell counts of validation/ten_short_checks.json; the
 geometric audit of src/quick_airfoil.cpp:51-58 (area, wall length, closure); and, where the
 reference CPU code is built (oracle/_ref, build container), every mesh and viscous-table array
-compared field by field with the reference mesher's output.
+compared field by field with the reference mesher's output, at LF1 / LF2 and at levels 7 and 8.
 """
 import numpy as np
 import pytest
@@ -51,6 +51,25 @@ def test_bit_identical_to_reference_euler_srd(thickness):
     _, ref, _ = case_mesh(naca_polygon(thickness), "LF1", over, lib=O.ref_lib(), prefix="ref_mesh")
     a, b = mine.arrays(), ref.arrays()
     assert mine.info.small_cells == ref.info.small_cells > 0
+    bad = [k for k in b if not same_records(a[k], b[k])]
+    assert not bad, bad
+
+
+@needs_ref
+@pytest.mark.parametrize("preset, over", [
+    ("L7", None),  # the serving loop's level-7 meshes (SA-neg)
+    ("L8", {"freestream": {"Mach": 0.75, "AOA_degrees": 2.0},  # the level-8 laminar AUSM+-up case
+            "physics": {"turbulence": "none", "viscosity": "constant"}, "solver": {"flux": "ausm"},
+            "boundaries": {"xmax": "pressure-outlet"}}),
+])
+def test_bit_identical_to_reference_deep_levels(preset, over):
+    """Level-7 and level-8 meshes (about 29k and 104k cells) of a cambered 14% section: every
+    mesh and viscous-table array equal to the reference mesher's."""
+    _, mine, _ = case_mesh(naca_polygon(0.14, 0.03, 0.4), preset, over)
+    _, ref, _ = case_mesh(naca_polygon(0.14, 0.03, 0.4), preset, over, lib=O.ref_lib(), prefix="ref_mesh")
+    assert mine.n_cells == ref.n_cells > (100_000 if preset == "L8" else 25_000)
+    a, b = mine.arrays(), ref.arrays()
+    assert set(a) == set(b)
     bad = [k for k in b if not same_records(a[k], b[k])]
     assert not bad, bad
 
