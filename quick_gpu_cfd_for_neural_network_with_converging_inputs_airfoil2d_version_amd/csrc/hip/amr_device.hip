@@ -122,8 +122,8 @@ std::vector<int> block_tiles(const std::vector<SlotCapacity>& caps, int tile, in
 
 // An arena of several slots runs its trial as independent branches (slot m in branch
 // m % branches) on their own streams, so that one branch's kernels fill the GPU while another's
-// drain.  Three measured best for 256 LF2 meshes (833.8 us per step, against 847.0 with two,
-// 857.0 with four and 951 with one).
+// drain.  Three measured best for 256 LF2 meshes (703.1 us per step, against 715.7 with two and
+// 710.8 with four; 951 with one stream in an earlier build).
 int branch_count(size_t slots) { return static_cast<int>(std::min<size_t>(slots, 3)); }
 
 // The block tables of an arena: the whole arena first, then one per branch (when branched).
