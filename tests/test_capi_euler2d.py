@@ -156,3 +156,25 @@ def test_cut_setup_refusals_before_any_device_work():
     s.cut.nbad_nbhd = 1
     with pytest.raises(RuntimeError, match="refuses 1 invalid SRD neighbourhoods"):
         E.DeviceSession(s)
+
+
+def test_cut_builder_memory_safety_under_sanitizers(tmp_path):
+    """The native cut-cell builder (host code) compiled with AddressSanitizer and
+    UndefinedBehaviorSanitizer: every view array read end to end on three grids and three bodies,
+    and the refusals (too few vertices, a body at the sides, an edge on a grid line) clean."""
+    import shutil
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    root = Path(__file__).resolve().parents[1]
+    host = root / "quick_gpu_cfd_for_neural_network_with_converging_inputs_airfoil2d_version_amd" / "csrc" / "host"
+    exe = tmp_path / "cut_builder_asan"
+    build = subprocess.run([gxx, "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-omit-frame-pointer",
+                            "-I", str(root / "include"), "-I", str(host), str(CALLER.parent / "cut_builder_asan.cpp"),
+                            str(host / "ugrid_cut.cpp"), str(host / "cutcell.cpp"), str(host / "ugrid_axis.cpp"),
+                            "-o", str(exe)], capture_output=True, text=True, timeout=300)
+    assert build.returncode == 0, build.stderr
+    run = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300,
+                         env={"ASAN_OPTIONS": "detect_leaks=1", "UBSAN_OPTIONS": "halt_on_error=1"})
+    assert run.returncode == 0, run.stdout + run.stderr
+    assert "all ok" in run.stdout and run.stdout.count("ok nx=") == 9
