@@ -365,7 +365,7 @@ def main():
                     help="requests of the bounded serving leg that gives p50/p99 (0: none)")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads of the threaded CPU baseline (0: OMP_NUM_THREADS, else the host's cores up to 16)")
-    ap.add_argument("--budget", type=float, default=240.0, help="wall budget of each steady solve (s)")
+    ap.add_argument("--budget", type=float, default=400.0, help="wall budget of each steady solve (s)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--solve", action=argparse.BooleanOptionalAction, default=True,
