@@ -237,6 +237,32 @@ def legacy_workload(args) -> dict:
         log(f"legacy grid with a body: {cut['ms_per_step']:.3f} ms/step ({rejected} rejected trials in "
             f"{args.steps} steps, {cut['ms_per_trial']:.3f} ms per trial), {cut['cell_steps_per_second'] / 1e9:.2f} "
             f"G cell-steps/s; {sc.cut.ncut} cut, {sc.cut.nsmall} small cells")
+    # Backward Euler (pseudo-time defect relaxation) on the same batch at 512^2: physical steps,
+    # pseudo updates (each one residual pass and one trial over every case) and their rate.
+    be = None
+    if args.legacy_be:
+        nb = max(64, n // 2)
+        sb = blast_batch(nb, ncase)
+        sb.par.time_integrator = E.TIME_BACKWARD_EULER
+        sb.par.cfl = 2.0
+        with sb.device_session() as d:
+            status, msg = d.run(1e9, 1)  # warm-up: one physical step
+            assert status in (1, 4), msg
+            st0 = d.implicit_stats()
+            t0 = time.perf_counter()
+            status, msg = d.run(1e9, args.legacy_be_steps)
+            wall_b = time.perf_counter() - t0
+            assert status in (1, 4), msg  # 4: a step not converged within the iteration limit
+            st1 = d.implicit_stats()
+        updates = st1["updates"] - st0["updates"]
+        be = {"cells": nb * nb * ncase, "physical_steps": args.legacy_be_steps, "status": status,
+              "pseudo_updates": int(updates),
+              "retries": int(st1["retries"] - st0["retries"]), "seconds": round(wall_b, 3),
+              "us_per_update": round(wall_b / max(updates, 1) * 1e6, 2),
+              "cell_updates_per_second": round(nb * nb * ncase * updates / wall_b)}
+        log(f"legacy grid backward Euler: {ncase} cases of {nb}x{nb}, {args.legacy_be_steps} physical steps, "
+            f"{updates} pseudo updates in {wall_b:.2f} s ({be['us_per_update']:.1f} us each, "
+            f"{be['cell_updates_per_second'] / 1e9:.2f} G cell-updates/s)")
     # CPU baseline: the oracle on one case of 256 x 256.
     cpu_n, cpu_steps = 256, 0
     c = blast_batch(cpu_n, 1)
@@ -263,7 +289,7 @@ def legacy_workload(args) -> dict:
         "cpu_baseline": {"value": round(cpu_rate), "unit": "cell-steps/s", "cores": 1, "kind": "port",
                          "sample": f"{cpu_steps} steps of one {cpu_n}x{cpu_n} blast case on the FP64 oracle"},
         "launch": info,
-        "detail": {"with_body": cut},
+        "detail": {"with_body": cut, "backward_euler": be},
     }
 
 
@@ -386,6 +412,9 @@ def main():
     ap.add_argument("--legacy-cases", type=int, default=16, help="legacy grid: cases in the batch")
     ap.add_argument("--legacy-cut", action=argparse.BooleanOptionalAction, default=True,
                     help="legacy grid: also time the batch with a solid body (cut cells, SRD)")
+    ap.add_argument("--legacy-be", action=argparse.BooleanOptionalAction, default=True,
+                    help="legacy grid: also time backward Euler on the batch at half the resolution")
+    ap.add_argument("--legacy-be-steps", type=int, default=3, help="legacy grid: backward-Euler physical steps timed")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU (this build has no multi-GPU path); got --gpus {args.gpus}",
