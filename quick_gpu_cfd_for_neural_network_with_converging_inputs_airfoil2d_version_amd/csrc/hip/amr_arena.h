@@ -97,8 +97,7 @@ struct ArenaPtrs {
   int commit_in_flux;
   const MeshDesc<T>* desc;
   Control* ctrl;
-  const int* cell_block_mesh;  // n_cell_blocks
-  const int* block_tile;       // n_grid_blocks: tile of each block of a cell launch, -1 none
+  const int2* block_rec;       // n_grid_blocks: {tile, slot} of each block of a cell launch, {-1, -1} none
   const int* face_block_mesh;  // n_face_blocks + n_wall_blocks (faces first, then walls)
   // cells
   const T* vol;

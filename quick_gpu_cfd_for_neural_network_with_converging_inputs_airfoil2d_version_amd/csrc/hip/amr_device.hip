@@ -172,7 +172,7 @@ template <class T>
 SlabPlan plan_slab(const Totals& t, int nv) {
   const auto arr = [](size_t elem, size_t count) { return align256(elem * std::max<size_t>(count, 1)); };
   SlabPlan p{};
-  p.geometry = arr(sizeof(MeshDesc<T>), t.meshes) + arr(4, t.cell_blocks) + arr(4, t.grid_blocks) +
+  p.geometry = arr(sizeof(MeshDesc<T>), t.meshes) + arr(8, t.grid_blocks) +
                arr(4, t.face_blocks + t.wall_blocks) +
                5 * arr(sizeof(T), t.cells) + 2 * arr(4, t.cells) + 2 * arr(8, t.cells) + 4 * arr(4, t.cells + 1) +
                arr(2, t.csr) + 2 * arr(4 * sizeof(T), t.csr) + arr(4, t.terms) + 2 * arr(sizeof(T), t.terms) +
@@ -267,6 +267,7 @@ template <class T>
 struct HostArrays {
   std::vector<MeshDesc<T>> desc;
   std::vector<int> cell_block_mesh, block_tile, face_block_mesh;
+  std::vector<int2> block_rec;  // the device's block tables: {tile, slot} per block_tile entry
   std::vector<T> vol, inv_vol, dt_area, wall_dist, mem_inv;
   std::vector<int> wall_first, wall_cnt, csr_off, term_off, grp_off, mem_off;
   std::vector<double> cell_cx, cell_cy;
@@ -359,6 +360,8 @@ class DeviceArena final : public Arena {
     n_wall_blocks_ = static_cast<int>(h.face_block_mesh.size()) - n_face_blocks_;
     n_cell_blocks_ = static_cast<int>(h.cell_block_mesh.size());
     h.block_tile = all_block_tiles(caps, tile, &table_offsets_);
+    h.block_rec.clear();
+    for (const int tl : h.block_tile) h.block_rec.push_back(tl < 0 ? make_int2(-1, -1) : make_int2(tl, h.cell_block_mesh[tl]));
     if (h.face_block_mesh.empty()) h.face_block_mesh.push_back(0);
     const auto sized = [](auto& v, size_t n, auto value) { v.assign(std::max<size_t>(n, 1), value); };
     sized(h.vol, nc, T(1));
@@ -658,6 +661,13 @@ class DeviceArena final : public Arena {
   std::string check_tables() const {
     const HostArrays<T>& h = h_;
     if (static_cast<int>(h.cell_block_mesh.size()) * tile_ != nc_) return "layout: cell block table";
+    if (h.block_rec.size() != h.block_tile.size()) return "layout: block record table";
+    for (size_t i = 0; i < h.block_tile.size(); ++i) {
+      const int tl = h.block_tile[i];
+      const bool none = tl < 0 || tl >= n_cell_blocks_;
+      if (h.block_rec[i].x != (none ? -1 : tl) || h.block_rec[i].y != (none ? -1 : h.cell_block_mesh[tl]))
+        return "layout: block record table";
+    }
     // The whole-arena table lists every tile once; the branch tables together list every tile
     // once, each only tiles of its own slots.
     const int tables = static_cast<int>(table_offsets_.size()) - 1;
@@ -719,8 +729,7 @@ class DeviceArena final : public Arena {
       columns_.push_back({vec.data(), nullptr, sizeof(V), vec.size(), sp});
     };
     col(h.desc, TABLE);
-    col(h.cell_block_mesh, TABLE);
-    col(h.block_tile, TABLE);
+    col(h.block_rec, TABLE);
     col(h.face_block_mesh, TABLE);
     for (auto* v : {&h.vol, &h.inv_vol, &h.dt_area, &h.wall_dist, &h.mem_inv}) col(*v, CELLS);
     for (auto* v : {&h.wall_first, &h.wall_cnt}) col(*v, CELLS);
@@ -776,8 +785,7 @@ class DeviceArena final : public Arena {
     size_t i = 0;
     const auto next = [&](auto* typed) { return reinterpret_cast<decltype(typed)>(columns_[i++].dev); };
     a.desc = next(static_cast<const MeshDesc<T>*>(nullptr));
-    a.cell_block_mesh = next(static_cast<const int*>(nullptr));
-    a.block_tile = next(static_cast<const int*>(nullptr));
+    a.block_rec = next(static_cast<const int2*>(nullptr));
     a.face_block_mesh = next(static_cast<const int*>(nullptr));
     a.vol = next(static_cast<const T*>(nullptr));
     a.inv_vol = next(static_cast<const T*>(nullptr));
@@ -960,7 +968,7 @@ class DeviceArena final : public Arena {
   int launch_trial_branch(int j, hipStream_t s) override {
     if (n_branches() == 1) return hipamr::launch_trial<T>(ptrs_, s, nullptr);
     ArenaPtrs<T> a = ptrs_;  // the branch's block table; its slots commit in its own launch
-    a.block_tile = ptrs_.block_tile + table_offsets_[1 + j];
+    a.block_rec = ptrs_.block_rec + table_offsets_[1 + j];
     a.n_grid_blocks = table_offsets_[2 + j] - table_offsets_[1 + j];
     a.commit_first = j;
     a.commit_stride = n_branches();
@@ -1210,7 +1218,7 @@ class DeviceArena final : public Arena {
   uint64_t slab_bytes_ = 0, geometry_bytes_ = 0, state_bytes_ = 0;
   int nc_ = 0, nf_ = 0, nw_ = 0, tile_ = 256;
   int n_cell_blocks_ = 0, n_face_blocks_ = 0, n_wall_blocks_ = 0;
-  std::vector<int> table_offsets_;  // block tables in block_tile: whole arena, then each branch
+  std::vector<int> table_offsets_;  // block tables in block_tile / block_rec: whole arena, then each branch
 };
 
 template <class T>

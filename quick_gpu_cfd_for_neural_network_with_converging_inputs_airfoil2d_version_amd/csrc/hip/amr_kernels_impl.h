@@ -73,13 +73,14 @@ __device__ __forceinline__ bool stage_deltas(const MeshDesc<T>& d) {
   return sizeof(T) == 4 && !d.small;
 }
 
-// Cell tile of this block, or -1.  The dispatcher hands consecutive blocks to the 8 XCDs in
-// turn, each with its own L2.  The host-built table splits every slot's tiles into 8 contiguous
-// runs, one per XCD: neighbouring tiles share an L2, and whichever slots are running, their
-// work is spread over all XCDs.
+// Cell tile of this block and its slot, or {-1, -1}: one 8-byte load, so that the slot's control
+// record is one dependent load away from the kernel's start.  The dispatcher hands consecutive
+// blocks to the 8 XCDs in turn, each with its own L2.  The host-built table splits every slot's
+// tiles into 8 contiguous runs, one per XCD: neighbouring tiles share an L2, and whichever slots
+// are running, their work is spread over all XCDs.
 template <class T>
-__device__ __forceinline__ int cell_tile(const ArenaPtrs<T>& a) {
-  return a.block_tile[blockIdx.x];
+__device__ __forceinline__ int2 cell_block(const ArenaPtrs<T>& a) {
+  return a.block_rec[blockIdx.x];
 }
 
 // ---- cell records ---------------------------------------------------------------------------
@@ -337,9 +338,10 @@ __device__ __forceinline__ void publish_dt(Control* c, double local) {
 // ------------------------------------------------------------------------------------------
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_base_prim(ArenaPtrs<T> a) {
-  const int tile = cell_tile(a);
+  const int2 blk = cell_block(a);
+  const int tile = blk.x;
   if (tile < 0) return;
-  const int m = a.cell_block_mesh[tile];
+  const int m = blk.y;
   Control* c = a.ctrl + m;
   if (!pass_on(c, PASS_BASE) || c->w_valid) return;
   const MeshDesc<T>& d = a.desc[m];
@@ -412,9 +414,10 @@ __device__ __forceinline__ void first_stage(const ArenaPtrs<T>& a, const MeshDes
 // GREC.  STAGE 0 = base pass, 1 and 2 = RK stages.
 template <class T, int STAGE>
 __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
-  const int tile = cell_tile(a);
+  const int2 blk = cell_block(a);
+  const int tile = blk.x;
   if (tile < 0) return;
-  const int m = a.cell_block_mesh[tile];
+  const int m = blk.y;
   Control* c = a.ctrl + m;
   if (!pass_live(c, STAGE == 0 ? PASS_BASE : PASS_STAGE)) return;
   const MeshDesc<T>& d = a.desc[m];
@@ -769,9 +772,10 @@ template <class T, int STAGE, int SPREAD>
 __global__ __launch_bounds__(kMaxTile) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (STAGE == 0 ? 6 : 7) : 1)))
 void k_flux(ArenaPtrs<T> a) {
   constexpr bool BASE = STAGE == 0;
-  const int tile = cell_tile(a);
+  const int2 blk = cell_block(a);
+  const int tile = blk.x;
   if (tile < 0) return;
-  const int m = a.cell_block_mesh[tile];
+  const int m = blk.y;
   Control* c = a.ctrl + m;
   // The last stage pass commits the trial: every block of a slot with an active trial reports
   // (also when the trial has already failed), the last one commits.
@@ -1008,9 +1012,10 @@ void k_flux(ArenaPtrs<T> a) {
 // (reference diffusion_group_kernel and choose_dt SRD branch, src/static_amr2d_cuda.cu:164-176).
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_diffusion_group(ArenaPtrs<T> a) {
-  const int tile = cell_tile(a);
+  const int2 blk = cell_block(a);
+  const int tile = blk.x;
   if (tile < 0) return;
-  const int m = a.cell_block_mesh[tile];
+  const int m = blk.y;
   Control* c = a.ctrl + m;
   if (!pass_live(c, PASS_BASE)) return;  // uniform over the block: it meets a barrier in publish_dt
   const MeshDesc<T>& d = a.desc[m];
@@ -1024,9 +1029,10 @@ __global__ __launch_bounds__(kMaxTile) void k_diffusion_group(ArenaPtrs<T> a) {
 
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_dt_group(ArenaPtrs<T> a) {
-  const int tile = cell_tile(a);
+  const int2 blk = cell_block(a);
+  const int tile = blk.x;
   if (tile < 0) return;
-  const int m = a.cell_block_mesh[tile];
+  const int m = blk.y;
   Control* c = a.ctrl + m;
   if (!pass_live(c, PASS_BASE)) return;  // uniform over the block: it meets a barrier in publish_dt
   const MeshDesc<T>& d = a.desc[m];
@@ -1052,9 +1058,10 @@ __global__ __launch_bounds__(kMaxTile) void k_dt_group(ArenaPtrs<T> a) {
 // 256 LF2 meshes, and no gain in the latency of a single-mesh trial, 70.5 against 70.4 us.)
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
-  const int tile = cell_tile(a);
+  const int2 blk = cell_block(a);
+  const int tile = blk.x;
   if (tile < 0) return;
-  const int m = a.cell_block_mesh[tile];
+  const int m = blk.y;
   Control* c = a.ctrl + m;
   if (!running(c)) return;
   const MeshDesc<T>& d = a.desc[m];
@@ -1075,9 +1082,10 @@ __global__ __launch_bounds__(kMaxTile) void k_stage0(ArenaPtrs<T> a) {
 // `stage` selects the candidate: 0 and 2 write U[cur ^ 1], 1 writes U2.
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_group(ArenaPtrs<T> a, int stage) {
-  const int tile = cell_tile(a);
+  const int2 blk = cell_block(a);
+  const int tile = blk.x;
   if (tile < 0) return;
-  const int m = a.cell_block_mesh[tile];
+  const int m = blk.y;
   Control* c = a.ctrl + m;
   if (!healthy(c)) return;
   const MeshDesc<T>& d = a.desc[m];
@@ -1103,9 +1111,10 @@ __global__ __launch_bounds__(kMaxTile) void k_group(ArenaPtrs<T> a, int stage) {
 
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_redist(ArenaPtrs<T> a, int stage) {
-  const int tile = cell_tile(a);
+  const int2 blk = cell_block(a);
+  const int tile = blk.x;
   if (tile < 0) return;
-  const int m = a.cell_block_mesh[tile];
+  const int m = blk.y;
   Control* c = a.ctrl + m;
   if (!healthy(c)) return;
   const MeshDesc<T>& d = a.desc[m];
@@ -1174,9 +1183,10 @@ __device__ __forceinline__ void block_sum(double (&v)[N]) {
 // reference src/quick_airfoil.cpp:76-81).
 template <class T>
 __global__ __launch_bounds__(kMaxTile) void k_monitor_cells(ArenaPtrs<T> a) {
-  const int tile = cell_tile(a);
+  const int2 blk = cell_block(a);
+  const int tile = blk.x;
   if (tile < 0) return;
-  const int m = a.cell_block_mesh[tile];
+  const int m = blk.y;
   const MeshDesc<T>& d = a.desc[m];
   const int k = tile * static_cast<int>(blockDim.x) + static_cast<int>(threadIdx.x);
   double acc[6] = {0, 0, 0, 0, 0, 0};
