@@ -49,19 +49,21 @@ __host__ __device__ constexpr int prec_width() { return sizeof(T) == 4 ? 3 : 4; 
 constexpr int kItemRightBits = 27;
 constexpr int kItemInfoMask = 15;
 
-// Face items held in LDS at once per block (a tile of an LF1 / LF2 mesh touches 2.2 - 2.7 faces
-// per cell; a tile with more items is processed in several chunks): 5/2 per cell of the tile in
-// a stage pass, 3/2 in the base pass, whose items take 9 T instead of 5.  Measured on 256 LF2
-// meshes at tile 128: the base pass gains from the smaller chunk (202 against 210 us, more
-// blocks per CU), a stage pass loses from it (236 against 216 us, a second chunk round).
-__host__ __device__ constexpr int item_chunk(int tile, bool base) { return base ? tile * 3 / 2 : tile * 5 / 2; }
+// Face items held in LDS at once per block: 5/2 per cell of the tile (a tile of an LF1 / LF2
+// mesh touches 2.2 - 2.7 faces per cell; a tile with more items is processed in several chunks).
+// Measured on 256 LF2 meshes at tile 128 with 8-word base items: base chunks of 3/2, 2, 9/4 - 1/32,
+// 5/2, 11/4 and 3 per cell give 164, 164, 178, 159, 171 and 171 us for the base pass and 700,
+// 699, 705, 683, 690 and 691 us per batch step (one chunk round for most tiles against losing a
+// block per CU); a stage chunk of 11/4 gains nothing, 3 loses 13 us per stage pass.
+__host__ __device__ constexpr int item_chunk(int tile, bool) { return tile * 5 / 2; }
 
 // Dynamic LDS of a flux launch: 8 T per cell (primitive vectors), then per item of a chunk 4 + 1
-// T (convective and viscous flux, SA flux) and, in the base pass, 4 T more (dt terms).
+// T (convective and viscous flux, SA flux) in a stage pass, 4 + 4 T (fluxes; dt terms and SA
+// flux) in the base pass.
 // Gradient launches: 4 T per cell.
 template <class T>
 constexpr size_t flux_lds_bytes(int tile, bool base) {
-  return (static_cast<size_t>(tile) * 8 + static_cast<size_t>(item_chunk(tile, base)) * (base ? 9 : 5)) * sizeof(T);
+  return (static_cast<size_t>(tile) * 8 + static_cast<size_t>(item_chunk(tile, base)) * (base ? 8 : 5)) * sizeof(T);
 }
 template <class T>
 constexpr size_t grad_lds_bytes(int tile) {

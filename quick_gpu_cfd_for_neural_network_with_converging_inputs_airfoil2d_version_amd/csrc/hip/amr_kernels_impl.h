@@ -191,15 +191,16 @@ struct TileCells {
 };
 
 // Results of one chunk of face items: f = mass, x- and y-momentum, energy flux (times the
-// length, convective minus viscous), sf = SA flux, x = spectral radius x length, viscous and SA
-// dt coefficients (base pass only).
+// length, convective minus viscous); stage passes: sf = SA flux; base pass: x = spectral radius
+// x length, viscous and SA dt coefficients and the SA flux (4 per item, so that a base item takes
+// 8 words).
 template <class T>
 struct FaceSlots {
   T* f;
-  T* sf;
-  T* x;
+  T* sf;  // stage passes
+  T* x;   // base pass
   __device__ FaceSlots(int tile, bool base)
-      : f(reinterpret_cast<T*>(tile_lds) + 8 * tile), sf(f + 4 * item_chunk(tile, base)), x(sf + item_chunk(tile, base)) {}
+      : f(reinterpret_cast<T*>(tile_lds) + 8 * tile), sf(f + 4 * item_chunk(tile, base)), x(sf) {}
 };
 
 template <class T>
@@ -761,8 +762,9 @@ __device__ __forceinline__ void arrive_and_commit(Control* c, int n_blocks, bool
 // ------------------------------------------------------------------------------------------
 // Residual of every cell from its faces and wall fragments.  STAGE 0: base residual R0 and the
 // local dt.  1, 2: second and third SSP-RK3 combinations, then the candidate's primitives.
-// FP32 register budget: 6 waves per SIMD in the base pass (80 VGPRs, no spill) and 7 in the
-// stage passes (72 VGPRs; the last stage spills 2), which the LDS of a 128-cell tile also admits.
+// FP32 register budget: 6 waves per SIMD in the base pass (80 VGPRs, 2 spilled) and 7 in the
+// stage passes (72 VGPRs, 2 and 4 spilled); the LDS of a 128-cell tile admits 11 and 14 blocks
+// of two waves per CU.
 // On 256 LF2 meshes: base flux 187 -> 177 us, stage flux 188 -> 181 us against the unbounded
 // kernels; 8 waves (64 VGPRs) spills 11-14 and is slower.  FP64 (190-200 VGPRs) has no bound.
 // SPREAD threads per cell (1 or 2): a tile's face items are spread over SPREAD * tile threads,
@@ -855,8 +857,8 @@ void k_flux(ArenaPtrs<T> a) {
       ok = ok && f.ok;
       const int sl = i - c0;
       store4(slots.f + 4 * sl, f.f0, f.fx, f.fy, f.f3);
-      if (sa) slots.sf[sl] = f.sflux;
-      if (BASE) store4(slots.x + 4 * sl, f.speed * it.geo.len, f.fcoef * it.geo.len, f.sfcoef, T(0));
+      if (BASE) store4(slots.x + 4 * sl, f.speed * it.geo.len, f.fcoef * it.geo.len, f.sfcoef, f.sflux);
+      else if (sa) slots.sf[sl] = f.sflux;
     }
     __syncthreads();
     if (valid) {
@@ -873,13 +875,15 @@ void k_flux(ArenaPtrs<T> a) {
         s1 += sgn * fx;
         s2 += sgn * fy;
         s3 += sgn * f3;
-        if (sa) sa_net += sgn * slots.sf[sl];
         if (BASE) {
-          T xs, xd, xa, pad;
-          load4(slots.x + 4 * sl, xs, xd, xa, pad);
+          T xs, xd, xa, xf;
+          load4(slots.x + 4 * sl, xs, xd, xa, xf);
+          if (sa) sa_net += sgn * xf;
           spectral += xs;
           if (laminar) diffusion += xd;
           if (sa) sa_spec += xa;
+        } else if (sa) {
+          sa_net += sgn * slots.sf[sl];
         }
       }
     }
