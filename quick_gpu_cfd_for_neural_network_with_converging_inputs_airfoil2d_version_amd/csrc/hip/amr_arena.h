@@ -57,6 +57,21 @@ constexpr int kItemInfoMask = 15;
 // block per CU); a stage chunk of 11/4 gains nothing, 3 loses 13 us per stage pass.
 __host__ __device__ constexpr int item_chunk(int tile, bool) { return tile * 5 / 2; }
 
+// One LSQ gradient term of a cell: its source (>= 0 cell, -1 body wall, -2 - side: no-slip domain
+// side) and its two weights, fetched by one load (12 bytes in FP32) instead of three.
+template <class T>
+struct TermRec;
+template <>
+struct TermRec<float> {
+  int src;
+  float wx, wy;
+};
+template <>
+struct TermRec<double> {
+  double wx, wy;
+  int src, pad;
+};
+
 // Dynamic LDS of a flux launch: 8 T per cell (primitive vectors), then per item of a chunk 4 + 1
 // T (convective and viscous flux, SA flux) in a stage pass, 4 + 4 T (fluxes; dt terms and SA
 // flux) in the base pass.
@@ -112,9 +127,7 @@ struct ArenaPtrs {
   const int* wall_first; // first wall of the cell
   const int* wall_cnt;
   const int* term_off;   // nc + 1
-  const int* term_src;   // >= 0 cell, -1 body wall, -2 - side: no-slip domain side
-  const T* term_wx;
-  const T* term_wy;
+  const TermRec<T>* terms;  // LSQ gradient terms in the cells' term order
   // SRD neighbourhoods: group g = cell g
   const int* grp_off;    // nc + 1
   const int* grp_mem;

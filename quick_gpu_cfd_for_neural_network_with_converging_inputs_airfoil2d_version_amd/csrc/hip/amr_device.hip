@@ -175,7 +175,7 @@ SlabPlan plan_slab(const Totals& t, int nv) {
   p.geometry = arr(sizeof(MeshDesc<T>), t.meshes) + arr(8, t.grid_blocks) +
                arr(4, t.face_blocks + t.wall_blocks) +
                5 * arr(sizeof(T), t.cells) + 2 * arr(4, t.cells) + 2 * arr(8, t.cells) + 4 * arr(4, t.cells + 1) +
-               arr(2, t.csr) + 2 * arr(4 * sizeof(T), t.csr) + arr(4, t.terms) + 2 * arr(sizeof(T), t.terms) +
+               arr(2, t.csr) + 2 * arr(4 * sizeof(T), t.csr) + arr(sizeof(TermRec<T>), t.terms) +
                arr(4, t.groups) + arr(sizeof(T), t.groups) + arr(4, t.members) + arr(8, t.cell_blocks) +
                2 * arr(4, t.walls) + 4 * arr(sizeof(T), t.walls) + 2 * arr(8, t.walls);
   p.control = arr(sizeof(Control), t.meshes) + arr(sizeof(double), t.meshes) + arr(sizeof(long long), t.meshes) +
@@ -276,6 +276,7 @@ struct HostArrays {
   std::vector<T> items_a, items_b;  // 4 + 4 per item (in the cell-face index space)
   std::vector<int> item_range;     // 2 per cell block
   std::vector<T> term_wx, term_wy, grp_w;
+  std::vector<TermRec<T>> term_rec;  // the device's term records (term_src, term_wx, term_wy)
   std::vector<int> face_idx;  // host only, for the layout check: 4 per face: left, right, info, 0
   std::vector<int> wall_owner, wall_donor;
   std::vector<T> wall_nx, wall_ny, wall_len, wall_ddist;
@@ -384,6 +385,9 @@ class DeviceArena final : public Arena {
     sized(h.term_src, nt, -1);
     sized(h.term_wx, nt, T(0));
     sized(h.term_wy, nt, T(0));
+    TermRec<T> no_term{};
+    no_term.src = -1;
+    sized(h.term_rec, nt, no_term);
     sized(h.grp_mem, ng, 0);
     sized(h.grp_w, ng, T(0));
     sized(h.mem_grp, nm, 0);
@@ -469,6 +473,9 @@ class DeviceArena final : public Arena {
           h.term_src[term] = t.source >= 0 ? cb + pos_of[t.source] : (t.source == -1 ? -1 : -2 - t.side);
           h.term_wx[term] = static_cast<T>(t.x);
           h.term_wy[term] = static_cast<T>(t.y);
+          h.term_rec[term].src = h.term_src[term];
+          h.term_rec[term].wx = h.term_wx[term];
+          h.term_rec[term].wy = h.term_wy[term];
           ++term;
         }
       const uint64_t n_k = v.member_offsets[k + 1] - v.member_offsets[k];
@@ -598,6 +605,8 @@ class DeviceArena final : public Arena {
       for (int j = h.term_off[k]; j < h.term_off[k + 1]; ++j) {
         const int src = h.term_src[j];
         if (!(cell_ok(src) || (src <= -1 && src >= -5))) return "layout: gradient term source";
+        const TermRec<T>& r = h.term_rec[j];
+        if (r.src != src || !(r.wx == h.term_wx[j]) || !(r.wy == h.term_wy[j])) return "layout: gradient term record";
       }
       if (h.grp_off[k] < s.grp_begin || h.grp_off[k] > h.grp_off[k + 1] || h.grp_off[k + 1] > s.grp_begin + s.cap.groups ||
           h.mem_off[k] < s.mem_begin || h.mem_off[k] > h.mem_off[k + 1] || h.mem_off[k + 1] > s.mem_begin + s.cap.members)
@@ -738,9 +747,7 @@ class DeviceArena final : public Arena {
     col(h.slot_ent, CSR);
     columns_.push_back({h.items_a.data(), nullptr, 4 * sizeof(T), h.items_a.size() / 4, CSR});
     columns_.push_back({h.items_b.data(), nullptr, 4 * sizeof(T), h.items_b.size() / 4, CSR});
-    col(h.term_src, TERMS);
-    col(h.term_wx, TERMS);
-    col(h.term_wy, TERMS);
+    col(h.term_rec, TERMS);
     col(h.grp_mem, GROUPS);
     col(h.grp_w, GROUPS);
     col(h.mem_grp, MEMBERS);
@@ -803,9 +810,7 @@ class DeviceArena final : public Arena {
     a.SLOT = next(static_cast<const uint16_t*>(nullptr));
     a.ITEM_A = next(static_cast<const T*>(nullptr));
     a.ITEM_B = next(static_cast<const T*>(nullptr));
-    a.term_src = next(static_cast<const int*>(nullptr));
-    a.term_wx = next(static_cast<const T*>(nullptr));
-    a.term_wy = next(static_cast<const T*>(nullptr));
+    a.terms = next(static_cast<const TermRec<T>*>(nullptr));
     a.grp_mem = next(static_cast<const int*>(nullptr));
     a.grp_w = next(static_cast<const T*>(nullptr));
     a.mem_grp = next(static_cast<const int*>(nullptr));

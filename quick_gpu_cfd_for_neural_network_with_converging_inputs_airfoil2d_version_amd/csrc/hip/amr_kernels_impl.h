@@ -452,9 +452,10 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
 #pragma unroll
     for (int q = 0; q < kTerms; ++q) {
       const int j = min(j0 + q, end - 1);  // terms past the end are loaded in range and not summed
-      src[q] = a.term_src[j];
-      wx[q] = a.term_wx[j];
-      wy[q] = a.term_wy[j];
+      const TermRec<T> t = a.terms[j];
+      src[q] = t.src;
+      wx[q] = t.wx;
+      wy[q] = t.wy;
     }
     // Neighbour vectors without branches: one generic-address load per term, from the tile's LDS
     // copy when the neighbour is in the tile, else from the record array (a side term reads the
