@@ -764,7 +764,7 @@ __device__ __forceinline__ void arrive_and_commit(Control* c, int n_blocks, bool
 // Residual of every cell from its faces and wall fragments.  STAGE 0: base residual R0 and the
 // local dt.  1, 2: second and third SSP-RK3 combinations, then the candidate's primitives.
 // FP32 register budget: 6 waves per SIMD in the base pass (80 VGPRs, 2 spilled) and 7 in the
-// stage passes (72 VGPRs, 2 and 4 spilled); the LDS of a 128-cell tile admits 11 and 14 blocks
+// stage passes (72 VGPRs, 6 spilled); the LDS of a 128-cell tile admits 11 and 14 blocks
 // of two waves per CU.
 // On 256 LF2 meshes: base flux 187 -> 177 us, stage flux 188 -> 181 us against the unbounded
 // kernels; 8 waves (64 VGPRs) spills 11-14 and is slower.  FP64 (190-200 VGPRs) has no bound.
@@ -863,12 +863,23 @@ void k_flux(ArenaPtrs<T> a) {
     }
     __syncthreads();
     if (valid) {
+      // Stage passes take the cell's slot entries four at a time: the entries of a batch are loaded
+      // together (entries past the end are loaded in range and skipped), then summed in cell-face
+      // order, so the cell waits for one memory round trip per batch instead of one per face
+      // (stage flux 157 / 167 -> 155 / 160 us on 256 LF2 meshes; the base pass, 2 us slower with
+      // batches of four or two, keeps one entry at a time).
       const int lo = c0 - range.x, n_slots = c1 - c0;
-      const int end = a.csr_off[k + 1];
-      for (int j = a.csr_off[k]; j < end; ++j) {
-        const int e = a.SLOT[j];
+      const int begin = a.csr_off[k], end = a.csr_off[k + 1];
+      constexpr int kEntries = BASE ? 1 : 4;
+      for (int j0 = begin; j0 < end; j0 += kEntries) {
+        int ent[kEntries];
+#pragma unroll
+        for (int q = 0; q < kEntries; ++q) ent[q] = a.SLOT[min(j0 + q, end - 1)];
+#pragma unroll
+        for (int q = 0; q < kEntries; ++q) {
+        const int e = ent[q];
         const int sl = (e >> 1) - lo;
-        if (sl < 0 || sl >= n_slots) continue;
+        if (j0 + q >= end || sl < 0 || sl >= n_slots) continue;
         const T sgn = (e & 1) ? T(-1) : T(1);
         T f0, fx, fy, f3;
         load4(slots.f + 4 * sl, f0, fx, fy, f3);
@@ -885,6 +896,7 @@ void k_flux(ArenaPtrs<T> a) {
           if (sa) sa_spec += xa;
         } else if (sa) {
           sa_net += sgn * slots.sf[sl];
+        }
         }
       }
     }
