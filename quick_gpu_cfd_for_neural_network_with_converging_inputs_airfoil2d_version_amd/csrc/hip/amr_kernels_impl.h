@@ -400,12 +400,40 @@ __device__ __forceinline__ void first_stage(const ArenaPtrs<T>& a, const MeshDes
   const double* u = a.U[cur];
   for (int v = 0; v < 5; ++v) q[v] = 0.0;
   if (stage_deltas(d)) {
-    for (int v = 0; v < d.nv; ++v)
-      q[v] = u[v * nc + k] + static_cast<double>(static_cast<T>(dt * static_cast<double>(a.R0[v * nc + k])));
+    // Every load of the cell first (the fifth variable only with SA), then the sums: the loads of
+    // a loop over the run-time variable count would each wait for the one before.
+    double uv[5] = {0, 0, 0, 0, 0};
+    T rv[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      uv[v] = u[v * nc + k];
+      rv[v] = a.R0[v * nc + k];
+    }
+    if (d.nv == 5) {
+      uv[4] = u[4 * nc + k];
+      rv[4] = a.R0[4 * nc + k];
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v)
+      if (v < d.nv) q[v] = uv[v] + static_cast<double>(static_cast<T>(dt * static_cast<double>(rv[v])));
   } else {
-    for (int v = 0; v < d.nv; ++v) {
-      q[v] = u[v * nc + k] + dt * static_cast<double>(a.R0[v * nc + k]);
-      if (store) a.U[cur ^ 1][v * nc + k] = q[v];
+    double uv[5] = {0, 0, 0, 0, 0};
+    T rv[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      uv[v] = u[v * nc + k];
+      rv[v] = a.R0[v * nc + k];
+    }
+    if (d.nv == 5) {
+      uv[4] = u[4 * nc + k];
+      rv[4] = a.R0[4 * nc + k];
+    }
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      if (v < d.nv) {
+        q[v] = uv[v] + dt * static_cast<double>(rv[v]);
+        if (store) a.U[cur ^ 1][v * nc + k] = q[v];
+      }
     }
   }
 }
@@ -979,16 +1007,29 @@ void k_flux(ArenaPtrs<T> a) {
       const double* u = a.U[cur];
       const T rv[5] = {r0, r1, r2, r3, r4};
       double q[5] = {0, 0, 0, 0, 0};
-      for (int v = 0; v < d.nv; ++v) {
+      // Every load of the cell first (the fifth variable only with SA), then the sums and stores:
+      // a loop over the run-time variable count that stores as it goes waits for each load.
+      T pv[5] = {0, 0, 0, 0, 0};
+      double uv[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        if (v == 4 && d.nv != 5) break;
         const int i = v * nc + k;
-        const T prev = STAGE == 1 ? static_cast<T>(dt * static_cast<double>(ld_last(a.R0 + i))) : ld_last(a.DU + i);
+        pv[v] = STAGE == 1 ? ld_last(a.R0 + i) : ld_last(a.DU + i);
+        uv[v] = STAGE == 1 ? u[i] : ld_last(u + i);
+      }
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        if (v == 4 && d.nv != 5) break;
+        const int i = v * nc + k;
+        const T prev = STAGE == 1 ? static_cast<T>(dt * static_cast<double>(pv[v])) : pv[v];
         const double inc = static_cast<double>(prev) + dt * static_cast<double>(rv[v]);
         if (STAGE == 1) {
           const T dn = static_cast<T>(0.25 * inc);
           a.DU[i] = dn;
-          q[v] = u[i] + static_cast<double>(dn);
+          q[v] = uv[v] + static_cast<double>(dn);
         } else {
-          q[v] = ld_last(u + i) + (2.0 / 3.0) * inc;
+          q[v] = uv[v] + (2.0 / 3.0) * inc;
           a.U[cur ^ 1][i] = q[v];
         }
       }
@@ -1003,14 +1044,29 @@ void k_flux(ArenaPtrs<T> a) {
       double* out = STAGE == 1 ? a.U2 : a.U[cur ^ 1];
       const double* prev = STAGE == 1 ? a.U[cur ^ 1] : a.U2;
       const double wu = STAGE == 1 ? 0.75 : 1.0 / 3.0, wp = STAGE == 1 ? 0.25 : 2.0 / 3.0;
-      out[k] = wu * u[k] + wp * (prev[k] + dt * static_cast<double>(r0));
-      out[nc + k] = wu * u[nc + k] + wp * (prev[nc + k] + dt * static_cast<double>(r1));
-      out[2 * nc + k] = wu * u[2 * nc + k] + wp * (prev[2 * nc + k] + dt * static_cast<double>(r2));
-      out[3 * nc + k] = wu * u[3 * nc + k] + wp * (prev[3 * nc + k] + dt * static_cast<double>(r3));
-      if (sa) out[4 * nc + k] = wu * u[4 * nc + k] + wp * (prev[4 * nc + k] + dt * static_cast<double>(r4));
+      // Every load first, then the combinations and stores, then the primitives from registers
+      // (a store before the next variable's loads makes each load wait for the one before).
+      double uv[5] = {0, 0, 0, 0, 0}, pv[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        uv[v] = u[v * nc + k];
+        pv[v] = prev[v * nc + k];
+      }
+      if (sa) {
+        uv[4] = u[4 * nc + k];
+        pv[4] = prev[4 * nc + k];
+      }
+      const T rv[5] = {r0, r1, r2, r3, r4};
+      double q[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+      for (int v = 0; v < 5; ++v) {
+        if (v == 4 && !sa) break;
+        q[v] = wu * uv[v] + wp * (pv[v] + dt * static_cast<double>(rv[v]));
+        out[v * nc + k] = q[v];
+      }
       if (!d.small) {
         CellPrim<T> w;
-        if (!make_prim(out, nc, k, d, w)) bad |= BAD_STAGE;
+        if (!make_prim_q(q, d, w)) bad |= BAD_STAGE;
         else store_prim(a, write_buffer(c, STAGE), k, w);
       }
     }
