@@ -937,7 +937,15 @@ void k_flux(ArenaPtrs<T> a) {
     // face loop (fewer live registers there).
     const CellPrim<T> own = tile_prim(lds, tid);
     const T rho = own.rho, p = own.p;
+    // The cell's own words, loaded together before the wall loop (each would otherwise wait for
+    // the loads above it): wall range, inverse volume, SA point source; base pass: dt area,
+    // volume, SA source rate.
     const int w0 = a.wall_first[k], w1 = w0 + a.wall_cnt[k];
+    const T iv = a.inv_vol[k];
+    const T sa_point_source = sa ? a.GREC[8 * k + 7] : T(0);
+    const T dt_area = BASE ? a.dt_area[k] : T(0);
+    const T cell_vol = BASE && (laminar || sa) ? a.vol[k] : T(0);
+    const T src_rate = BASE && sa ? a.SRATE[k] : T(0);
     if (w1 > w0) {
       const T snd = sqrt(d.gamma * p / rho);
       for (int j = w0; j < w1; ++j) {
@@ -965,11 +973,10 @@ void k_flux(ArenaPtrs<T> a) {
       }
     }
     if (!ok) bad |= bit;
-    const T iv = a.inv_vol[k];
     const T r0 = -s0 * iv, r1 = -s1 * iv, r2 = -s2 * iv, r3 = -s3 * iv;
     T r4 = T(0);
     if (sa) {
-      r4 = -sa_net * iv + a.GREC[8 * k + 7];
+      r4 = -sa_net * iv + sa_point_source;
       if (!isfinite(r4)) bad |= bit;
     }
     if (BASE) {
@@ -978,10 +985,10 @@ void k_flux(ArenaPtrs<T> a) {
       a.R0[2 * nc + k] = r2;
       a.R0[3 * nc + k] = r3;
       if (sa) a.R0[4 * nc + k] = r4;
-      T dt = spectral > T(0) ? d.cfl * a.dt_area[k] / spectral : T(INFINITY);
+      T dt = spectral > T(0) ? d.cfl * dt_area / spectral : T(INFINITY);
       if (laminar) {
         if (d.visc_mode == VISC_VOLUME) {
-          if (diffusion > T(0)) dt = fmin(dt, d.transport.viscous_cfl * a.vol[k] / diffusion);
+          if (diffusion > T(0)) dt = fmin(dt, d.transport.viscous_cfl * cell_vol / diffusion);
         } else if (d.visc_mode == VISC_SRD_RATE) {
           const T rate = diffusion * iv;
           if (rate > T(0)) dt = fmin(dt, d.transport.viscous_cfl / rate);
@@ -990,7 +997,7 @@ void k_flux(ArenaPtrs<T> a) {
         }
       }
       if (sa) {
-        const T rate = T(2) * sa_spec / (rho * a.vol[k]) + a.SRATE[k];
+        const T rate = T(2) * sa_spec / (rho * cell_vol) + src_rate;
         if (rate > T(0)) dt = fmin(dt, d.transport.sa_cfl / rate);
       }
       // SRD-grouped viscous limit: the cell's convective (and SA) limit waits for the group
