@@ -460,8 +460,13 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
   const bool sa = d.sa != 0;
   T* const vel = reinterpret_cast<T*>(tile_lds);  // the tile's own velocity vectors (4 per cell)
   CellPrim<T> own{};
+  int begin = 0, end = 0;  // the cell's terms, loaded with its record (not after the barrier)
+  T wall_distance = T(1);
   if (k < d.cell_begin + d.n_cells) {
     own = load_prim(a, rb, k);
+    begin = a.term_off[k];
+    end = a.term_off[k + 1];
+    if (sa) wall_distance = a.wall_dist[k];
     store4(vel + 4 * threadIdx.x, own.rho, own.u, own.v, own.phi);
   }
   __syncthreads();
@@ -473,7 +478,6 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
   // two memory round trips per batch instead of two per term.  The sums run term by term in the
   // stored order, as one term at a time would.
   constexpr int kTerms = 4;
-  const int begin = a.term_off[k], end = a.term_off[k + 1];
   for (int j0 = begin; j0 < end; j0 += kTerms) {
     int src[kTerms];
     T wx[kTerms], wy[kTerms];
@@ -532,7 +536,7 @@ __global__ __launch_bounds__(kMaxTile) void k_grad(ArenaPtrs<T> a) {
   CellGrad<T> g{g0, g1, g2, g3, sx, sy, T(0)};
   T sa_source = T(0);
   if (sa) {
-    const SaPoint<T> pt = sa_point(phi, own.mu / rho, a.wall_dist[k], fabs(g2 - g1));
+    const SaPoint<T> pt = sa_point(phi, own.mu / rho, wall_distance, fabs(g2 - g1));
     const T source = rho * (pt.source + T(SaConst<T>::cb2 / SaConst<T>::sigma) * (sx * sx + sy * sy)) -
                      pt.diffusion * (rx * sx + ry * sy);
     g.sdiff = pt.diffusion;
@@ -830,7 +834,17 @@ void k_flux(ArenaPtrs<T> a) {
   const FaceSlots<T> slots(tn, BASE);
   __shared__ int failed;  // block snapshot of the slot's admissibility bits (or the trial ended)
   if (tid == 0) failed = live ? c->bad : 1;
-  if (valid && live) stage_cell(lds, tid, load_prim(a, rb, k));
+  // Stage passes load the cell's face-entry range with its record instead of after the barrier
+  // (stage flux -2.5 to -3 us on 256 LF2 meshes; the base pass, held to 80 VGPRs, spilled more
+  // and lost 4.5 us, and loads it where it is used).
+  int csr_begin = 0, csr_end = 0;
+  if (valid && live) {
+    stage_cell(lds, tid, load_prim(a, rb, k));
+    if (!BASE) {
+      csr_begin = a.csr_off[k];
+      csr_end = a.csr_off[k + 1];
+    }
+  }
   __syncthreads();
   int bad = 0;  // admissibility bits this thread raises
   double local_dt = INFINITY;
@@ -897,7 +911,7 @@ void k_flux(ArenaPtrs<T> a) {
       // (stage flux 157 / 167 -> 155 / 160 us on 256 LF2 meshes; the base pass, 2 us slower with
       // batches of four or two, keeps one entry at a time).
       const int lo = c0 - range.x, n_slots = c1 - c0;
-      const int begin = a.csr_off[k], end = a.csr_off[k + 1];
+      const int begin = BASE ? a.csr_off[k] : csr_begin, end = BASE ? a.csr_off[k + 1] : csr_end;
       constexpr int kEntries = BASE ? 1 : 4;
       for (int j0 = begin; j0 < end; j0 += kEntries) {
         int ent[kEntries];
