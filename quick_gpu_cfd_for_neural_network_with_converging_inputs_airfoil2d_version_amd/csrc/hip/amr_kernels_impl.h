@@ -795,16 +795,17 @@ __device__ __forceinline__ void arrive_and_commit(Control* c, int n_blocks, bool
 // ------------------------------------------------------------------------------------------
 // Residual of every cell from its faces and wall fragments.  STAGE 0: base residual R0 and the
 // local dt.  1, 2: second and third SSP-RK3 combinations, then the candidate's primitives.
-// FP32 register budget: 6 waves per SIMD in the base pass (80 VGPRs, 2 spilled) and 7 in the
-// stage passes (72 VGPRs, 6 spilled); the LDS of a 128-cell tile admits 11 and 14 blocks
-// of two waves per CU.
+// FP32 register budget: 5 waves per SIMD in the base pass (89 VGPRs, no spill; its LDS admits
+// 11 blocks of two waves per CU, 5.5 waves per SIMD, so a 6-wave bound, which spilled, bought no
+// occupancy: base flux 151 -> 141 us with the batched entries below) and 7 in the stage passes
+// (72 VGPRs, 4 and 6 spilled; LDS for 14 blocks per CU).
 // On 256 LF2 meshes: base flux 187 -> 177 us, stage flux 188 -> 181 us against the unbounded
 // kernels; 8 waves (64 VGPRs) spills 11-14 and is slower.  FP64 (190-200 VGPRs) has no bound.
 // SPREAD threads per cell (1 or 2): a tile's face items are spread over SPREAD * tile threads,
 // the cell work stays on the first tile of them (a separate instantiation, so that the one-thread
 // kernel keeps its registers).
 template <class T, int STAGE, int SPREAD>
-__global__ __launch_bounds__(kMaxTile) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (STAGE == 0 ? 6 : 7) : 1)))
+__global__ __launch_bounds__(kMaxTile) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? (STAGE == 0 ? 5 : 7) : 1)))
 void k_flux(ArenaPtrs<T> a) {
   constexpr bool BASE = STAGE == 0;
   const int2 blk = cell_block(a);
@@ -834,16 +835,13 @@ void k_flux(ArenaPtrs<T> a) {
   const FaceSlots<T> slots(tn, BASE);
   __shared__ int failed;  // block snapshot of the slot's admissibility bits (or the trial ended)
   if (tid == 0) failed = live ? c->bad : 1;
-  // Stage passes load the cell's face-entry range with its record instead of after the barrier
-  // (stage flux -2.5 to -3 us on 256 LF2 meshes; the base pass, held to 80 VGPRs, spilled more
-  // and lost 4.5 us, and loads it where it is used).
+  // The cell's face-entry range is loaded with its record instead of after the barrier (stage
+  // flux -2.5 to -3 us on 256 LF2 meshes).
   int csr_begin = 0, csr_end = 0;
   if (valid && live) {
     stage_cell(lds, tid, load_prim(a, rb, k));
-    if (!BASE) {
-      csr_begin = a.csr_off[k];
-      csr_end = a.csr_off[k + 1];
-    }
+    csr_begin = a.csr_off[k];
+    csr_end = a.csr_off[k + 1];
   }
   __syncthreads();
   int bad = 0;  // admissibility bits this thread raises
@@ -905,14 +903,14 @@ void k_flux(ArenaPtrs<T> a) {
     }
     __syncthreads();
     if (valid) {
-      // Stage passes take the cell's slot entries four at a time: the entries of a batch are loaded
-      // together (entries past the end are loaded in range and skipped), then summed in cell-face
-      // order, so the cell waits for one memory round trip per batch instead of one per face
-      // (stage flux 157 / 167 -> 155 / 160 us on 256 LF2 meshes; the base pass, 2 us slower with
-      // batches of four or two, keeps one entry at a time).
+      // The cell's slot entries four at a time: the entries of a batch are loaded together
+      // (entries past the end are loaded in range and skipped), then summed in cell-face order,
+      // so the cell waits for one memory round trip per batch instead of one per face (stage flux
+      // 157 / 167 -> 155 / 160 us on 256 LF2 meshes; base flux 151 -> 141 us once its register
+      // bound no longer made it spill).
       const int lo = c0 - range.x, n_slots = c1 - c0;
-      const int begin = BASE ? a.csr_off[k] : csr_begin, end = BASE ? a.csr_off[k + 1] : csr_end;
-      constexpr int kEntries = BASE ? 1 : 4;
+      const int begin = csr_begin, end = csr_end;
+      constexpr int kEntries = 4;
       for (int j0 = begin; j0 < end; j0 += kEntries) {
         int ent[kEntries];
 #pragma unroll
